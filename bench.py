@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the optimal-perturb hot path on one MI355X.
+
+One step = one twisty_hip_run over the whole workload: N=64 segments, arclength
+1.5 between (0,0,0) and (0,0,1) with both end tangents +z, circular-arc initial
+curve, K=100 perturbations per path, 2^22 paths, one 4096-bin radiative-transfer
+table (Gaussian phase sigma=0.05 -> eps=sigma^2, mu 1.0, scatter 0.1,
+absorption 0.004, numStepsInt 20000), Philox seed 1.  Inputs are built on the
+host and uploaded before timing (table construction and H2D are excluded).
+
+Prints one JSON line.  `value` is joint-weights/s = paths * (N-1) / wall
+seconds per step; `roofline` prices the forward kernel's HIP-event time against
+the FP32 vector peak with the algorithmic flop count per path
+F = K (40 + 18 r) + 30 N - 14, r = E[diff] - 1; `cpu_baseline` times the
+reference CPU path restated in oracle/ (FP32 positions, FP64 weights, same
+draws) on the host cores over a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "joint-weights/s at N=64, 2^22 paths; fwd+adjoint ms; batched vs plain"
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, AMD published
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = {"N": 64, "L": 1.5, "K": 100, "paths": 1 << 22, "sigma": 0.05, "bins": 4096, "mu": 1.0,
+            "scatter": 0.1, "absorption": 0.004, "numStepsInt": 20000, "seed": 1}
+
+
+def mean_rotated_points(N):
+    """E[diff] - 1 for diff ~ U{2..min(N-2,25)}."""
+    md = min(N - 2, 25)
+    return (2 + md) / 2.0 - 1.0
+
+
+def flops_per_path(N, K):
+    return K * (40 + 18 * mean_rotated_points(N)) + 30 * N - 14
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/), or None."""
+    p = os.path.join(ROOT, "profiles", "forward_c2_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(table, pos, w, target_seconds):
+    """Reference CPU path (oracle/, FP32 positions + FP64 weights) on the host."""
+    from oracle import pyoracle
+
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    threads = os.cpu_count() or 1
+
+    class T:
+        pass
+
+    t = T()
+    t.table, t.ds, t.minK, t.maxK, t.step = table.table, table.ds, table.minK, table.maxK, table.step
+
+    def run(n, th):
+        r = pyoracle.simulate(pos.astype("float64"), w["N"], w["L"], w["K"], n, t, w["absorption"], seed=w["seed"],
+                              precision=32, threads=th, want_w=False)
+        return r["seconds"]
+
+    n0 = 1 << 12
+    s0 = run(n0, threads)
+    n = n0
+    while n < w["paths"] and s0 * (n / n0) * 2 <= target_seconds * 1.5:
+        n *= 2
+    secs = run(n, threads) if n != n0 else s0
+    jw = n * (w["N"] - 1) / secs
+    out = {"value": jw, "unit": "joint-weights/s", "cores": threads, "kind": "port",
+           "sample": f"{n} paths of the same workload (paths 0..{n - 1}), reference CPU path restated in oracle/ "
+                     f"(FP32 positions, FP64 weights, same Philox draws), {threads} std::threads, {secs:.1f} s"}
+    # The reference's own split: 1e6-path granules per thread, so at 2^22 paths
+    # only ceil(2^22/1e6) = 5 threads work (FullExperimentRunnerOptimalPerturb.cpp:255-277).
+    ref_threads = min(threads, math.ceil(w["paths"] / 1_000_000))
+    n_ref = max(n0, n * ref_threads // threads)
+    secs_ref = run(n_ref, ref_threads)
+    ref_part = {"value": n_ref * (w["N"] - 1) / secs_ref, "unit": "joint-weights/s", "cores": ref_threads,
+                "kind": "port", "sample": f"{n_ref} paths on {ref_threads} threads ({secs_ref:.1f} s)"}
+    return out, ref_part
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--paths", type=int, default=WORKLOAD["paths"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              f"(--gpus {a.gpus} requested); rerun with --gpus 1", file=sys.stderr)
+        return 2
+    import numpy as np
+
+    from twisty_amd import capi, host
+
+    w = dict(WORKLOAD)
+    w["paths"] = a.paths
+    N, L, K = w["N"], w["L"], w["K"]
+    ds = float(np.float32(L) / np.float32(N))  # Curve.cpp:24 computes ds in float
+    eps = float(np.float32(w["sigma"]) ** 2)
+    table = host.build_rt_table(ds, w["mu"], eps, w["scatter"], w["absorption"], numStepsInt=w["numStepsInt"],
+                                numCurvatureSteps=w["bins"])
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
+    pos = host.initial_arc(bc, N)
+    ctx = capi.Context(0)
+    ctx.set_tables(table.table, table.minK, table.maxK, table.step, table.ds, w["absorption"])
+    ctx.set_curve(pos, bc)
+    sched = ctx.get_schedule()
+    for _ in range(a.warmup):
+        ctx.run(w["paths"], K, seed=w["seed"])
+    fwd = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = ctx.run(w["paths"], K, seed=w["seed"])
+        fwd.append(ctx.last_timing()[0])
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    value = w["paths"] * (N - 1) / (ms_per_step / 1e3)
+    fwd_ms = statistics.fmean(fwd)
+    F = flops_per_path(N, K)
+    achieved = F * w["paths"] / (fwd_ms / 1e3) / 1e12
+    mean_l, rel_se = capi.estimator(res)
+    out = {
+        "metric": METRIC, "value": value, "unit": "joint-weights/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "C2 plain forward: N=64 segments, arclength 1.5, K=100 perturbations/path, "
+                               "2^22 paths, 4096-bin RT table sigma=0.05", "N": N, "arclength": L,
+                   "perturbations_per_path": K, "paths": w["paths"], "table_bins": w["bins"], "sigma": w["sigma"],
+                   "mu": w["mu"], "scatter": w["scatter"], "absorption": w["absorption"], "seed": w["seed"],
+                   "schedule": sched},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(),
+                     "kernel": "forward_kernel", "kernel_ms": fwd_ms, "flops_per_path": F},
+        "estimate": {"log10_avg_path_weight": mean_l, "rel_standard_error": rel_se,
+                     "num_valid": res["num_valid"], "num_invalid": res["num_invalid"]},
+        "fwd_adjoint_ms": None,
+        "batched": None,
+    }
+    if not a.no_cpu_baseline:
+        cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["cpu_baseline_reference_partitioning"] = refpart
+        out["speedup_vs_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,125 @@
+/*
+ * twisty_hip.h -- C ABI of libtwisty_hip.so, the gfx950 implementation of
+ * Twisty's optimal-perturb path-integral hot path.
+ *
+ * Conventions: every function returns 0 (TWISTY_HIP_OK) or an error code; the
+ * message of the last error of a context is twisty_hip_last_error(ctx).  No
+ * exceptions cross this boundary.  All array arguments are HOST pointers and
+ * every call is synchronous.  One host thread per context; contexts are
+ * independent and each owns its device memory and HIP stream.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the Twisty repository root).
+ */
+#ifndef TWISTY_HIP_H
+#define TWISTY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWISTY_HIP_OK 0
+#define TWISTY_HIP_ERR_ARG 1         /* invalid argument (nothing was launched) */
+#define TWISTY_HIP_ERR_HIP 2         /* a HIP runtime call failed */
+#define TWISTY_HIP_ERR_STATE 3       /* call order: tables / curve not set */
+#define TWISTY_HIP_ERR_UNSUPPORTED 4 /* shape outside what the kernels support */
+
+typedef struct twisty_hip_ctx twisty_hip_ctx;
+
+/* Result of one run.  Replaces the per-1e6-path CombinedWeightValues_C array
+ * and its Boost big-float extraction (CombinedWeightUtils.h:28-174,
+ * FullExperimentRunnerOptimalPerturb.cpp:342-378): the estimator numerator is
+ * returned as a base-10 logarithm computed in FP64 from per-block partials. */
+typedef struct twisty_hip_partials {
+    double log10_sum_w;   /* log10 of sum over valid paths of 10^{w_p}; -inf if none */
+    double log10_sum_w2;  /* log10 of sum over valid paths of 10^{2 w_p} */
+    double max_log10_w;   /* max over valid paths of w_p */
+    uint64_t num_paths;   /* paths generated (valid + invalid) */
+    uint64_t num_valid;
+    uint64_t num_invalid; /* a segment weight was NaN: PathWeighters.cpp:89-102 */
+    uint64_t num_clamped; /* joints clamped into the table range: PathWeighters.cpp:53-65 */
+} twisty_hip_partials;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int twisty_hip_version(void);
+
+/* Number of visible HIP devices.  Replaces SetupCudaDevice's device query
+ * (FullExperimentRunnerOptimalPerturbOptimized_GPU.cu:395-457). */
+int twisty_hip_device_count(int *count);
+
+/* Creates a context on `device`: stream, events and the per-block partials
+ * buffer.  Replaces SetupCudaDevice + the cudaMalloc half of SetupCudaPerturb
+ * (FullExperimentRunnerOptimalPerturbOptimized_GPU.cu:395-457, 527-658). */
+int twisty_hip_create(int device, twisty_hip_ctx **out);
+
+/* Frees everything the context owns.  Replaces CleanupCudaPerturb /
+ * CleanupCudaRandStates / CleanupCudaDevice (...Optimized_GPU.cu:370-376,
+ * 660-676). */
+int twisty_hip_destroy(twisty_hip_ctx *ctx);
+
+/* Message of the last failed call on ctx ("" when none).  Never NULL. */
+const char *twisty_hip_last_error(const twisty_hip_ctx *ctx);
+
+/* Uploads the curvature -> segment-weight table(s).  The argument list is the
+ * table half of WeightCurveViaCurvatureLog10_CudaSafe (PathWeighters.h:25-28,
+ * PathWeighters.cpp:27-104) and of the kernel arguments
+ * (...Optimized_GPU.cu:96-102), i.e. BaseWeightLookupTable's
+ * AccessLookupTable / GetMinCurvature / GetMaxCurvature /
+ * GetCurvatureStepSize / GetDs (PathWeightUtils.h:129-139) plus the
+ * absorption coefficient.  `tables` holds numTables tables of numBins floats
+ * each, sharing one curvature grid; numTables > 1 combines them by summing
+ * the interpolated segment weights.  The host buffer is copied; the caller
+ * keeps it.  Tables must not contain +-inf. */
+int twisty_hip_set_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
+      float minCurvature, float maxCurvature, float curvatureStepSize, float ds, float absorption);
+
+/* Uploads the initial curve: (numSegments+1) points as x,y,z floats (the
+ * reference's Farlor::Vector3 positions, Curve.h:155-157) and the boundary
+ * conditions in BoundaryConditions_CudaSafe layout (CurvePerturbUtils.h:70-76):
+ * startPos[3], startDir[3], endPos[3], endDir[3], arclength.  Replaces the
+ * per-thread curve replication of SetupCudaPerturb (...Optimized_GPU.cu:610-644)
+ * with one copy; the segment vectors p_{i+1} - p_i are formed on the host in
+ * FP64 and stored as floats. */
+int twisty_hip_set_curve(twisty_hip_ctx *ctx, const float *positions, uint32_t numSegments,
+      const float boundary[13]);
+
+/* Runs paths [pathOffset, pathOffset + numPaths).  Each path applies
+ * perturbationsPerPath endpoint-preserving sub-chain rotations, then weights
+ * its N-1 joints (the loop body of FullExperimentRunnerOptimalPerturb.cpp:469-593
+ * and ...Optimized_GPU.cu:735-888).  chainLength = 1 starts every path from the
+ * initial curve; chainLength = L > 1 carries the state through L consecutive
+ * path ids as the reference's per-thread Markov chain does: chains start at the
+ * path ids that are multiples of L (pathOffset must be one), and the last chain
+ * of a range may be shorter.  Draws: Philox4x32-10, key = seed,
+ * counter = (perturbation, 0, path).
+ * perPathLog10 (numPaths floats) and perJointCurvature (numPaths*(N-1)
+ * floats) are optional host outputs (NULL to skip); an invalid path reports
+ * NaN.  Replaces the kernel launch, cudaDeviceSynchronize and the D2H copy of
+ * the accumulators (...Optimized_GPU.cu:268-340). */
+int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint32_t perturbationsPerPath,
+      uint32_t chainLength, uint64_t seed, twisty_hip_partials *out, float *perPathLog10,
+      float *perJointCurvature);
+
+/* Launch schedule of twisty_hip_run: lanesPerPath in {1,2,4,8,16,32,64},
+ * blockSize a multiple of 64, blocksPerCU >= 1; 0 for any value selects the
+ * built-in choice for the current curve.  Replaces the compile-time
+ * PerturbGridSize / PerturbBlockSize (...Optimized_GPU.cu:45-47). */
+int twisty_hip_set_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
+      uint32_t blocksPerCU);
+
+/* The schedule the next run will use for the current curve (after resolving
+ * zeros) and the dynamic LDS bytes per block it needs. */
+int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_t *blockSize,
+      uint32_t *blocksPerCU, uint32_t *ldsBytes);
+
+/* Device time of the last run measured with HIP events on the context's
+ * stream: the forward kernel alone and forward + FP64 finalize. */
+int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TWISTY_HIP_H */

@@ -1,0 +1,60 @@
+/*
+ * twisty_host.h -- C ABI of libtwisty_host.so, the C++ host layer (weight
+ * tables, initial curve, ExperimentRunner) in front of libtwisty_hip.so.
+ *
+ * These entry points let non-C++ callers (the Python tooling, ctypes) reach
+ * the host layer.  Same conventions as twisty_hip.h: status codes, host
+ * pointers, synchronous.
+ */
+#ifndef TWISTY_HOST_H
+#define TWISTY_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ExperimentRunner::ExperimentResults (ExperimentRunner.h:73-82) with the
+ * experiment weight as log10 (the reference holds a Boost cpp_dec_float_100). */
+typedef struct twisty_host_result {
+    double log10_experiment_weight; /* log10 of sum_valid 10^{w_p} * path normaliser */
+    double log10_avg_path_weight;   /* log10 of experimentWeight / totalPathsGenerated
+                                       (Experiments/FullExperiment.cpp:244) */
+    double log10_sum_squares;       /* log10 of sum_valid 10^{2 w_p} * normaliser^2 */
+    uint64_t total_paths;
+    uint64_t failed_paths;
+    uint64_t clamped_joints;
+    double setup_ms, perturb_ms, weighting_ms, total_ms, kernel_ms;
+} twisty_host_result;
+
+/* WeightLookupTableIntegral(params, ds).Initialize() (PathWeightUtils.cpp:
+ * 281-348): numCurvatureSteps floats into tableOut; metaOut receives
+ * [ds, minCurvature, maxCurvature, curvatureStepSize, minSegmentWeight,
+ * maxSegmentWeight].  threads <= 0 uses every hardware thread. */
+int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorption, float bias,
+      uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound, float maxBound, float ds, int threads,
+      float *tableOut, float *metaOut);
+
+/* Initial curve for BoundaryConditions_CudaSafe `boundary` (13 floats): hinges
+ * as Curve::SetBoundaryConditions (Curve.cpp:20-30), circular arc between
+ * them; (numSegments+1)*3 floats.  Returns 1 when no arc exists. */
+int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut);
+
+/* PathGeneration::CalculateMinimumArclength (PathGeneration.cpp:12-43). */
+float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegments);
+
+/* log10 of NormalizerStuff::Norm (PathWeightUtils.cpp:721-752). */
+double twisty_host_norm_log10(uint32_t numSegments, float ds, const float boundary[13]);
+
+/* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
+ * config given as text: parse experiment.experimentParams and
+ * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.
+ * On failure returns nonzero and writes a message into errorOut. */
+int twisty_host_run_config(const char *jsonText, twisty_host_result *out, char *errorOut, uint32_t errorLen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TWISTY_HOST_H */

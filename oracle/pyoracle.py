@@ -1,0 +1,165 @@
+"""ctypes binding of oracle/_build/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker; never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _p(dt):
+    return np.ctypeslib.ndpointer(dtype=dt, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    u32, u64, f32, f64, i32 = C.c_uint32, C.c_uint64, C.c_float, C.c_double, C.c_int32
+    L.oracle_philox4x32_10.argtypes = [_p(np.uint32), _p(np.uint32), _p(np.uint32)]
+    L.oracle_draw.argtypes = [u64, u32, u64, u32, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(f32)]
+    L.oracle_rotate_vector_by_quaternion.argtypes = [_p(np.float64), _p(np.float64)]
+    L.oracle_rotate_quat_by_quat.argtypes = [_p(np.float64), _p(np.float64), _p(np.float64)]
+    L.oracle_apply_perturbation_f64.argtypes = [_p(np.float64), u32, u32, f32]
+    L.oracle_tangents_curvatures_f64.argtypes = [_p(np.float64), u32, f32, _p(np.float64), _p(np.float64)]
+    L.oracle_inv_ds.argtypes = [u32, f32]
+    L.oracle_inv_ds.restype = f32
+    L.oracle_initial_arc.argtypes = [_p(np.float64), f32, u32, _p(np.float64), C.POINTER(f64)]
+    L.oracle_initial_arc.restype = C.c_int
+    L.oracle_minimum_arclength.argtypes = [_p(np.float64), u32]
+    L.oracle_minimum_arclength.restype = f32
+    L.oracle_norm.argtypes = [u32, f32, _p(np.float64), f32, u32, u32]
+    L.oracle_norm.restype = f64
+    L.oracle_gaussian_phase.argtypes = [f32, f32]
+    L.oracle_gaussian_phase.restype = f32
+    L.oracle_integrand_rt.argtypes = [f64, f64, f64, f64, f64]
+    L.oracle_integrand_rt.restype = f64
+    L.oracle_table_range.argtypes = [u32, f32, C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)]
+    L.oracle_build_table.argtypes = [f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, C.c_int,
+                                     _p(np.float32), _p(np.float32)]
+    L.oracle_build_table.restype = C.c_int
+    L.oracle_integrate.argtypes = [f32, f32, f32, f32, f32, u32, f32, f32, f32]
+    L.oracle_integrate.restype = f32
+    L.oracle_weight_curvatures.argtypes = [_p(np.float32), u32, _p(np.float32), i32, f32, f32, f32, f32, f32,
+                                           C.POINTER(f64)]
+    L.oracle_weight_curvatures.restype = C.c_int
+    L.oracle_accumulate_log10.argtypes = [_p(np.float64), u64]
+    L.oracle_accumulate_log10.restype = f64
+    L.oracle_accumulate_combine_log10.argtypes = [_p(np.float64), u64, u64]
+    L.oracle_accumulate_combine_log10.restype = f64
+    L.oracle_simulate.argtypes = [C.c_int, _p(np.float64), u32, f32, u32, u32, u64, u32, u64, u64,
+                                  _p(np.float32), i32, f32, f32, f32, f32, f32, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _p(np.float64)]
+    L.oracle_simulate.restype = C.c_int
+    L.oracle_exit_bin.argtypes = [_p(np.float64), _p(np.float64), u32, u32, C.POINTER(u32)]
+    L.oracle_end_frame.argtypes = [_p(np.float64), _p(np.float64), _p(np.float64)]
+
+
+# ---------------------------------------------------------------------------
+# Thin Python helpers
+# ---------------------------------------------------------------------------
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def draw(seed, stream, path, k, N):
+    d, l, x = C.c_uint32(), C.c_uint32(), C.c_float()
+    lib().oracle_draw(seed, stream, path, k, N, C.byref(d), C.byref(l), C.byref(x))
+    return d.value, l.value, x.value
+
+
+def bc12(start=(0, 0, 0), sdir=(0, 0, 1), end=(0, 0, 1), edir=(0, 0, 1)):
+    return np.array(list(start) + list(sdir) + list(end) + list(edir), np.float64)
+
+
+def initial_arc(N, arclength, bc=None):
+    bc = bc12() if bc is None else np.asarray(bc, np.float64)
+    pos = np.zeros((N + 1) * 3, np.float64)
+    alpha = C.c_double()
+    rc = lib().oracle_initial_arc(bc, arclength, N, pos, C.byref(alpha))
+    if rc != 0:
+        raise ValueError("infeasible initial arc")
+    return pos.reshape(N + 1, 3), alpha.value
+
+
+def table_range(nbins, ds):
+    a, b, c = C.c_float(), C.c_float(), C.c_float()
+    lib().oracle_table_range(nbins, ds, C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+class Table:
+    def __init__(self, table, meta):
+        self.table = table
+        self.ds, self.minK, self.maxK, self.step, self.minW, self.maxW = [float(v) for v in meta]
+
+
+def build_table(ds, mu=1.0, eps=0.0025, scatter=0.1, absorption=0.004, bias=1.0, numStepsInt=20000,
+                numCurvatureSteps=4096, minBound=0.0, maxBound=None, threads=None):
+    if maxBound is None:
+        maxBound = np.float32(10.0) / np.float32(eps)  # ExperimentRunner.cpp:168-171
+    threads = threads or os.cpu_count() or 1
+    t = np.zeros(numCurvatureSteps, np.float32)
+    meta = np.zeros(6, np.float32)
+    rc = lib().oracle_build_table(mu, eps, scatter, absorption, bias, numStepsInt, numCurvatureSteps,
+                                  minBound, float(maxBound), ds, threads, t, meta)
+    if rc != 0:
+        raise ValueError("bad table parameters")
+    return Table(t, meta)
+
+
+def weight_curvatures(curv, table, absorption):
+    curv = np.ascontiguousarray(curv, np.float32)
+    w = C.c_double()
+    ok = lib().oracle_weight_curvatures(curv, len(curv), table.table, len(table.table), table.ds, table.minK,
+                                        table.maxK, table.step, absorption, C.byref(w))
+    return bool(ok), w.value
+
+
+def simulate(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, stream=0, path_offset=0,
+             chain_length=1, precision=64, threads=None, want_curv=False, want_exit=False, want_w=True):
+    threads = threads or os.cpu_count() or 1
+    init = np.ascontiguousarray(init_pos, np.float64).reshape(-1)
+    res = np.zeros(8, np.float64)
+    w = np.zeros(n_paths, np.float64) if want_w else None
+    v = np.zeros(n_paths, np.uint8) if want_w else None
+    cv = np.zeros(n_paths * (N - 1), np.float32) if want_curv else None
+    ex = np.zeros(n_paths * 3, np.float32) if want_exit else None
+    ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    rc = lib().oracle_simulate(precision, init, N, arclength, K, chain_length, seed, stream, path_offset,
+                               n_paths, table.table, len(table.table), table.ds, table.minK, table.maxK,
+                               table.step, absorption, threads, ptr(w), ptr(v), ptr(cv), ptr(ex), res)
+    if rc != 0:
+        raise ValueError("oracle_simulate rejected its arguments")
+    out = {"log10_sum_w": res[0], "log10_sum_w2": res[1], "max_log10_w": res[2], "n_valid": int(res[3]),
+           "n_invalid": int(res[4]), "n_clamped": int(res[5]), "seconds": res[6], "log10_sum_w_exact": res[7]}
+    if want_w:
+        out["log10w"] = w
+        out["valid"] = v.astype(bool)
+    if want_curv:
+        out["curv"] = cv.reshape(n_paths, N - 1)
+    if want_exit:
+        out["exit"] = ex.reshape(n_paths, 3)
+    return out

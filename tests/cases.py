@@ -1,0 +1,76 @@
+"""Shared test workloads (the configurations the hot path is specified on).
+
+Weighting parameters not given by the workload use the values most of the
+reference's experiment configs use (configs/**: mu 1.0, scatter 0.1,
+absorption 0.004, numStepsInt 20000, bias 1).  A Gaussian phase width sigma is
+passed as m = sigma^2 in the config `eps` slot, the slot that reaches
+GaussianPhase through IntegrandRT's argument swap (PathWeightUtils.cpp:401).
+"""
+import functools
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class Case:
+    name: str
+    N: int
+    L: float
+    K: int
+    sigma: float
+    bins: int = 4096
+    mu: float = 1.0
+    scatter: float = 0.1
+    absorption: float = 0.004
+    numStepsInt: int = 20000
+    seed: int = 1
+
+
+C1 = Case("C1-N20", N=20, L=1.2, K=50, sigma=0.1)
+C2 = Case("C2-N64", N=64, L=1.5, K=100, sigma=0.05)
+C4 = Case("C4-N100", N=100, L=1.3, K=100, sigma=0.08)
+C5 = Case("C5-N1024", N=1024, L=2.0, K=200, sigma=0.05)
+SMALL = Case("N4", N=4, L=1.6, K=8, sigma=0.1, bins=512, numStepsInt=2000)
+MID = Case("N33", N=33, L=1.4, K=40, sigma=0.07, bins=1024, numStepsInt=2000)
+BIG = Case("N256", N=256, L=1.8, K=60, sigma=0.05, bins=2048, numStepsInt=2000)
+
+
+def ds_of(case):
+    return float(np.float32(case.L) / np.float32(case.N))
+
+
+def bc13(case):
+    """BoundaryConditions_CudaSafe layout: start (0,0,0), end (0,0,1), dirs +z."""
+    return np.array([0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 1, case.L], np.float32)
+
+
+@functools.lru_cache(maxsize=None)
+def setup(case, oracle_mod):
+    """(table, float32 initial positions (N+1,3)) for a case, built by the oracle."""
+    O = oracle_mod
+    ds = ds_of(case)
+    eps = float(np.float32(case.sigma) ** 2)
+    table = O.build_table(ds, mu=case.mu, eps=eps, scatter=case.scatter, absorption=case.absorption,
+                          numStepsInt=case.numStepsInt, numCurvatureSteps=case.bins)
+    pos, _ = O.initial_arc(case.N, case.L)
+    return table, pos.astype(np.float32)
+
+
+def hip_context(case, oracle_mod, device=0, schedule=None):
+    from twisty_amd import capi
+
+    table, pos32 = setup(case, oracle_mod)
+    ctx = capi.Context(device)
+    ctx.set_tables(table.table, table.minK, table.maxK, table.step, table.ds, case.absorption)
+    ctx.set_curve(pos32, bc13(case))
+    if schedule:
+        ctx.set_schedule(*schedule)
+    return ctx
+
+
+def oracle_run(case, oracle_mod, n_paths, path_offset=0, chain_length=1, want_curv=False, precision=64, K=None):
+    table, pos32 = setup(case, oracle_mod)
+    return oracle_mod.simulate(pos32.astype(np.float64), case.N, case.L, case.K if K is None else K, n_paths, table,
+                               case.absorption, seed=case.seed, path_offset=path_offset, chain_length=chain_length,
+                               precision=precision, want_curv=want_curv)

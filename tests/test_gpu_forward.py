@@ -1,0 +1,259 @@
+"""GPU parity of the plain forward kernel against the FP64 oracle.
+
+Tolerances (FP32 kernel vs FP64 oracle, same Philox draws, same float initial
+curve and table):
+  * per-path log10 weight: |g - r| <= 1e-5 |r| on every path whose joint
+    curvatures all fall in the same table bin as the FP64 curvatures;
+  * a path with a joint on the other side of a bin edge than FP64 meets
+    |g - r| <= 1e-5 |r| + sum of the jumps of the reference's lerp at those
+    joints.  The reference's lerp fraction is in curvature units
+    (PathWeighters.cpp:79-82), so the segment weight is discontinuous at every
+    bin edge and no finite precision reproduces FP64 across one;
+  * the lookup/log stage alone (oracle weight function evaluated on the GPU's
+    own curvatures): |g - r| <= 2e-6 |r|, which isolates geometry error from
+    weighting error;
+  * joint curvatures: max |dk| no larger than the reference's own FP32-position
+    arithmetic on the same paths (oracle precision 32), floor 1e-4: FP32 and
+    FP64 geometry drift apart as the rotation axes feed back into the state;
+  * sum of weights over the same paths: bounded by the worst per-path linear
+    error, |10^{dS} - 1| <= max_p |10^{dw_p} - 1| + 1e-6;
+  * estimates from disjoint path sets: within 3 standard errors.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.lerp import joint_log10w
+from twisty_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+REL_PATH = 1e-5
+REL_WEIGHT_STAGE = 2e-6
+KAPPA_FLOOR = 1e-4
+
+
+def both_refs(case, oracle, n, **kw):
+    """FP64 oracle and the reference's own FP32-position arithmetic, same paths."""
+    return (cases.oracle_run(case, oracle, n, want_curv=True, **kw),
+            cases.oracle_run(case, oracle, n, want_curv=True, precision=32, **kw))
+
+
+def _report(name, **kw):
+    print(f"[{name}] " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def check_against_oracle(case, oracle, gpu, ref, ref32, table, tag):
+    g, r = gpu["log10w"].astype(np.float64), ref["log10w"]
+    assert gpu["num_paths"] == len(r)
+    assert gpu["num_valid"] == ref["n_valid"] and gpu["num_invalid"] == ref["n_invalid"]
+    assert np.array_equal(np.isnan(g), ~ref["valid"])
+    # zero-weight paths (a segment weight of exactly 0) are -inf on both sides
+    assert np.array_equal(np.isneginf(g), ref["valid"] & np.isneginf(r))
+    valid = ref["valid"] & np.isfinite(r)
+    # per-joint bins and lerped log weights of both curvature sets
+    lg, bg = joint_log10w(gpu["curv"], table, case.absorption)
+    lr, br = joint_log10w(ref["curv"], table, case.absorption)
+    straddle = bg != br
+    jump = np.where(straddle, np.abs(lg - lr), 0.0).sum(axis=1)
+    clean = valid & ~straddle.any(axis=1)
+    err = np.abs(g - r)
+    scale = np.maximum(np.abs(r), 1e-30)
+    rel_clean = (err[clean] / scale[clean]).max() if clean.any() else 0.0
+    excess = (err[valid] - REL_PATH * scale[valid] - jump[valid]).max()
+    dk = np.abs(gpu["curv"].astype(np.float64) - ref["curv"].astype(np.float64)).max()
+    dk_ref32 = np.abs(ref32["curv"].astype(np.float64) - ref["curv"].astype(np.float64)).max()
+    # weighting stage on the GPU's own curvatures
+    worst_stage = 0.0
+    for i in np.flatnonzero(valid)[:512]:
+        ok, w = oracle.weight_curvatures(gpu["curv"][i], table, case.absorption)
+        assert ok
+        worst_stage = max(worst_stage, abs(g[i] - w) / abs(w))
+    gv, rv = g[valid], r[valid]
+    dS = gpu["log10_sum_w"] - ref["log10_sum_w_exact"]
+    bound = np.max(np.abs(10.0 ** (gv - rv) - 1.0)) + 1e-6
+    _report(tag, paths=len(r), straddle_paths=int((valid & ~clean).sum()), max_rel_clean=f"{rel_clean:.3e}",
+            max_rel_all=f"{(err[valid] / scale[valid]).max():.3e}", max_dkappa=f"{dk:.3e}", ref32_dkappa=f"{dk_ref32:.3e}",
+            weight_stage=f"{worst_stage:.3e}", dlog10S=f"{dS:.3e}", agg_bound=f"{bound:.3e}")
+    assert rel_clean <= REL_PATH
+    assert excess <= 0.0
+    assert dk <= max(dk_ref32, KAPPA_FLOOR)
+    assert worst_stage <= REL_WEIGHT_STAGE
+    assert abs(10.0 ** dS - 1.0) <= bound
+    return {"straddle_paths": int((valid & ~clean).sum()), "max_rel_clean": float(rel_clean)}
+
+
+def test_c2_subsample_per_path(hip_device, oracle):
+    case = cases.C2
+    table, _ = cases.setup(case, oracle)
+    n = 4096
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+        _report("schedule", **ctx.get_schedule())
+    ref, ref32 = both_refs(case, oracle, n)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C2 2^12")
+
+
+def test_c1_all_paths(hip_device, oracle):
+    case = cases.C1
+    table, _ = cases.setup(case, oracle)
+    n = 1 << 16
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+    ref, ref32 = both_refs(case, oracle, n)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C1 2^16")
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
+def test_schedules(hip_device, oracle, lanes):
+    case = cases.C2
+    table, _ = cases.setup(case, oracle)
+    n = 1024
+    with cases.hip_context(case, oracle, schedule=(lanes, 0, 0)) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+        _report(f"lanes={lanes}", **ctx.get_schedule())
+    ref, ref32 = both_refs(case, oracle, n)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, f"C2 lanes={lanes}")
+
+
+@pytest.mark.parametrize("case", [cases.SMALL, cases.MID, cases.C4, cases.BIG], ids=lambda c: c.name)
+def test_other_shapes(hip_device, oracle, case):
+    table, _ = cases.setup(case, oracle)
+    n = 2048
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+    ref, ref32 = both_refs(case, oracle, n)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, case.name)
+
+
+def test_large_n_1024(hip_device, oracle):
+    case = cases.C5
+    table, _ = cases.setup(case, oracle)
+    n = 256
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+        _report("C5 schedule", **ctx.get_schedule())
+    ref, ref32 = both_refs(case, oracle, n)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C5 256 paths")
+
+
+def test_path_offset(hip_device, oracle):
+    case = cases.C2
+    table, _ = cases.setup(case, oracle)
+    off, n = 1_000_003, 777
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, case.K, seed=case.seed, path_offset=off, per_path=True, per_joint=True)
+    ref, ref32 = both_refs(case, oracle, n, path_offset=off)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C2 offset")
+
+
+def test_chain_mode(hip_device, oracle):
+    """K=1 with the state carried through a chain: the reference's own loop."""
+    case = cases.C2
+    table, _ = cases.setup(case, oracle)
+    L, chains = 64, 48
+    n = L * chains
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(n, 1, seed=case.seed, chain_length=L, per_path=True, per_joint=True)
+    ref, ref32 = both_refs(case, oracle, n, chain_length=L, K=1)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C2 chain K=1 L=64")
+
+
+def test_invalid_and_zero_weights(hip_device, oracle):
+    """NaN table entries make paths invalid; zero entries give zero weight."""
+    case = cases.MID
+    table, pos32 = cases.setup(case, oracle)
+    bad = table.table.copy()
+    bad[40:60] = np.nan
+    bad[200:260] = 0.0
+    bad_t = oracle.Table(bad, [table.ds, table.minK, table.maxK, table.step, table.minW, table.maxW])
+    n = 2048
+    with capi.Context(0) as ctx:
+        ctx.set_tables(bad, table.minK, table.maxK, table.step, table.ds, case.absorption)
+        ctx.set_curve(pos32, cases.bc13(case))
+        gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
+    ref, ref32 = (oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, n, bad_t, case.absorption,
+                                  seed=case.seed, want_curv=True, precision=prec) for prec in (64, 32))
+    _report("invalid", gpu_invalid=gpu["num_invalid"], ref_invalid=ref["n_invalid"],
+            zero_weight=int(np.isneginf(ref["log10w"]).sum()))
+    assert gpu["num_invalid"] == ref["n_invalid"] > 0
+    check_against_oracle(case, oracle, gpu, ref, ref32, bad_t, "bad table")
+
+
+def test_all_zero_table(hip_device, oracle):
+    case = cases.SMALL
+    table, pos32 = cases.setup(case, oracle)
+    with capi.Context(0) as ctx:
+        ctx.set_tables(np.zeros_like(table.table), table.minK, table.maxK, table.step, table.ds, case.absorption)
+        ctx.set_curve(pos32, cases.bc13(case))
+        gpu = ctx.run(512, case.K, seed=case.seed, per_path=True)
+    assert gpu["num_valid"] == 512 and gpu["num_invalid"] == 0
+    assert gpu["log10_sum_w"] == -math.inf
+    assert np.all(gpu["log10w"] == -np.inf)
+
+
+def test_statistical_agreement_c2(hip_device, oracle):
+    """GPU estimate from 2^20 paths vs oracle estimate from 2^14 other paths."""
+    case = cases.C2
+    with cases.hip_context(case, oracle) as ctx:
+        gpu = ctx.run(1 << 20, case.K, seed=case.seed, path_offset=1 << 30)
+    ref = cases.oracle_run(case, oracle, 1 << 14)
+    mg, sg = capi.estimator(gpu)
+    mr, sr = capi.estimator({"num_paths": ref["n_valid"] + ref["n_invalid"], "log10_sum_w": ref["log10_sum_w"],
+                             "log10_sum_w2": ref["log10_sum_w2"]})
+    diff = abs(10.0 ** (mg - mr) - 1.0)
+    sigma = math.hypot(sg, sr)
+    _report("C2 stat", gpu_log10_mean=f"{mg:.6f}", ref_log10_mean=f"{mr:.6f}", rel_diff=f"{diff:.3e}",
+            sigma_rel=f"{sigma:.3e}")
+    assert diff <= 3.0 * sigma
+
+
+def test_full_size_c2_properties(hip_device, oracle):
+    """2^22 paths: counts, determinism and split invariance of the aggregate."""
+    case = cases.C2
+    n = 1 << 22
+    with cases.hip_context(case, oracle) as ctx:
+        a = ctx.run(n, case.K, seed=case.seed)
+        b = ctx.run(n, case.K, seed=case.seed)
+        h1 = ctx.run(n // 2, case.K, seed=case.seed)
+        h2 = ctx.run(n // 2, case.K, seed=case.seed, path_offset=n // 2)
+        fwd_ms, tot_ms = ctx.last_timing()
+    assert a["num_paths"] == n and a["num_valid"] == n and a["num_invalid"] == 0
+    assert a["log10_sum_w"] == b["log10_sum_w"] and a["log10_sum_w2"] == b["log10_sum_w2"]
+    m = max(h1["log10_sum_w"], h2["log10_sum_w"])
+    merged = m + math.log10(10 ** (h1["log10_sum_w"] - m) + 10 ** (h2["log10_sum_w"] - m))
+    _report("C2 2^22", log10S=a["log10_sum_w"], merged=merged, half_ms=f"{fwd_ms:.3f}")
+    assert abs(merged - a["log10_sum_w"]) <= 1e-6
+    ref = cases.oracle_run(case, oracle, 1 << 14)
+    mg, sg = capi.estimator(a)
+    mr, sr = capi.estimator({"num_paths": 1 << 14, "log10_sum_w": ref["log10_sum_w"],
+                             "log10_sum_w2": ref["log10_sum_w2"]})
+    assert abs(10.0 ** (mg - mr) - 1.0) <= 3.0 * math.hypot(sg, sr)
+
+
+def test_error_codes(hip_device, oracle):
+    case = cases.SMALL
+    with capi.Context(0) as ctx:
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.N = case.N
+            ctx.run(16, 1)
+        assert e.value.code == capi.ERR_STATE
+        table, pos32 = cases.setup(case, oracle)
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.set_tables(np.full(8, np.inf, np.float32), 0.0, 1.0, 0.1, 0.1, 0.0)
+        assert e.value.code == capi.ERR_ARG
+        ctx.set_tables(table.table, table.minK, table.maxK, table.step, table.ds, case.absorption)
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.set_curve(np.zeros((3, 3), np.float32), cases.bc13(case))
+        assert e.value.code == capi.ERR_ARG
+        ctx.set_curve(pos32, cases.bc13(case))
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.run(10, 1, chain_length=4, path_offset=2)
+        assert e.value.code == capi.ERR_ARG
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.set_schedule(3, 0, 0)
+        assert e.value.code == capi.ERR_ARG
+        r = ctx.run(64, case.K)
+        assert r["num_paths"] == 64

@@ -1,0 +1,149 @@
+"""CPU tests of the product's host layer and C ABI surface (no device needed)."""
+import ctypes as C
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests import cases
+from twisty_amd import capi, host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(twisty_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,libpath,listed", [
+    ("twisty_hip.h", capi.LIB_PATH, capi.EXPORTED),
+    ("twisty_host.h", host.LIB_PATH, host.EXPORTED),
+])
+def test_library_exports_every_declared_symbol(header, libpath, listed):
+    names = declared_functions(header)
+    assert names and sorted(listed) == names
+    lib = C.CDLL(libpath)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_no_device_paths_fail_cleanly():
+    n = C.c_int(-1)
+    assert capi.lib().twisty_hip_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        h = C.c_void_p()
+        assert capi.lib().twisty_hip_create(0, C.byref(h)) == capi.ERR_ARG
+        assert not h.value
+    assert capi.lib().twisty_hip_create(-1, None) == capi.ERR_ARG
+    assert capi.lib().twisty_hip_destroy(None) == capi.ERR_ARG
+    assert capi.lib().twisty_hip_last_error(None) == b"null context"
+    assert capi.lib().twisty_hip_version() == 100
+
+
+@pytest.mark.parametrize("N,L,sigma,mu,bins,steps", [
+    (64, 1.5, 0.05, 1.0, 4096, 20000),
+    (20, 1.2, 0.1, 1.0, 4096, 20000),
+    (100, 1.3, 0.08, 1.0, 4096, 20000),
+    (66, 15.0, None, 0.1, 1000, 20000),   # the reference's NoisyCircle settings (eps 0.075)
+    (33, 1.4, 0.07, 0.5, 1024, 2000),
+])
+def test_table_builder_bit_exact_vs_oracle(oracle, N, L, sigma, mu, bins, steps):
+    ds = float(np.float32(L) / np.float32(N))
+    eps = float(np.float32(sigma) ** 2) if sigma else 0.075
+    a = oracle.build_table(ds, mu=mu, eps=eps, scatter=0.1, absorption=0.004, numStepsInt=steps,
+                           numCurvatureSteps=bins)
+    b = host.build_rt_table(ds, mu, eps, 0.1, 0.004, numStepsInt=steps, numCurvatureSteps=bins)
+    assert np.array_equal(a.table.view(np.uint32), b.table.view(np.uint32))
+    assert (a.ds, a.minK, a.maxK, a.step, a.minW, a.maxW) == (b.ds, b.minK, b.maxK, b.step, b.minW, b.maxW)
+
+
+def test_table_builder_rejects_odd_steps():
+    with pytest.raises(ValueError):
+        host.build_rt_table(0.02, 1.0, 0.0025, 0.1, 0.004, numStepsInt=2001, numCurvatureSteps=64)
+
+
+@pytest.mark.parametrize("N,L,edir", [(64, 1.5, (0, 0, 1)), (20, 1.2, (0, 0, 1)), (30, 2.2, (0.6, 0.0, 0.8)),
+                                      (1024, 2.0, (0, 0, 1))])
+def test_initial_arc_vs_oracle(oracle, N, L, edir):
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), edir, L)
+    got = host.initial_arc(bc, N)
+    ref, _ = oracle.initial_arc(N, L, oracle.bc12(edir=edir))
+    assert np.abs(got - ref).max() < 4e-7
+    ds = float(np.float32(L) / np.float32(N))
+    seg = np.linalg.norm(np.diff(got.astype(np.float64), axis=0), axis=1)
+    # float positions of size ~1 carry ~6e-8 rounding, so segment lengths do too
+    assert np.abs(seg - ds).max() < 3e-7
+
+
+def test_minimum_arclength_and_infeasible_arc(oracle):
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0.3, 0.0, 1.0), (0.6, 0.0, 0.8), 1.0)
+    lmin = host.minimum_arclength(bc, 24)
+    assert lmin == pytest.approx(oracle.lib().oracle_minimum_arclength(oracle.bc12(end=(0.3, 0, 1), edir=(0.6, 0, 0.8)), 24))
+    bc[12] = lmin * 1.01
+    host.initial_arc(bc, 24)
+    bc[12] = lmin * 0.99
+    with pytest.raises(ValueError):
+        host.initial_arc(bc, 24)
+
+
+def test_norm(oracle):
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.6)
+    assert host.norm_log10(64, 1.5 / 64, bc) == 0.0
+    ref = oracle.lib().oracle_norm(4, 0.4, oracle.bc12(), 1.6, 2, 2)
+    assert math.isclose(host.norm_log10(4, 0.4, bc), math.log10(ref), rel_tol=1e-9)
+
+
+def _config(**over):
+    cfg = {"experiment": {
+        "experimentParams": {
+            "pathsToGenerate": 4096, "pathsToSkip": 0, "numSegments": 20, "name": "t", "experimentDir": "/tmp/tw",
+            "maxPerturbThreads": 0, "maxWeightThreads": 0, "outputBigFloatWeights": False,
+            "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
+            "random": {"bootstrapSeed": 1, "perturbSeed": 1},
+            "weighting": {"weightFunction": 0, "mu": 1.0, "eps": 0.01, "numStepsInt": 2000,
+                          "numCurvatureSteps": 512, "absorption": 0.004, "scatter": 0.1},
+            "twisty_hip": {"perturbationsPerPath": 10, "chainLength": 1, "writeFiles": False}},
+        "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1],
+                            "endDir": [0, 0, 1], "arclength": 1.2}}}
+    for k, v in over.items():
+        cfg["experiment"]["experimentParams"][k] = v
+    return cfg
+
+
+def test_run_config_reports_errors_without_gpu():
+    if capi.device_count() > 0:
+        pytest.skip("a device is visible; the GPU tests cover the successful run")
+    with pytest.raises(RuntimeError, match="twisty_hip_create"):
+        host.run_config(_config())
+
+
+def test_run_config_rejects_bad_configs():
+    with pytest.raises(RuntimeError, match="JSON parse error"):
+        host.run_config("{ not json")
+    cfg = _config()
+    del cfg["experiment"]["experimentParams"]["numSegments"]
+    with pytest.raises(RuntimeError, match="numSegments"):
+        host.run_config(cfg)
+    with pytest.raises(RuntimeError, match="weightFunction 0"):
+        cfg = _config()
+        cfg["experiment"]["experimentParams"]["weighting"]["weightFunction"] = 1
+        host.run_config(cfg)
+    with pytest.raises(RuntimeError, match="minimum"):
+        cfg = _config()
+        cfg["experiment"]["basicExperiment"]["arclength"] = 0.5
+        host.run_config(cfg)
+
+
+def test_reference_config_schema_parses():
+    """A config from the reference's corpus layout parses (run stops at the device)."""
+    cfg = _config()
+    text = json.dumps(cfg)
+    try:
+        host.run_config(text)
+    except RuntimeError as e:
+        assert "twisty_hip_create" in str(e) or capi.device_count() > 0

@@ -1,0 +1,389 @@
+"""Pins the CPU oracle (test infrastructure) against known answers, closed forms
+and an independent NumPy formulation -- each check chosen so that a plausible
+mistake (wrong range end, rotation sign, lerp normalisation, missing argument
+swap, Simpson node set) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.lerp import joint_log10w
+
+# ---------------------------------------------------------------------------
+# A1 Philox4x32-10: published known-answer vectors (Random123 kat_vectors).
+# ---------------------------------------------------------------------------
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(oracle, ctr, key, expect):
+    assert list(oracle.philox(ctr, key)) == expect
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 9, 27, 30])
+def test_draw_ranges_exhaustive(oracle, N):
+    """diff ~ U{2..min(N-2,25)}, left ~ U{1..N-diff-1}: every value, nothing else."""
+    max_diff = min(N - 2, 25)
+    seen = {}
+    for path in range(3000 if N < 10 else 20000):
+        d, l, x = oracle.draw(7, 0, path, path % 5, N)
+        assert 2 <= d <= max_diff
+        assert 1 <= l <= N - d - 1
+        assert -0.5 < x < 0.5
+        # x is an odd multiple of 2^-25: exactly representable, never 0
+        q = x * 2.0 ** 25
+        assert q == round(q) and int(round(q)) % 2 == 1
+        seen.setdefault(d, set()).add(l)
+    assert set(seen) == set(range(2, max_diff + 1))
+    for d, lefts in seen.items():
+        assert lefts == set(range(1, N - d))
+
+
+def test_draw_uniformity(oracle):
+    N, n = 12, 20000
+    diffs = np.array([oracle.draw(3, 0, p, 0, N)[0] for p in range(n)])
+    counts = np.bincount(diffs, minlength=11)[2:11]
+    expect = n / 9
+    chi2 = ((counts - expect) ** 2 / expect).sum()
+    assert chi2 < 30.0  # 8 dof; p ~ 2e-4
+
+
+def test_draw_streams_and_counters_differ(oracle):
+    a = oracle.draw(1, 0, 5, 3, 64)
+    assert oracle.draw(1, 0, 5, 3, 64) == a
+    assert oracle.draw(1, 1, 5, 3, 64) != a
+    assert oracle.draw(1, 0, 6, 3, 64) != a
+    assert oracle.draw(2, 0, 5, 3, 64) != a
+
+
+# ---------------------------------------------------------------------------
+# A2/A3 quaternion rotation (CurvePerturbUtils.cpp:91-127).
+# ---------------------------------------------------------------------------
+def rodrigues(v, n, theta):
+    v, n = np.asarray(v, float), np.asarray(n, float)
+    return v * math.cos(theta) + np.cross(n, v) * math.sin(theta) + n * np.dot(n, v) * (1 - math.cos(theta))
+
+
+def quat(n, theta):
+    n = np.asarray(n, float)
+    return np.array([math.cos(theta / 2), *(n * math.sin(theta / 2))])
+
+
+def test_quaternion_quarter_turn(oracle):
+    v = np.array([1.0, 0.0, 0.0])
+    oracle.lib().oracle_rotate_vector_by_quaternion(quat([0, 0, 1], math.pi / 2), v)
+    assert np.allclose(v, [0.0, 1.0, 0.0], atol=1e-15)  # active, counter-clockwise
+
+
+def test_quaternion_matches_rodrigues(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        th = rng.uniform(-math.pi, math.pi)
+        v = rng.normal(size=3)
+        w = v.copy()
+        oracle.lib().oracle_rotate_vector_by_quaternion(quat(n, th), w)
+        assert np.allclose(w, rodrigues(v, n, th), atol=1e-13)
+
+
+def test_rotate_quat_by_quat_is_reversed_hamilton(oracle):
+    a, b = np.array([0.3, -1.2, 0.7, 2.0]), np.array([-0.4, 0.5, 1.1, -0.2])
+
+    def hamilton(p, q):
+        return np.array([p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3],
+                         p[0] * q[1] + p[1] * q[0] + p[2] * q[3] - p[3] * q[2],
+                         p[0] * q[2] - p[1] * q[3] + p[2] * q[0] + p[3] * q[1],
+                         p[0] * q[3] + p[1] * q[2] - p[2] * q[1] + p[3] * q[0]])
+
+    out = np.zeros(4)
+    oracle.lib().oracle_rotate_quat_by_quat(a, b, out)
+    assert np.allclose(out, hamilton(b, a), atol=1e-14)
+
+
+def _arc(oracle, N, L):
+    pos, alpha = oracle.initial_arc(N, L)
+    return pos.copy(), alpha
+
+
+def test_perturbation_invariants(oracle):
+    N, L = 16, 1.3
+    pos, _ = _arc(oracle, N, L)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        d = int(rng.integers(2, 15))
+        left = int(rng.integers(1, N - d))
+        x = np.float32(rng.uniform(-0.49, 0.49))
+        before = pos.copy()
+        flat = np.ascontiguousarray(pos.reshape(-1))
+        oracle.lib().oracle_apply_perturbation_f64(flat, left, d, float(x))
+        after = flat.reshape(N + 1, 3)
+        right = left + d
+        # points outside (left, right) do not move
+        assert np.array_equal(after[: left + 1], before[: left + 1])
+        assert np.array_equal(after[right:], before[right:])
+        # rigid: every segment length preserved
+        seg_b = np.linalg.norm(np.diff(before, axis=0), axis=1)
+        seg_a = np.linalg.norm(np.diff(after, axis=0), axis=1)
+        assert np.allclose(seg_a, seg_b, rtol=1e-12)
+        # the interior moved as Rodrigues about p_left -> p_right by 2 pi x
+        n = (before[right] - before[left]) / np.linalg.norm(before[right] - before[left])
+        for k in range(left + 1, right):
+            expect = before[left] + rodrigues(before[k] - before[left], n, 2 * math.pi * float(x))
+            assert np.allclose(after[k], expect, atol=1e-12)
+        # +theta then -theta restores the curve
+        oracle.lib().oracle_apply_perturbation_f64(flat, left, d, float(-x))
+        assert np.allclose(flat.reshape(N + 1, 3), before, atol=1e-12)
+        pos = after
+
+
+# ---------------------------------------------------------------------------
+# A4/A5 tangents and joint curvature (CurvePerturbUtils.cpp:147-154, 219-243).
+# ---------------------------------------------------------------------------
+def _curv(oracle, pos, N, L):
+    tan = np.zeros(N * 3)
+    curv = np.zeros(N - 1)
+    oracle.lib().oracle_tangents_curvatures_f64(np.ascontiguousarray(pos.reshape(-1)), N, L, tan, curv)
+    return tan.reshape(N, 3), curv
+
+
+def test_curvature_closed_forms(oracle):
+    N, L = 40, 1.6
+    inv_ds = float(np.float32(N) / np.float32(L))
+    # straight line: 0
+    line = np.zeros((N + 1, 3))
+    line[:, 2] = np.linspace(0, L, N + 1)
+    _, k = _curv(oracle, line, N, L)
+    assert len(k) == N - 1 and np.allclose(k, 0.0, atol=1e-9)
+    # full reversal at one joint: 2/ds
+    rev = line.copy()
+    rev[N // 2 + 1:, 2] = rev[N // 2, 2] - (rev[N // 2 + 1:, 2] - rev[N // 2, 2])
+    _, k = _curv(oracle, rev, N, L)
+    assert math.isclose(k[N // 2 - 1], 2.0 * inv_ds, rel_tol=1e-12)
+    # regular arc: interior joints 2 sin(alpha/2) / ds
+    pos, alpha = _arc(oracle, N, L)
+    _, k = _curv(oracle, pos, N, L)
+    assert np.allclose(k[1:N - 2], 2.0 * math.sin(alpha / 2) * inv_ds, rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# A11 initial arc (build-defined).
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("N,L", [(20, 1.2), (64, 1.5), (100, 1.3), (1024, 2.0), (5, 1.8)])
+def test_initial_arc(oracle, N, L):
+    pos, alpha = _arc(oracle, N, L)
+    ds = float(np.float32(L) / np.float32(N))
+    seg = np.linalg.norm(np.diff(pos, axis=0), axis=1)
+    assert np.allclose(seg, ds, rtol=1e-12)
+    assert np.allclose(pos[0], [0, 0, 0]) and np.allclose(pos[N], [0, 0, 1])
+    assert np.allclose(pos[1], [0, 0, ds]) and np.allclose(pos[N - 1], [0, 0, 1 - ds])
+    # planar: every point in the plane spanned by the chord (z) and the bulge (x)
+    assert np.allclose(pos[:, 1], 0.0, atol=1e-14)
+    # closed form of the turning angle: |p_{N-1} - p_1| = ds sin(M a/2) / sin(a/2)
+    M = N - 2
+    assert math.isclose(ds * math.sin(M * alpha / 2) / math.sin(alpha / 2), 1 - 2 * ds, rel_tol=1e-10)
+
+
+def test_arc_feasibility_matches_minimum_arclength(oracle):
+    bc = oracle.bc12(edir=(0.6, 0.0, 0.8))
+    N = 30
+    lmin = oracle.lib().oracle_minimum_arclength(bc, N)
+    pos = np.zeros((N + 1) * 3)
+    a = np.zeros(1)
+    import ctypes as C
+    alpha = C.c_double()
+    assert oracle.lib().oracle_initial_arc(bc, float(lmin) * 1.001, N, pos, C.byref(alpha)) == 0
+    assert oracle.lib().oracle_initial_arc(bc, float(lmin) * 0.999, N, pos, C.byref(alpha)) == 1
+    del a
+
+
+# ---------------------------------------------------------------------------
+# A6 per-path weight (PathWeighters.cpp:27-104).
+# ---------------------------------------------------------------------------
+class _T:
+    def __init__(self, table, ds, minK, maxK, step):
+        self.table = np.asarray(table, np.float32)
+        self.ds, self.minK, self.maxK, self.step = ds, minK, maxK, step
+
+
+def test_weight_constant_table(oracle):
+    c, a, ds = 0.37, 0.25, 0.05
+    t = _T(np.full(64, c), ds, 0.0, 31.5, 0.5)
+    k = np.linspace(0.0, 31.0, 17).astype(np.float32)
+    ok, w = oracle.weight_curvatures(k, t, a)
+    absorb = float(np.float32(math.exp(float(np.float32(-np.float32(a) * np.float32(ds))))))
+    # float lerp of equal entries rounds at 1e-7; a missing absorption term is off by 0.09
+    assert ok and math.isclose(w, len(k) * math.log10(np.float32(c) * absorb), rel_tol=1e-6)
+
+
+def test_weight_lerp_in_curvature_units(oracle):
+    """T[i] = i + 1, step 0.5: kappa 1.2 -> bin 2, fraction 0.2 (not 0.4)."""
+    t = _T(np.arange(1, 65), 0.1, 0.0, 31.5, 0.5)
+    ok, w = oracle.weight_curvatures(np.array([1.2], np.float32), t, 0.0)
+    assert ok and math.isclose(10 ** w, 3.0 * 0.8 + 4.0 * 0.2, rel_tol=1e-6)
+    # last bin uses itself on the right (PathWeighters.cpp:72-74) and clamps above maxK
+    ok, w = oracle.weight_curvatures(np.array([31.5, 40.0], np.float32), t, 0.0)
+    assert ok and math.isclose(10 ** w, 64.0 * 64.0, rel_tol=1e-6)
+
+
+def test_weight_nan_and_zero(oracle):
+    t = _T(np.ones(16), 0.1, 0.0, 7.5, 0.5)
+    t.table[3] = np.nan
+    ok, _ = oracle.weight_curvatures(np.array([1.6], np.float32), t, 0.0)
+    assert not ok
+    t.table[:] = 0.0
+    ok, w = oracle.weight_curvatures(np.array([1.0], np.float32), t, 0.0)
+    assert ok and w == -math.inf
+
+
+# ---------------------------------------------------------------------------
+# A7 table (PathWeightUtils.cpp:19-33, 281-433).
+# ---------------------------------------------------------------------------
+def test_gaussian_phase_closed_form(oracle):
+    for m in (0.0025, 0.01, 0.075):
+        for p in (0.0, 1.0, 7.5):
+            expect = math.sqrt(math.pi * m / 2) / (1 - math.exp(-2 / m)) * math.exp(-m * p * p / 2)
+            assert math.isclose(oracle.lib().oracle_gaussian_phase(p, m), expect, rel_tol=2e-6)
+
+
+@pytest.mark.parametrize("N,L,c", [(64, 1.5, 1.0), (20, 1.2, 1.0), (30, 3.0, 0.7)])
+def test_table_scatter_free_closed_form(oracle, N, L, c):
+    """scatter = 0: T(k) = bias/(2 pi^2) sqrt(pi/2) c^-3 exp(-(k ds)^2/(2 c^2)),
+    with c the config `mu` (the argument swap makes mu the regulariser)."""
+    ds = float(np.float32(L) / np.float32(N))
+    t = oracle.build_table(ds, mu=c, eps=0.0025, scatter=0.0, absorption=0.0, numStepsInt=20000,
+                           numCurvatureSteps=512)
+    k = t.minK + np.arange(512, dtype=np.float32) * np.float32(t.step)
+    kds = (k.astype(np.float32) * np.float32(ds)).astype(np.float64)
+    expect = 1 / (2 * math.pi ** 2) * math.sqrt(math.pi / 2) * c ** -3 * np.exp(-kds ** 2 / (2 * c * c))
+    rel = np.abs(t.table / expect - 1)
+    assert rel.max() < 2e-6
+
+
+def test_table_simpson_node_set(oracle):
+    """numStepsInt = 4 keeps nodes 0 (w), 1 (4w) and 4 (w) only: with a p^2
+    integrand (mu = 0, scatter = 0, kappa = 0) T(0) = (20 h^3 / 3) / (2 pi^2),
+    not the full Simpson value 64 h^3 / 3."""
+    hmax = np.float32(2.0)
+    h = float(np.float32(hmax / np.float32(4)))
+    v = oracle.lib().oracle_integrate(0.0, 0.0, 1.0, 0.0, 1.0, 4, 0.0, float(hmax), 0.1)
+    expect = (20 * h ** 3 / 3) / (2 * float(np.float32(math.pi)) ** 2)
+    assert math.isclose(v, expect, rel_tol=1e-6)
+
+
+def test_table_range(oracle):
+    for N, L in [(20, 1.2), (64, 1.5), (1024, 2.0)]:
+        ds = np.float32(L) / np.float32(N)
+        mn, mx, st = oracle.table_range(4096, float(ds))
+        assert mn == 0.0
+        assert mx == np.float32((2.0 / float(ds)) * float(np.float32(1.01)))
+        assert st == np.float32(np.float32(mx) / np.float32(4095))
+
+
+# ---------------------------------------------------------------------------
+# A8 accumulator (CombinedWeightUtils.h:48-174).
+# ---------------------------------------------------------------------------
+def _lse10(w):
+    w = np.asarray(w, np.longdouble)
+    m = w.max()
+    return float(m + np.log10(np.sum(np.power(np.longdouble(10), w - m))))
+
+
+def test_accumulator_brute_force(oracle):
+    rng = np.random.default_rng(4)
+    w = rng.uniform(-120.0, -60.0, size=2_500_000)  # crosses the 1e6-path accumulators
+    got = oracle.lib().oracle_accumulate_log10(w, len(w))
+    assert math.isclose(got, _lse10(w), rel_tol=0, abs_tol=2e-12)
+    perm = rng.permutation(len(w))
+    assert math.isclose(oracle.lib().oracle_accumulate_log10(np.ascontiguousarray(w[perm]), len(w)), got,
+                        abs_tol=2e-12)
+
+
+def test_accumulator_combine_equals_add(oracle):
+    rng = np.random.default_rng(5)
+    w = rng.uniform(-50.0, 10.0, size=4000)
+    one = oracle.lib().oracle_accumulate_log10(w, len(w))
+    two = oracle.lib().oracle_accumulate_combine_log10(w, 1234, len(w))
+    assert math.isclose(one, two, abs_tol=1e-12)
+
+
+def test_accumulator_first_zero_weight_is_nan(oracle):
+    """The reference's AddValue with -inf as the first value sets offset = +inf
+    and the running total to NaN (CombinedWeightUtils.h:62-69); the oracle keeps
+    that, and the exact log-sum-exp aggregate is reported beside it."""
+    w = np.array([-math.inf, -3.0, -4.0])
+    assert math.isnan(oracle.lib().oracle_accumulate_log10(w, 3))
+    w = np.array([-3.0, -math.inf, -4.0])
+    assert math.isclose(oracle.lib().oracle_accumulate_log10(w, 3), _lse10([-3.0, -4.0]), abs_tol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# A9 path normaliser (PathWeightUtils.cpp:643-752).
+# ---------------------------------------------------------------------------
+def test_norm_k4_and_default(oracle):
+    bc = oracle.bc12()
+    ds = 0.4
+    z = abs(1.0 / ds - 2.0)
+    expect = (1.0 / z) * 2 * math.pi / ds ** 3 if z < 2 else 0.0
+    assert math.isclose(oracle.lib().oracle_norm(4, ds, bc, 1.6, 2, 2), expect, rel_tol=1e-5)
+    for N in (7, 20, 64):
+        assert oracle.lib().oracle_norm(N, 0.05, bc, 1.0, 2, 2) == 1.0
+
+
+# ---------------------------------------------------------------------------
+# End to end: an independent NumPy path (Rodrigues on positions, float32 lerp)
+# against the oracle's FP64 path for small N.
+# ---------------------------------------------------------------------------
+def numpy_paths(oracle, pos0, N, L, K, n, table, absorption, seed=1):
+    out = []
+    inv_ds = np.float32(N) / np.float32(L)
+    for p in range(n):
+        pos = pos0.copy()
+        for k in range(K):
+            d, l, x = oracle.draw(seed, 0, p, k, N)
+            r = l + d
+            axis = (pos[r] - pos[l]) / np.linalg.norm(pos[r] - pos[l])
+            for i in range(l + 1, r):
+                pos[i] = pos[l] + rodrigues(pos[i] - pos[l], axis, 2 * math.pi * float(x))
+        t = np.diff(pos, axis=0)
+        t /= np.linalg.norm(t, axis=1, keepdims=True)
+        curv = np.linalg.norm(np.diff(t, axis=0) * float(inv_ds), axis=1)
+        lw, _ = joint_log10w(curv[None, :], table, absorption)
+        out.append(lw.sum())
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", [cases.SMALL, cases.MID], ids=lambda c: c.name)
+def test_end_to_end_against_numpy(oracle, case):
+    table, pos32 = cases.setup(case, oracle)
+    n = 64
+    ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, n, table, case.absorption,
+                          seed=case.seed)
+    npw = numpy_paths(oracle, pos32.astype(np.float64), case.N, case.L, case.K, n, table, case.absorption,
+                      seed=case.seed)
+    assert np.allclose(ref["log10w"], npw, rtol=1e-9, atol=1e-9)
+
+
+def test_chain_mode_replays_from_chain_start(oracle):
+    case = cases.MID
+    table, pos32 = cases.setup(case, oracle)
+    full = oracle.simulate(pos32.astype(np.float64), case.N, case.L, 2, 96, table, case.absorption, chain_length=32)
+    part = oracle.simulate(pos32.astype(np.float64), case.N, case.L, 2, 40, table, case.absorption, chain_length=32,
+                           path_offset=40)
+    assert np.array_equal(full["log10w"][40:80], part["log10w"])
+
+
+def test_reference_fp32_close_to_fp64(oracle):
+    """The reference's own float positions stay within 1e-5 per path at C2."""
+    case = cases.C2
+    a = cases.oracle_run(case, oracle, 512)
+    b = cases.oracle_run(case, oracle, 512, precision=32)
+    rel = np.abs(a["log10w"] - b["log10w"]) / np.abs(a["log10w"])
+    assert rel.max() < 2e-5

@@ -1,0 +1,98 @@
+"""Builds libtwisty_hip.so (HIP kernels + C ABI) in-tree for gfx950.
+
+    python -m twisty_amd.build           # build
+    python -m twisty_amd.build --asm     # also dump gfx950 assembly to _build/asm
+
+The library lands in twisty_amd/lib/ so that it travels with the source tree.
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+BUILD = os.path.join(HERE, "_build")
+ROOT = os.path.dirname(HERE)
+
+ARCH = "gfx950"
+HIP_SOURCES = ["forward.hip", "capi.hip"]
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    # packed FP32 (v_pk_*_f32) stays off: the compiler must not form it.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{os.path.join(ROOT, 'include')}",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(asm=False, verbose=True):
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BUILD, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        run([cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+        if asm:
+            adir = os.path.join(BUILD, "asm")
+            os.makedirs(adir, exist_ok=True)
+            run([cc, *HIPCC_FLAGS, "--cuda-device-only", "-S", os.path.join(CSRC, src), "-o",
+                 os.path.join(adir, os.path.splitext(src)[0] + ".s")])
+    out = os.path.join(LIBDIR, "libtwisty_hip.so")
+    tmp = out + ".tmp"
+    run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    build_host()
+    return out
+
+
+HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "host_capi.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+              f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def build_host():
+    """libtwisty_host.so (C++ host layer) and the twisty_full_experiment driver."""
+    cxx = os.environ.get("CXX", "g++")
+    hdir = os.path.join(CSRC, "host")
+    srcs = [os.path.join(hdir, s) for s in HOST_SOURCES]
+    link = [f"-L{LIBDIR}", "-ltwisty_hip", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    out = os.path.join(LIBDIR, "libtwisty_host.so")
+    run([cxx, *HOST_FLAGS, "-shared", "-o", out + ".tmp", *srcs, *link])
+    os.replace(out + ".tmp", out)
+    exe = os.path.join(LIBDIR, "twisty_full_experiment")
+    run([cxx, *HOST_FLAGS, "-o", exe + ".tmp", os.path.join(hdir, "full_experiment_main.cpp"), *srcs, *link])
+    os.replace(exe + ".tmp", exe)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--asm", action="store_true")
+    a = ap.parse_args(argv)
+    print(build(asm=a.asm))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,169 @@
+"""ctypes binding of libtwisty_hip.so (include/twisty_hip.h).
+
+This is the Python face of the C ABI; it does no computation of its own.  If
+the library is missing the import of `lib()` raises: there is no CPU fallback.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtwisty_hip.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+
+# Every symbol include/twisty_hip.h declares.
+EXPORTED = [
+    "twisty_hip_version", "twisty_hip_device_count", "twisty_hip_create", "twisty_hip_destroy",
+    "twisty_hip_last_error", "twisty_hip_set_tables", "twisty_hip_set_curve", "twisty_hip_run",
+    "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
+]
+
+
+class Partials(C.Structure):
+    _fields_ = [
+        ("log10_sum_w", C.c_double),
+        ("log10_sum_w2", C.c_double),
+        ("max_log10_w", C.c_double),
+        ("num_paths", C.c_uint64),
+        ("num_valid", C.c_uint64),
+        ("num_invalid", C.c_uint64),
+        ("num_clamped", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TwistyHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"twisty_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m twisty_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, u32, u64, f32, i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
+    L.twisty_hip_version.restype = i
+    L.twisty_hip_device_count.argtypes = [C.POINTER(i)]
+    L.twisty_hip_create.argtypes = [i, C.POINTER(vp)]
+    L.twisty_hip_destroy.argtypes = [vp]
+    L.twisty_hip_last_error.argtypes = [vp]
+    L.twisty_hip_last_error.restype = C.c_char_p
+    L.twisty_hip_set_tables.argtypes = [vp, vp, u32, u32, f32, f32, f32, f32, f32]
+    L.twisty_hip_set_curve.argtypes = [vp, vp, u32, vp]
+    L.twisty_hip_run.argtypes = [vp, u64, u64, u32, u32, u64, C.POINTER(Partials), vp, vp]
+    L.twisty_hip_set_schedule.argtypes = [vp, u32, u32, u32]
+    L.twisty_hip_get_schedule.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().twisty_hip_device_count(C.byref(n))
+    return n.value
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context: stream, device buffers, tables and initial curve."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().twisty_hip_create(device, C.byref(self._h))
+        if rc != OK:
+            raise TwistyHipError(rc, f"twisty_hip_create(device={device}) failed")
+
+    def close(self):
+        if self._h:
+            lib().twisty_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise TwistyHipError(rc, lib().twisty_hip_last_error(self._h).decode())
+
+    def set_tables(self, tables, minK, maxK, step, ds, absorption):
+        t = np.ascontiguousarray(tables, np.float32)
+        if t.ndim == 1:
+            t = t[None, :]
+        self._check(lib().twisty_hip_set_tables(self._h, _ptr(t), t.shape[1], t.shape[0], minK, maxK, step, ds,
+                                                absorption))
+
+    def set_curve(self, positions, boundary):
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        bc = np.ascontiguousarray(boundary, np.float32).reshape(13)
+        n = pos.size // 3 - 1
+        self._check(lib().twisty_hip_set_curve(self._h, _ptr(pos), n, _ptr(bc)))
+        self.N = n
+
+    def set_schedule(self, lanes_per_path=0, block_size=0, blocks_per_cu=0):
+        self._check(lib().twisty_hip_set_schedule(self._h, lanes_per_path, block_size, blocks_per_cu))
+
+    def get_schedule(self):
+        g, b, c, l = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(lib().twisty_hip_get_schedule(self._h, C.byref(g), C.byref(b), C.byref(c), C.byref(l)))
+        return {"lanes_per_path": g.value, "block_size": b.value, "blocks_per_cu": c.value, "lds_bytes": l.value}
+
+    def run(self, num_paths, K, seed=1, path_offset=0, chain_length=1, per_path=False, per_joint=False):
+        out = Partials()
+        pp = np.empty(num_paths, np.float32) if per_path else None
+        pj = np.empty(num_paths * (self.N - 1), np.float32) if per_joint else None
+        self._check(lib().twisty_hip_run(self._h, num_paths, path_offset, K, chain_length, seed, C.byref(out),
+                                         _ptr(pp), _ptr(pj)))
+        res = out.as_dict()
+        if per_path:
+            res["log10w"] = pp
+        if per_joint:
+            res["curv"] = pj.reshape(num_paths, self.N - 1)
+        return res
+
+    def last_timing(self):
+        a, b = C.c_float(), C.c_float()
+        self._check(lib().twisty_hip_last_timing(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+def estimator(res, norm=1.0):
+    """Average path weight and its Monte Carlo standard error, both as log10.
+
+    Mirrors the driver's "Avg path weight" = experimentWeight / paths
+    (Experiments/FullExperiment.cpp:244): invalid paths count with weight 0.
+    """
+    n = res["num_paths"]
+    l1, l2 = res["log10_sum_w"], res["log10_sum_w2"]
+    mean_l = l1 - math.log10(n) + math.log10(norm)
+    # var = E[W^2] - E[W]^2, both scaled by 10^{-2 l1} to stay finite
+    e2 = 10.0 ** (l2 - 2 * l1) * n  # E[W^2]/E[W]^2
+    rel_var = max(e2 - 1.0, 0.0) / max(n - 1, 1)
+    return mean_l, math.sqrt(rel_var)  # relative standard error of the mean

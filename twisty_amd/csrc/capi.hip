@@ -1,0 +1,414 @@
+// capi.hip -- C ABI of libtwisty_hip.so (declared in include/twisty_hip.h).
+//
+// Owns device memory, the stream and timing events of a context; validates
+// every shape on the host before a kernel is launched.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/twisty_hip.h"
+#include "forward.hpp"
+
+using namespace twisty_hip;
+
+struct twisty_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    uint32_t numCUs = 0;
+    uint32_t ldsPerBlock = 0;
+    uint32_t ldsPerCU = 0;
+    std::string lastError;
+
+    float *dTable = nullptr;
+    uint32_t tableSize = 0;
+    float minK = 0, maxK = 0, step = 0, ds = 0, absorption = 0;
+    bool haveTables = false;
+
+    float *dInitSeg = nullptr;
+    uint32_t N = 0;
+    float arclength = 0;
+    bool haveCurve = false;
+
+    BlockPartial *dPartials = nullptr;
+    uint32_t partialsCap = 0;
+    FinalOut *dFinal = nullptr;
+
+    float *dPerPath = nullptr;
+    uint64_t perPathCap = 0;
+    float *dPerJoint = nullptr;
+    uint64_t perJointCap = 0;
+
+    uint32_t schedG = 0, schedBlock = 0, schedBpc = 0;
+    float lastFwdMs = 0.0f, lastTotalMs = 0.0f;
+};
+
+namespace {
+
+int fail(twisty_hip_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->lastError = msg;
+    return code;
+}
+
+int hipfail(twisty_hip_ctx *ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, TWISTY_HIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                                                                        \
+    do {                                                                                                          \
+        hipError_t _e = (expr);                                                                                   \
+        if (_e != hipSuccess) return hipfail((ctx), _e, #expr);                                                   \
+    } while (0)
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+uint32_t table_stride(uint32_t n) { return (n + 63u) & ~63u; }
+
+// Resolves a schedule for segment count N.  Zero fields are filled in: G from N
+// (about 16 segments per lane), then the largest compiled block size whose LDS
+// fits one block, then as many blocks per CU as the per-CU LDS holds (at most
+// 2048 lanes per CU).
+int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_t &block, uint32_t &bpc,
+      uint32_t &lds, std::string &why)
+{
+    const uint32_t ts = table_stride(ctx->tableSize ? ctx->tableSize : 4096);
+    if (G == 0) {
+        G = 1;
+        while (G < 64 && (N + G - 1) / G > 16) G <<= 1;
+    }
+    if (!is_pow2(G) || G > 64) {
+        why = "lanesPerPath must be a power of two <= 64";
+        return TWISTY_HIP_ERR_ARG;
+    }
+    uint32_t sizes[32];
+    const int ns = forward_block_sizes(G, sizes, 32);
+    if (block == 0) {
+        for (int k = ns - 1; k >= 0; --k) {
+            if (forward_lds_bytes(G, sizes[k], N, ts) <= ctx->ldsPerBlock) {
+                block = sizes[k];
+                break;
+            }
+        }
+        if (block == 0) {
+            why = "no compiled block size fits the LDS of one block for N=" + std::to_string(N);
+            return TWISTY_HIP_ERR_UNSUPPORTED;
+        }
+    }
+    bool compiled = false;
+    for (int k = 0; k < ns; ++k) compiled |= (sizes[k] == block);
+    if (!compiled) {
+        why = "no kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
+        return TWISTY_HIP_ERR_UNSUPPORTED;
+    }
+    lds = forward_lds_bytes(G, block, N, ts);
+    if (lds > ctx->ldsPerBlock) {
+        why = "schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
+              std::to_string(ctx->ldsPerBlock);
+        return TWISTY_HIP_ERR_UNSUPPORTED;
+    }
+    if (bpc == 0) {
+        bpc = ctx->ldsPerCU / lds;
+        if (bpc * block > 2048) bpc = 2048 / block;
+        if (bpc < 1) bpc = 1;
+    }
+    return TWISTY_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int twisty_hip_version(void) { return 100; }
+
+int twisty_hip_device_count(int *count)
+{
+    if (!count) return TWISTY_HIP_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_create(int device, twisty_hip_ctx **out)
+{
+    if (!out) return TWISTY_HIP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return TWISTY_HIP_ERR_ARG;
+    twisty_hip_ctx *ctx = new twisty_hip_ctx();
+    ctx->device = device;
+    auto bail = [&](hipError_t e, const char *what) {
+        std::fprintf(stderr, "twisty_hip_create: %s: %s\n", what, hipGetErrorString(e));
+        twisty_hip_destroy(ctx);
+        return TWISTY_HIP_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    int v = 0;
+    if ((e = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return bail(e, "CU count");
+    ctx->numCUs = (uint32_t)v;
+    if ((e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device)) != hipSuccess)
+        return bail(e, "LDS per block");
+    ctx->ldsPerBlock = (uint32_t)v;
+    if ((e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device)) != hipSuccess)
+        return bail(e, "LDS per CU");
+    ctx->ldsPerCU = (uint32_t)v;
+    if (ctx->ldsPerCU < ctx->ldsPerBlock) ctx->ldsPerCU = ctx->ldsPerBlock;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "stream");
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "event");
+    if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "event");
+    if ((e = hipEventCreate(&ctx->ev2)) != hipSuccess) return bail(e, "event");
+    if ((e = hipMalloc(&ctx->dFinal, sizeof(FinalOut))) != hipSuccess) return bail(e, "final buffer");
+    *out = ctx;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_destroy(twisty_hip_ctx *ctx)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    // Teardown is best effort: errors here have no caller to act on them.
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->dTable);
+    (void)hipFree(ctx->dInitSeg);
+    (void)hipFree(ctx->dPartials);
+    (void)hipFree(ctx->dFinal);
+    (void)hipFree(ctx->dPerPath);
+    (void)hipFree(ctx->dPerJoint);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TWISTY_HIP_OK;
+}
+
+const char *twisty_hip_last_error(const twisty_hip_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->lastError.c_str();
+}
+
+int twisty_hip_set_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
+      float minCurvature, float maxCurvature, float curvatureStepSize, float ds, float absorption)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!tables || numBins < 2 || numBins > 65536 || numTables < 1 || numTables > 16)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_tables: need 2 <= numBins <= 65536 and 1 <= numTables <= 16");
+    if (!(curvatureStepSize > 0.0f) || !std::isfinite(curvatureStepSize) || !std::isfinite(minCurvature) ||
+          !std::isfinite(maxCurvature) || !(maxCurvature >= minCurvature) || !std::isfinite(ds) ||
+          !std::isfinite(absorption))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_tables: non-finite or inverted curvature range");
+    // Tables sharing one grid combine exactly by summing entries: the lerp of
+    // PathWeighters.cpp:81-82 is linear in the table values.
+    std::vector<float> sum(numBins);
+    for (uint32_t b = 0; b < numBins; ++b) {
+        double acc = 0.0;
+        for (uint32_t t = 0; t < numTables; ++t) {
+            const float v = tables[(uint64_t)t * numBins + b];
+            if (std::isinf(v)) return fail(ctx, TWISTY_HIP_ERR_ARG, "set_tables: table contains inf");
+            acc += (double)v;
+        }
+        sum[b] = (float)acc;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->tableSize != numBins) {
+        HIP_TRY(ctx, hipFree(ctx->dTable));
+        ctx->dTable = nullptr;
+        HIP_TRY(ctx, hipMalloc(&ctx->dTable, (size_t)numBins * sizeof(float)));
+    }
+    HIP_TRY(ctx, hipMemcpy(ctx->dTable, sum.data(), (size_t)numBins * sizeof(float), hipMemcpyHostToDevice));
+    ctx->tableSize = numBins;
+    ctx->minK = minCurvature;
+    ctx->maxK = maxCurvature;
+    ctx->step = curvatureStepSize;
+    ctx->ds = ds;
+    ctx->absorption = absorption;
+    ctx->haveTables = true;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_curve(twisty_hip_ctx *ctx, const float *positions, uint32_t numSegments, const float boundary[13])
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!positions || !boundary || numSegments < 4 || numSegments > 65535)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_curve: need 4 <= numSegments <= 65535");
+    const float arclength = boundary[12];
+    if (!(arclength > 0.0f) || !std::isfinite(arclength))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_curve: arclength must be positive");
+    const uint32_t N = numSegments;
+    // Segment vectors p_{i+1} - p_i, differenced in FP64 and rounded once.
+    std::vector<float> tan((size_t)N * 3);
+    for (uint32_t i = 0; i < N; ++i) {
+        double d[3], n2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            d[a] = (double)positions[3 * (i + 1) + a] - (double)positions[3 * i + a];
+            n2 += d[a] * d[a];
+        }
+        if (!(n2 > 0.0) || !std::isfinite(n2))
+            return fail(ctx, TWISTY_HIP_ERR_ARG, "set_curve: zero-length or non-finite segment " + std::to_string(i));
+        for (int a = 0; a < 3; ++a) tan[3 * i + a] = (float)d[a];
+        if (!((double)tan[3 * i] * tan[3 * i] + (double)tan[3 * i + 1] * tan[3 * i + 1] +
+                      (double)tan[3 * i + 2] * tan[3 * i + 2] >
+                    1e-60))
+            return fail(ctx, TWISTY_HIP_ERR_ARG, "set_curve: segment " + std::to_string(i) + " underflows in float");
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->N != N) {
+        HIP_TRY(ctx, hipFree(ctx->dInitSeg));
+        ctx->dInitSeg = nullptr;
+        HIP_TRY(ctx, hipMalloc(&ctx->dInitSeg, tan.size() * sizeof(float)));
+    }
+    HIP_TRY(ctx, hipMemcpy(ctx->dInitSeg, tan.data(), tan.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->N = N;
+    ctx->arclength = arclength;
+    ctx->haveCurve = true;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize, uint32_t blocksPerCU)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (lanesPerPath && (!is_pow2(lanesPerPath) || lanesPerPath > 64))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: lanesPerPath must be a power of two <= 64");
+    if (blockSize && (blockSize % 64 != 0 || blockSize > 1024))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: blockSize must be a multiple of 64 <= 1024");
+    if (blocksPerCU > 32) return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: blocksPerCU <= 32");
+    ctx->schedG = lanesPerPath;
+    ctx->schedBlock = blockSize;
+    ctx->schedBpc = blocksPerCU;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_t *blockSize,
+      uint32_t *blocksPerCU, uint32_t *ldsBytes)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "get_schedule: set_curve first");
+    uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
+    std::string why;
+    const int rc = resolve_schedule(ctx, ctx->N, G, B, bpc, lds, why);
+    if (rc) return fail(ctx, rc, "get_schedule: " + why);
+    if (lanesPerPath) *lanesPerPath = G;
+    if (blockSize) *blockSize = B;
+    if (blocksPerCU) *blocksPerCU = bpc;
+    if (ldsBytes) *ldsBytes = lds;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint32_t perturbationsPerPath,
+      uint32_t chainLength, uint64_t seed, twisty_hip_partials *out, float *perPathLog10,
+      float *perJointCurvature)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!ctx->haveTables || !ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "run: set_tables and set_curve first");
+    if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run: out is NULL");
+    if (numPaths == 0 || perturbationsPerPath == 0 || chainLength == 0)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run: numPaths, perturbationsPerPath and chainLength must be >= 1");
+    if (pathOffset % chainLength != 0)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run: pathOffset must be a multiple of chainLength");
+    if (numPaths > (1ull << 40)) return fail(ctx, TWISTY_HIP_ERR_ARG, "run: numPaths > 2^40");
+    const uint32_t N = ctx->N;
+    uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
+    std::string why;
+    int rc = resolve_schedule(ctx, N, G, B, bpc, lds, why);
+    if (rc) return fail(ctx, rc, "run: " + why);
+    const uint32_t grid = ctx->numCUs * bpc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (grid > ctx->partialsCap) {
+        HIP_TRY(ctx, hipFree(ctx->dPartials));
+        ctx->dPartials = nullptr;
+        HIP_TRY(ctx, hipMalloc(&ctx->dPartials, (size_t)grid * sizeof(BlockPartial)));
+        ctx->partialsCap = grid;
+    }
+    if (perPathLog10 && numPaths > ctx->perPathCap) {
+        HIP_TRY(ctx, hipFree(ctx->dPerPath));
+        ctx->dPerPath = nullptr;
+        HIP_TRY(ctx, hipMalloc(&ctx->dPerPath, numPaths * sizeof(float)));
+        ctx->perPathCap = numPaths;
+    }
+    const uint64_t nJoint = numPaths * (uint64_t)(N - 1);
+    if (perJointCurvature && nJoint > ctx->perJointCap) {
+        HIP_TRY(ctx, hipFree(ctx->dPerJoint));
+        ctx->dPerJoint = nullptr;
+        HIP_TRY(ctx, hipMalloc(&ctx->dPerJoint, nJoint * sizeof(float)));
+        ctx->perJointCap = nJoint;
+    }
+
+    // Per-segment absorption factor, as PathWeighters.cpp:37 computes it
+    // (float argument -> float exp), applied to every one of the N-1 joints.
+    const double absorb = (double)std::exp((float)(-ctx->absorption * ctx->ds));
+    const double absorbLog2PerPath = (double)(N - 1) * std::log2(absorb);
+
+    FwdParams p {};
+    p.initSeg = ctx->dInitSeg;
+    p.table = ctx->dTable;
+    p.partials = ctx->dPartials;
+    p.perPathLog10 = perPathLog10 ? ctx->dPerPath : nullptr;
+    p.perJointCurv = perJointCurvature ? ctx->dPerJoint : nullptr;
+    p.seed = seed;
+    p.pathOffset = pathOffset;
+    p.numChains = (numPaths + chainLength - 1) / chainLength;
+    p.numPaths = numPaths;
+    p.chainLength = chainLength;
+    p.N = N;
+    p.K = perturbationsPerPath;
+    p.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
+    p.stream = 0;
+    p.S = (N + G - 1) / G;
+    p.scratchRow = forward_scratch_row(G, N);
+    p.tableSize = (int32_t)ctx->tableSize;
+    p.tableStride = table_stride(ctx->tableSize);
+    p.minK = ctx->minK;
+    p.maxK = ctx->maxK;
+    p.step = ctx->step;
+    p.invDs = (float)N / ctx->arclength;  // CurvePerturbUtils.cpp:223
+    p.log10of2 = 0.30102999566398119521f;
+    p.absorbLog10PerPath = (float)(absorbLog2PerPath * 0.30102999566398119521);
+
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "forward kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "finalize kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    FinalOut fo;
+    HIP_TRY(ctx, hipMemcpyAsync(&fo, ctx->dFinal, sizeof(FinalOut), hipMemcpyDeviceToHost, ctx->stream));
+    if (perPathLog10)
+        HIP_TRY(ctx, hipMemcpyAsync(perPathLog10, ctx->dPerPath, numPaths * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    if (perJointCurvature)
+        HIP_TRY(ctx, hipMemcpyAsync(perJointCurvature, ctx->dPerJoint, nJoint * sizeof(float),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev2));
+    out->log10_sum_w = fo.log10SumW;
+    out->log10_sum_w2 = fo.log10SumW2;
+    out->max_log10_w = fo.maxLog10W;
+    out->num_paths = numPaths;
+    out->num_valid = fo.nValid;
+    out->num_invalid = fo.nInvalid;
+    out->num_clamped = fo.nClamped;
+    ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (forwardMs) *forwardMs = ctx->lastFwdMs;
+    if (totalMs) *totalMs = ctx->lastTotalMs;
+    return TWISTY_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,307 @@
+// device_common.hpp -- device-side building blocks of the optimal-perturb hot path.
+//
+// Counter-based draws, the table lookup of PathWeighters.cpp:27-104 and the
+// log-domain accumulator used by every kernel in this library.  Compiled for
+// gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace twisty_hip {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter =
+// (perturbation, stream, path_lo, path_hi), key = (seed_lo, seed_hi).  Stands in
+// for the reference's per-thread std::mt19937_64 / curand XORWOW states
+// (FullExperimentRunnerOptimalPerturb.cpp:113-120, ...Optimized_GPU.cu:678-686):
+// no state, no init kernel, any path range reproducible from its counter.
+// ---------------------------------------------------------------------------
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+      uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return { c0, c1, c2, c3 };
+}
+
+// One perturbation's draw: diff ~ U{2..maxDiff}, left ~ U{1..N-diff-1},
+// half-angle = pi*x with x exactly representable in float
+// (FullExperimentRunnerOptimalPerturb.cpp:483-502 ranges).
+struct Draw {
+    uint32_t diff;
+    uint32_t left;
+    float ch;  // cos(theta/2)
+    float sh;  // sin(theta/2)
+};
+
+// A draw as the rotation kernels consume it: packed indices and the
+// trigonometric terms of the rotation matrix (cos theta, sin theta,
+// 1 - cos theta = 2 sin^2(theta/2) without cancellation).
+struct DrawRec {
+    uint32_t packed;  // diff | left << 16
+    float c, s, C;
+};
+
+__device__ __forceinline__ DrawRec pack_draw(const Draw &d)
+{
+    DrawRec r;
+    r.packed = d.diff | (d.left << 16);
+    r.c = d.ch * d.ch - d.sh * d.sh;
+    r.s = 2.0f * d.sh * d.ch;
+    r.C = 2.0f * d.sh * d.sh;
+    return r;
+}
+
+__device__ __forceinline__ Draw make_draw(uint64_t seed, uint32_t stream, uint64_t path, uint32_t k, uint32_t N,
+      uint32_t maxDiff)
+{
+    const U4 r = philox4x32_10(k, stream, (uint32_t)path, (uint32_t)(path >> 32), (uint32_t)seed,
+          (uint32_t)(seed >> 32));
+    Draw d;
+    d.diff = 2u + __umulhi(r.x, maxDiff - 1u);
+    d.left = 1u + __umulhi(r.y, N - d.diff - 1u);
+    const int32_t odd = (int32_t)(((r.z >> 8) << 1) | 1u) - (int32_t)(1u << 24);
+    const float x = (float)odd * 0x1p-25f;
+    sincospif(x, &d.sh, &d.ch);
+    return d;
+}
+
+// Rotation matrix of the quaternion (cos(theta/2), n sin(theta/2)) from
+// c = cos theta, s = sin theta, C = 1 - cos theta: active rotation by theta
+// about n, identical to the sandwich q v q* of CurvePerturbUtils.cpp:113-127.
+struct Mat3 {
+    float m00, m01, m02, m10, m11, m12, m20, m21, m22;
+};
+
+__device__ __forceinline__ Mat3 rotation_from_axis(float nx, float ny, float nz, float c, float s, float C)
+{
+    const float xC = nx * C, yC = ny * C, zC = nz * C;
+    const float xs = nx * s, ys = ny * s, zs = nz * s;
+    Mat3 R;
+    R.m00 = fmaf(nx, xC, c);
+    R.m01 = fmaf(nx, yC, -zs);
+    R.m02 = fmaf(nx, zC, ys);
+    R.m10 = fmaf(ny, xC, zs);
+    R.m11 = fmaf(ny, yC, c);
+    R.m12 = fmaf(ny, zC, -xs);
+    R.m20 = fmaf(nz, xC, -ys);
+    R.m21 = fmaf(nz, yC, xs);
+    R.m22 = fmaf(nz, zC, c);
+    return R;
+}
+
+// ---------------------------------------------------------------------------
+// Segment weight lookup.  PathWeighters.cpp:41-82 with the reference's float
+// rounding kept step by step (no contraction), so identical curvatures give
+// identical lerped weights: snap/clamp (:45-65), floor index with the last-bin
+// fix (:67-74), fraction in curvature units (:79), float lerp (:81-82).  The
+// bin index is clamped into the table for curvatures that round past the last
+// bin (the reference would read one past the end there).
+// ---------------------------------------------------------------------------
+struct TableView {
+    const float *w;  // LDS
+    int32_t size;
+    float minK, maxK, step;
+};
+
+__device__ __forceinline__ float lookup_weight(const TableView &t, float curvature, uint32_t &clamped)
+{
+    if (curvature < t.minK) {
+        if (!(fabsf(curvature - t.minK) < 1e-3f)) ++clamped;
+        curvature = t.minK;
+    }
+    if (curvature > t.maxK) {
+        if (!(fabsf(t.maxK - curvature) < 1e-3f)) ++clamped;
+        curvature = t.maxK;
+    }
+    const float distanceFromMin = __fsub_rn(curvature, t.minK);
+    const float realIdx = __fdiv_rn(distanceFromMin, t.step);
+    int32_t leftIdx = (int32_t)floorf(realIdx);
+    leftIdx = min(max(leftIdx, 0), t.size - 1);
+    const int32_t rightIdx = (leftIdx == t.size - 1) ? leftIdx : leftIdx + 1;
+    const float interpDist = __fsub_rn(distanceFromMin, __fmul_rn((float)leftIdx, t.step));
+    const float a = __fmul_rn(t.w[leftIdx], __fsub_rn(1.0f, interpDist));
+    const float b = __fmul_rn(t.w[rightIdx], interpDist);
+    return __fadd_rn(a, b);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated float arithmetic.
+// ---------------------------------------------------------------------------
+struct FF {  // unevaluated sum hi + lo
+    float hi, lo;
+};
+
+__device__ __forceinline__ FF two_sum(float a, float b)
+{
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return { s, e };
+}
+
+__device__ __forceinline__ FF ff_add(FF a, FF b)
+{
+    FF s = two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    const float hi = s.hi + s.lo;
+    return { hi, s.lo - (hi - s.hi) };
+}
+
+__device__ __forceinline__ FF ff_add_f(FF a, float b)
+{
+    FF s = two_sum(a.hi, b);
+    s.lo += a.lo;
+    const float hi = s.hi + s.lo;
+    return { hi, s.lo - (hi - s.hi) };
+}
+
+// ---------------------------------------------------------------------------
+// Log-domain accumulator in base 2: represents sum_p 2^{v_p} as 2^m * s and
+// sum_p 2^{2 v_p} as 2^{2m} * s2, with s and s2 compensated.  This replaces the
+// reference's CombinedWeightValues_C (CombinedWeightUtils.h:28-158), which keeps
+// a double running total per 1e6 paths with a decimal offset; both carry the
+// same quantity without overflow.
+// ---------------------------------------------------------------------------
+struct LseAcc {
+    float m;   // running max of v (-inf when empty)
+    FF s;      // sum 2^{v-m}
+    FF s2;     // sum 2^{2(v-m)}
+    uint32_t nValid, nInvalid, nClamped, pad;
+};
+
+__device__ __forceinline__ void lse_init(LseAcc &a)
+{
+    a.m = -INFINITY;
+    a.s = { 0.0f, 0.0f };
+    a.s2 = { 0.0f, 0.0f };
+    a.nValid = a.nInvalid = a.nClamped = a.pad = 0;
+}
+
+// Adds one path with log2 weight v = vhi + vlo.
+__device__ __forceinline__ void lse_add(LseAcc &a, float vhi, float vlo)
+{
+    if (vhi == -INFINITY) return;  // zero weight: counted as valid by the caller
+    if (vhi > a.m) {
+        const float sc = (a.m == -INFINITY) ? 0.0f : exp2f(a.m - vhi);
+        const float sc2 = sc * sc;
+        a.s.hi *= sc;
+        a.s.lo *= sc;
+        a.s2.hi *= sc2;
+        a.s2.lo *= sc2;
+        a.m = vhi;
+    }
+    const float d = (vhi - a.m) + vlo;
+    const float e = exp2f(d);
+    a.s = ff_add_f(a.s, e);
+    a.s2 = ff_add_f(a.s2, e * e);
+}
+
+__device__ __forceinline__ void lse_merge(LseAcc &a, const LseAcc &b)
+{
+    a.nValid += b.nValid;
+    a.nInvalid += b.nInvalid;
+    a.nClamped += b.nClamped;
+    if (b.m == -INFINITY) return;
+    if (a.m == -INFINITY) {
+        a.m = b.m;
+        a.s = b.s;
+        a.s2 = b.s2;
+        return;
+    }
+    const float M = fmaxf(a.m, b.m);
+    const float fa = exp2f(a.m - M), fb = exp2f(b.m - M);
+    FF sa { a.s.hi * fa, a.s.lo * fa }, sb { b.s.hi * fb, b.s.lo * fb };
+    FF qa { a.s2.hi * fa * fa, a.s2.lo * fa * fa }, qb { b.s2.hi * fb * fb, b.s2.lo * fb * fb };
+    a.m = M;
+    a.s = ff_add(sa, sb);
+    a.s2 = ff_add(qa, qb);
+}
+
+// Per-block partial written to global memory; combined in FP64 by
+// finalize_kernel.  m is also the block's largest path log2 weight.
+struct BlockPartial {
+    float m;
+    float s_hi, s_lo;
+    float s2_hi, s2_lo;
+    uint32_t nValid, nInvalid, nClamped;
+};
+
+// ---------------------------------------------------------------------------
+// Cross-lane sums inside aligned groups of G lanes.  G <= 16 uses DPP row
+// patterns (xor 1, xor 2 inside a quad, then half-row and row mirrors, after
+// which every lane of the group holds the group total); larger groups finish
+// with ds_swizzle-free bpermute shuffles.
+// ---------------------------------------------------------------------------
+template<int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template<int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int DPP_QUAD_XOR1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+
+template<int G>
+__device__ __forceinline__ float group_sum(float v)
+{
+    if constexpr (G >= 2) v += dpp_f<DPP_QUAD_XOR1>(v);
+    if constexpr (G >= 4) v += dpp_f<DPP_QUAD_XOR2>(v);
+    if constexpr (G >= 8) v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
+    if constexpr (G >= 16) v += dpp_f<DPP_ROW_MIRROR>(v);
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+template<int G>
+__device__ __forceinline__ uint32_t group_or(uint32_t v)
+{
+    if constexpr (G >= 2) v |= dpp_u<DPP_QUAD_XOR1>(v);
+    if constexpr (G >= 4) v |= dpp_u<DPP_QUAD_XOR2>(v);
+    if constexpr (G >= 8) v |= dpp_u<DPP_ROW_HALF_MIRROR>(v);
+    if constexpr (G >= 16) v |= dpp_u<DPP_ROW_MIRROR>(v);
+    if constexpr (G >= 32) v |= (uint32_t)__shfl_xor((int)v, 16, 64);
+    if constexpr (G >= 64) v |= (uint32_t)__shfl_xor((int)v, 32, 64);
+    return v;
+}
+
+template<int G>
+__device__ __forceinline__ FF group_sum_ff(FF v)
+{
+    // Exchanges (hi, lo) pairs with the same partner pattern as group_sum.
+    if constexpr (G >= 2) v = ff_add(v, FF { dpp_f<DPP_QUAD_XOR1>(v.hi), dpp_f<DPP_QUAD_XOR1>(v.lo) });
+    if constexpr (G >= 4) v = ff_add(v, FF { dpp_f<DPP_QUAD_XOR2>(v.hi), dpp_f<DPP_QUAD_XOR2>(v.lo) });
+    if constexpr (G >= 8)
+        v = ff_add(v, FF { dpp_f<DPP_ROW_HALF_MIRROR>(v.hi), dpp_f<DPP_ROW_HALF_MIRROR>(v.lo) });
+    if constexpr (G >= 16) v = ff_add(v, FF { dpp_f<DPP_ROW_MIRROR>(v.hi), dpp_f<DPP_ROW_MIRROR>(v.lo) });
+    if constexpr (G >= 32) v = ff_add(v, FF { __shfl_xor(v.hi, 16, 64), __shfl_xor(v.lo, 16, 64) });
+    if constexpr (G >= 64) v = ff_add(v, FF { __shfl_xor(v.hi, 32, 64), __shfl_xor(v.lo, 32, 64) });
+    return v;
+}
+
+}  // namespace twisty_hip

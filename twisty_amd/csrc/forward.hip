@@ -1,0 +1,405 @@
+// forward.hip -- plain forward kernel of the optimal-perturb hot path (gfx950).
+//
+// One group of G lanes owns one path (or one Markov chain of chainLength paths).
+// The state is the N segment vectors d_i = p_{i+1} - p_i, not the points: a
+// rigid rotation of points left+1..right-1 about the left point
+// (FullExperimentRunnerOptimalPerturb.cpp:495-519) is exactly a rotation of
+// d_left..d_{right-1}, and the rotation axis p_right - p_left is their sum.
+// Segment vectors keep each curvature free of the 1/ds amplification that
+// differencing rounded positions introduces; they are normalised into tangents
+// only when the path is weighted.
+//
+// LDS layout per block (BLOCK lanes, S = ceil(N/G) slots per lane):
+//   [state: S * 3 * BLOCK floats][table: tableStride floats][draws: 4*BLOCK words if G > 1]
+//   state element (slot s, component c) of lane L is at s*3*BLOCK + c*BLOCK + L.
+// Lane j of a group stores segments i = s*G + j, so every lane touches only its
+// own column: bank = L mod 32 for any slot, i.e. the data-dependent sub-chain
+// accesses are bank-conflict free, and sub-chain slot w of a perturbation is a
+// compile-time offset from one per-perturbation base address.
+#include "device_common.hpp"
+#include "forward.hpp"
+
+namespace twisty_hip {
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block-wide merge of the per-lane accumulators into partials[blockIdx.x].
+__device__ void block_reduce_store(LseAcc acc, float *scratch, BlockPartial *partials)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        LseAcc b;
+        b.m = __shfl_xor(acc.m, o, 64);
+        b.s.hi = __shfl_xor(acc.s.hi, o, 64);
+        b.s.lo = __shfl_xor(acc.s.lo, o, 64);
+        b.s2.hi = __shfl_xor(acc.s2.hi, o, 64);
+        b.s2.lo = __shfl_xor(acc.s2.lo, o, 64);
+        b.nValid = (uint32_t)__shfl_xor((int)acc.nValid, o, 64);
+        b.nInvalid = (uint32_t)__shfl_xor((int)acc.nInvalid, o, 64);
+        b.nClamped = (uint32_t)__shfl_xor((int)acc.nClamped, o, 64);
+        lse_merge(acc, b);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+    LseAcc *sh = reinterpret_cast<LseAcc *>(scratch);
+    if ((threadIdx.x & 63) == 0) sh[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        LseAcc a = sh[0];
+        for (int w = 1; w < nWaves; ++w) lse_merge(a, sh[w]);
+        BlockPartial p;
+        p.m = a.m;
+        p.s_hi = a.s.hi;
+        p.s_lo = a.s.lo;
+        p.s2_hi = a.s2.hi;
+        p.s2_lo = a.s2.lo;
+        p.nValid = a.nValid;
+        p.nInvalid = a.nInvalid;
+        p.nClamped = a.nClamped;
+        partials[blockIdx.x] = p;
+    }
+}
+
+__device__ __forceinline__ DrawRec load_rec(const uint32_t *p)
+{
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return { v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w) };
+}
+
+// One perturbation applied by lane j of its group: read this lane's segments of
+// the sub-chain [left, right), sum them across the group into the axis
+// p_right - p_left, build the rotation and write the rotated segments back.
+// Window slots past this lane's count are redirected to the lane's zeroed
+// scratch row (scr = lane's element of that row): they read zeros, add nothing
+// to the axis and write zeros back, so every load and store is unconditional
+// with a compile-time offset.
+template<int G, int BLOCK>
+__device__ __forceinline__ void perturb_one(float *smem, uint32_t lane, uint32_t j, uint32_t scr, const DrawRec &rec)
+{
+    constexpr int WMAX = (25 + G - 1) / G;  // sub-chain slots per lane (diff <= 25)
+    constexpr uint32_t SLOT = 3u * BLOCK;   // floats per slot row
+    const uint32_t diff = rec.packed & 0xFFFFu;
+    const uint32_t left = rec.packed >> 16;
+    const uint32_t right = left + diff;
+    // First segment index >= left owned by this lane, and how many of this
+    // lane's segments lie in [left, right).
+    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
+    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
+    const uint32_t base = (i0 / G) * SLOT + lane;
+    uint32_t addr[WMAX];
+    float dx[WMAX], dy[WMAX], dz[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        addr[w] = ((uint32_t)w < cnt) ? base : scr - (uint32_t)w * SLOT;
+        dx[w] = smem[addr[w] + w * SLOT];
+        dy[w] = smem[addr[w] + w * SLOT + BLOCK];
+        dz[w] = smem[addr[w] + w * SLOT + 2 * BLOCK];
+    }
+    float ax = dx[0], ay = dy[0], az = dz[0];
+#pragma unroll
+    for (int w = 1; w < WMAX; ++w) {
+        ax += dx[w];
+        ay += dy[w];
+        az += dz[w];
+    }
+    if constexpr (G > 1) {
+        ax = group_sum<G>(ax);
+        ay = group_sum<G>(ay);
+        az = group_sum<G>(az);
+    }
+    // |axis|^2 >= ds^2 > 0 for any sub-chain, far from denormals.
+    const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+    const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec.c, rec.s, rec.C);
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        const float x = dx[w], y = dy[w], z = dz[w];
+        smem[addr[w] + w * SLOT] = fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z));
+        smem[addr[w] + w * SLOT + BLOCK] = fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z));
+        smem[addr[w] + w * SLOT + 2 * BLOCK] = fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z));
+    }
+}
+
+template<int G, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
+{
+    constexpr uint32_t SLOT = 3u * BLOCK;  // floats per slot row
+    extern __shared__ float smem[];
+    const uint32_t N = p.N, S = p.S, K = p.K;
+    // Rows 0..S-1 hold segments; row p.scratchRow (>= S and >= WMAX-1, so that
+    // scr - w*SLOT never underflows) is the lane's zero row.
+    const uint32_t tabOff = (p.scratchRow + 1) * SLOT;
+    const uint32_t drawOff = tabOff + p.tableStride;
+    uint32_t *drawbuf = reinterpret_cast<uint32_t *>(smem + drawOff);
+
+    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
+    __syncthreads();
+    const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+
+    const uint32_t lane = threadIdx.x;
+    const uint32_t j = lane & (G - 1);
+    const uint32_t groupBase = lane - j;
+    constexpr uint32_t groupsPerBlock = BLOCK / G;
+    const uint32_t scr = p.scratchRow * SLOT + lane;
+    smem[scr] = 0.0f;
+    smem[scr + BLOCK] = 0.0f;
+    smem[scr + 2 * BLOCK] = 0.0f;
+
+    LseAcc acc;
+    lse_init(acc);
+    uint32_t clamped = 0;
+
+    for (uint64_t chain = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; chain < p.numChains;
+          chain += (uint64_t)gridDim.x * groupsPerBlock) {
+        for (uint32_t sIdx = 0; sIdx < p.chainLength; ++sIdx) {
+            const uint64_t local = chain * p.chainLength + sIdx;
+            if (local >= p.numPaths) break;  // the last chain may be shorter
+            const uint64_t path = p.pathOffset + local;
+            if (sIdx == 0) {
+                for (uint32_t s = 0; s < S; ++s) {
+                    const uint32_t i = s * G + j;
+                    if (i < N) {
+                        smem[s * SLOT + lane] = p.initSeg[3 * i + 0];
+                        smem[s * SLOT + BLOCK + lane] = p.initSeg[3 * i + 1];
+                        smem[s * SLOT + 2 * BLOCK + lane] = p.initSeg[3 * i + 2];
+                    }
+                }
+            }
+
+            for (uint32_t kb = 0; kb < K; kb += G) {
+                const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
+                if constexpr (G == 1) {
+                    perturb_one<G, BLOCK>(smem, lane, j, scr, pack_draw(make_draw(p.seed, p.stream, path, kb, N, p.maxDiff)));
+                } else {
+                    // Each lane draws one perturbation of the batch into the
+                    // group's slice of drawbuf.
+                    if (kb + j < K) {
+                        const DrawRec r = pack_draw(make_draw(p.seed, p.stream, path, kb + j, N, p.maxDiff));
+                        *reinterpret_cast<uint4 *>(drawbuf + lane * 4) =
+                              make_uint4(r.packed, __float_as_uint(r.c), __float_as_uint(r.s), __float_as_uint(r.C));
+                    }
+                    wave_sync_lds();
+                    if constexpr (G <= 8) {
+                        // Whole batch into registers, then apply it.
+                        DrawRec recs[G];
+#pragma unroll
+                        for (int kk = 0; kk < G; ++kk) recs[kk] = load_rec(drawbuf + (groupBase + kk) * 4);
+                        wave_sync_lds();  // read before drawbuf is rewritten
+#pragma unroll
+                        for (int kk = 0; kk < G; ++kk) {
+                            if ((uint32_t)kk >= kcount) break;
+                            perturb_one<G, BLOCK>(smem, lane, j, scr, recs[kk]);
+                        }
+                    } else {
+                        for (uint32_t kk = 0; kk < kcount; ++kk)
+                            perturb_one<G, BLOCK>(smem, lane, j, scr, load_rec(drawbuf + (groupBase + kk) * 4));
+                        wave_sync_lds();
+                    }
+                }
+            }
+
+            // Weighting: tangents (CurvePerturbUtils.cpp:147-154), curvature
+            // (CurvePerturbUtils.cpp:219-243), lookup and log
+            // (PathWeighters.cpp:41-96).  Lane j handles joints i = j mod G.
+            if constexpr (G > 1) wave_sync_lds();
+            FF lsum { 0.0f, 0.0f };
+            uint32_t flags = 0;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
+            float ta0 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
+            if constexpr (G == 1) {
+                const float a0 = smem[lane], a1 = smem[BLOCK + lane], a2 = smem[2 * BLOCK + lane];
+                const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
+                ta0 = a0 * ia;
+                ta1 = a1 * ia;
+                ta2 = a2 * ia;
+            }
+            for (uint32_t i = j; i + 1 < N; i += G) {
+                if constexpr (G > 1) {
+                    const uint32_t ai = (i / G) * SLOT + lane;
+                    const float a0 = smem[ai], a1 = smem[ai + BLOCK], a2 = smem[ai + 2 * BLOCK];
+                    const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
+                    ta0 = a0 * ia;
+                    ta1 = a1 * ia;
+                    ta2 = a2 * ia;
+                }
+                const uint32_t i1 = i + 1;
+                const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
+                const float b0 = smem[bi], b1 = smem[bi + BLOCK], b2 = smem[bi + 2 * BLOCK];
+                const float ib = __builtin_amdgcn_rsqf(fmaf(b0, b0, fmaf(b1, b1, b2 * b2)));
+                const float tb0 = b0 * ib, tb1 = b1 * ib, tb2 = b2 * ib;
+                const float e0 = (tb0 - ta0) * p.invDs;
+                const float e1 = (tb1 - ta1) * p.invDs;
+                const float e2 = (tb2 - ta2) * p.invDs;
+                const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+                if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = kappa;
+                const float wseg = lookup_weight(tv, kappa, clamped);
+                const float l = log2f(wseg);
+                if (isnan(l)) flags |= 1u;
+                else if (l == -INFINITY) flags |= 2u;
+                else if (l == INFINITY) flags |= 4u;
+                else lsum = ff_add_f(lsum, l);
+                if constexpr (G == 1) {
+                    ta0 = tb0;
+                    ta1 = tb1;
+                    ta2 = tb2;
+                }
+            }
+            if constexpr (G > 1) {
+                lsum = group_sum_ff<G>(lsum);
+                flags = group_or<G>(flags);
+                wave_sync_lds();  // joints read other lanes' columns before the next reset
+            }
+            if (j == 0) {
+                const bool invalid = (flags & 1u) || ((flags & 6u) == 6u);
+                float outv;
+                if (invalid) {
+                    ++acc.nInvalid;
+                    outv = __int_as_float(0x7fc00000);
+                } else {
+                    ++acc.nValid;
+                    if (flags & 2u) {
+                        outv = -INFINITY;
+                    } else if (flags & 4u) {
+                        outv = INFINITY;
+                        lse_add(acc, INFINITY, 0.0f);
+                    } else {
+                        lse_add(acc, lsum.hi, lsum.lo);
+                        outv = fmaf(lsum.hi, p.log10of2, fmaf(lsum.lo, p.log10of2, p.absorbLog10PerPath));
+                    }
+                }
+                if (p.perPathLog10) p.perPathLog10[local] = outv;
+            }
+        }
+    }
+    acc.nClamped = clamped;
+    block_reduce_store(acc, smem, p.partials);
+}
+
+// FP64 cross-block normaliser: combines the per-block partials in a fixed
+// order (deterministic for a fixed grid) and adds the per-path absorption term.
+__global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts, uint32_t n,
+      double absorbLog2PerPath, FinalOut *out)
+{
+    __shared__ double shM[256], shS[256], shS2[256];
+    __shared__ unsigned long long shV[256], shI[256], shC[256];
+    double M = -INFINITY, S = 0.0, S2 = 0.0;
+    unsigned long long nv = 0, ni = 0, nc = 0;
+    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) {
+        const BlockPartial q = parts[b];
+        nv += q.nValid;
+        ni += q.nInvalid;
+        nc += q.nClamped;
+        if (q.m == -INFINITY) continue;
+        const double m = q.m;
+        const double s = (double)q.s_hi + (double)q.s_lo;
+        const double s2 = (double)q.s2_hi + (double)q.s2_lo;
+        if (m > M) {
+            const double f = (M == -INFINITY) ? 0.0 : exp2(M - m);
+            S = S * f + s;
+            S2 = S2 * f * f + s2;
+            M = m;
+        } else {
+            const double f = exp2(m - M);
+            S += s * f;
+            S2 += s2 * f * f;
+        }
+    }
+    shM[threadIdx.x] = M;
+    shS[threadIdx.x] = S;
+    shS2[threadIdx.x] = S2;
+    shV[threadIdx.x] = nv;
+    shI[threadIdx.x] = ni;
+    shC[threadIdx.x] = nc;
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const uint32_t o = threadIdx.x + h;
+            const double ma = shM[threadIdx.x], mb = shM[o];
+            if (mb != -INFINITY) {
+                if (ma == -INFINITY) {
+                    shM[threadIdx.x] = mb;
+                    shS[threadIdx.x] = shS[o];
+                    shS2[threadIdx.x] = shS2[o];
+                } else {
+                    const double Mx = fmax(ma, mb);
+                    const double fa = exp2(ma - Mx), fb = exp2(mb - Mx);
+                    shS[threadIdx.x] = shS[threadIdx.x] * fa + shS[o] * fb;
+                    shS2[threadIdx.x] = shS2[threadIdx.x] * fa * fa + shS2[o] * fb * fb;
+                    shM[threadIdx.x] = Mx;
+                }
+            }
+            shV[threadIdx.x] += shV[o];
+            shI[threadIdx.x] += shI[o];
+            shC[threadIdx.x] += shC[o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double l2 = 0.30102999566398119521;  // log10(2)
+        const double m = shM[0];
+        if (m == -INFINITY) {
+            out->log10SumW = -INFINITY;
+            out->log10SumW2 = -INFINITY;
+            out->maxLog10W = -INFINITY;
+        } else {
+            out->log10SumW = (m + log2(shS[0]) + absorbLog2PerPath) * l2;
+            out->log10SumW2 = (2.0 * m + log2(shS2[0]) + 2.0 * absorbLog2PerPath) * l2;
+            out->maxLog10W = (m + absorbLog2PerPath) * l2;
+        }
+        out->nValid = shV[0];
+        out->nInvalid = shI[0];
+        out->nClamped = shC[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Instances: (lanes per path, block size).  Sub-chain slot offsets are
+// immediates, so BLOCK is a template parameter; the set below covers the
+// shapes the schedule resolver and the autotuner choose from.
+// ---------------------------------------------------------------------------
+#define TWISTY_FWD_INSTANCES(X)                                                                                   \
+    X(1, 128) X(1, 192) X(2, 192) X(2, 256) X(2, 320) X(4, 256) X(4, 384) X(4, 512) X(4, 640) X(8, 512)         \
+    X(8, 768) X(8, 1024) X(16, 512) X(16, 1024) X(32, 512) X(32, 1024) X(64, 256) X(64, 512) X(64, 640)          \
+    X(64, 704) X(64, 1024)
+
+template<int G, int B>
+static hipError_t launch_gb(const FwdParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    auto *fn = forward_kernel<G, B>;
+    hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, s, p);
+    return hipGetLastError();
+}
+
+int forward_block_sizes(uint32_t G, uint32_t *out, int cap)
+{
+    int n = 0;
+#define X(g, b)                                                                                                   \
+    if (G == g && n < cap) out[n++] = b;
+    TWISTY_FWD_INSTANCES(X)
+#undef X
+    return n;
+}
+
+hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+      hipStream_t s)
+{
+#define X(g, b)                                                                                                   \
+    if (G == g && block == b) return launch_gb<g, b>(p, grid, lds, s);
+    TWISTY_FWD_INSTANCES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_finalize(const BlockPartial *parts, uint32_t n, double absorbLog2PerPath, FinalOut *out,
+      hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, parts, n, absorbLog2PerPath, out);
+    return hipGetLastError();
+}
+
+}  // namespace twisty_hip

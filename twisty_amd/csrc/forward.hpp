@@ -1,0 +1,72 @@
+// forward.hpp -- kernel parameter blocks and launchers shared by the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace twisty_hip {
+
+struct FwdParams {
+    const float *initSeg;  // N segment vectors p_{i+1} - p_i, x,y,z floats
+    const float *table;    // tableSize floats (device)
+    BlockPartial *partials;
+    float *perPathLog10;   // nullable, numChains*chainLength floats
+    float *perJointCurv;   // nullable, numChains*chainLength*(N-1) floats
+    uint64_t seed;
+    uint64_t pathOffset;
+    uint64_t numChains;    // ceil(numPaths / chainLength)
+    uint64_t numPaths;
+    uint32_t chainLength;
+    uint32_t N;
+    uint32_t K;
+    uint32_t maxDiff;
+    uint32_t stream;
+    uint32_t S;            // segment slots per lane = ceil(N / G)
+    uint32_t scratchRow;   // index of the lane's zero row: max(S, window - 1)
+    int32_t tableSize;
+    uint32_t tableStride;  // floats reserved for the table in LDS (multiple of 64)
+    float minK, maxK, step, invDs;
+    float log10of2;
+    float absorbLog10PerPath;
+};
+
+struct FinalOut {
+    double log10SumW;
+    double log10SumW2;
+    double maxLog10W;
+    unsigned long long nValid, nInvalid, nClamped;
+};
+
+// Block sizes compiled for G lanes per path (ascending); returns the count.
+int forward_block_sizes(uint32_t G, uint32_t *out, int cap);
+
+// Sub-chain window slots per lane for G lanes per path (diff <= 25).
+inline uint32_t forward_window(uint32_t G) { return (25u + G - 1u) / G; }
+
+// Row index of the per-lane zero row: at least S (past the segments) and at
+// least window-1, so the redirected addresses scr - w*SLOT stay >= 0.
+inline uint32_t forward_scratch_row(uint32_t G, uint32_t N)
+{
+    const uint32_t S = (N + G - 1) / G, w = forward_window(G);
+    return S > w - 1 ? S : w - 1;
+}
+
+// Dynamic LDS bytes of the forward kernel: segment rows + zero row, table,
+// draw records (G > 1); at least the block reduction's scratch.
+inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride)
+{
+    const uint64_t rows = (uint64_t)forward_scratch_row(G, N) + 1;
+    const uint64_t floats = rows * 3ull * block + (uint64_t)tableStride + (G > 1 ? 4ull * block : 0ull);
+    const uint64_t reduceBytes = (uint64_t)(block / 64) * sizeof(LseAcc);
+    const uint64_t bytes = floats * 4ull;
+    return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);
+}
+
+hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+      hipStream_t s);
+hipError_t launch_finalize(const BlockPartial *parts, uint32_t n, double absorbLog2PerPath, FinalOut *out,
+      hipStream_t s);
+
+}  // namespace twisty_hip

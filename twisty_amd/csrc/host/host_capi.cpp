@@ -1,0 +1,115 @@
+// host_capi.cpp -- C ABI of libtwisty_host.so (include/twisty_host.h).
+#include <cstring>
+#include <stdexcept>
+
+#include "../../../include/twisty_host.h"
+#include "twisty_host.hpp"
+
+using namespace twisty;
+
+namespace {
+PerturbUtils::BoundaryConditions from13(const float b[13])
+{
+    PerturbUtils::BoundaryConditions bc;
+    bc.m_startPos = Vec3(b[0], b[1], b[2]);
+    bc.m_startDir = Vec3(b[3], b[4], b[5]);
+    bc.m_endPos = Vec3(b[6], b[7], b[8]);
+    bc.m_endDir = Vec3(b[9], b[10], b[11]);
+    bc.arclength = b[12];
+    return bc;
+}
+
+void put_error(char *buf, uint32_t len, const std::string &msg)
+{
+    if (!buf || len == 0) return;
+    const size_t n = std::min<size_t>(len - 1, msg.size());
+    std::memcpy(buf, msg.data(), n);
+    buf[n] = '\0';
+}
+}  // namespace
+
+extern "C" {
+
+int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorption, float bias,
+      uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound, float maxBound, float ds, int threads,
+      float *tableOut, float *metaOut)
+{
+    if (!tableOut || !metaOut || numCurvatureSteps < 2 || numStepsInt < 4 || numStepsInt % 2 != 0 || !(ds > 0.0f))
+        return 1;
+    WeightingParameters wp;
+    wp.mu = mu;
+    wp.eps = eps;
+    wp.scatter = scatter;
+    wp.absorption = absorption;
+    wp.bias = bias;
+    wp.numStepsInt = numStepsInt;
+    wp.numCurvatureSteps = numCurvatureSteps;
+    wp.minBound = minBound;
+    wp.maxBound = maxBound;
+    try {
+        PathWeighting::WeightLookupTableIntegral t(wp, ds);
+        t.Initialize(threads);
+        const auto &v = t.AccessLookupTable();
+        std::memcpy(tableOut, v.data(), v.size() * sizeof(float));
+        const float meta[6] = { t.GetDs(), t.GetMinCurvature(), t.GetMaxCurvature(), t.GetCurvatureStepSize(),
+            t.GetMinSegmentWeight(), t.GetMaxSegmentWeight() };
+        std::memcpy(metaOut, meta, sizeof(meta));
+    } catch (const std::exception &) {
+        return 1;
+    }
+    return 0;
+}
+
+int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut)
+{
+    if (!boundary || !positionsOut) return 1;
+    std::vector<Vec3> pos;
+    if (!PathGeneration::InitialArc(from13(boundary), numSegments, pos)) return 1;
+    std::memcpy(positionsOut, pos.data(), pos.size() * sizeof(Vec3));
+    return 0;
+}
+
+float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegments)
+{
+    return PathGeneration::CalculateMinimumArclength(from13(boundary), numSegments);
+}
+
+double twisty_host_norm_log10(uint32_t numSegments, float ds, const float boundary[13])
+{
+    return PathWeighting::NormLog10((int)numSegments, ds, from13(boundary));
+}
+
+int twisty_host_run_config(const char *jsonText, twisty_host_result *out, char *errorOut, uint32_t errorLen)
+{
+    if (!jsonText || !out) return 1;
+    try {
+        const Json cfg = Json::Parse(jsonText);
+        ExperimentRunner::ExperimentParameters ep = ExperimentRunner::ParseExperimentParamsFromConfig(cfg);
+        PerturbUtils::BoundaryConditions bc = ParseBasicExperiment(cfg);
+        ep.arclength = bc.arclength;
+        ep.perExperimentDirSubfolder = "main";
+        FullExperimentRunnerOptimalPerturb_HIP runner(ep);
+        const auto res = runner.RunExperiment(bc);
+        if (!res) {
+            put_error(errorOut, errorLen, runner.LastError());
+            return 1;
+        }
+        out->log10_experiment_weight = res->experimentWeight.log10;
+        out->log10_avg_path_weight = (res->experimentWeight / (double)res->totalPathsGenerated).log10;
+        out->log10_sum_squares = res->log10SumSquares;
+        out->total_paths = res->totalPathsGenerated;
+        out->failed_paths = res->numFailedPaths;
+        out->clamped_joints = res->numClamped;
+        out->setup_ms = (double)res->setupExperimentMs;
+        out->perturb_ms = (double)res->perturbExperimentMs;
+        out->weighting_ms = (double)res->weightingExperimentMs;
+        out->total_ms = (double)res->totalExperimentMs;
+        out->kernel_ms = res->kernelMs;
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+// runner.cpp -- ExperimentRunner and the HIP runner behind it.
+#include <cstdio>
+#include <filesystem>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+#include "../../../include/twisty_hip.h"
+#include "twisty_host.hpp"
+
+namespace twisty {
+
+std::string BigWeight::ToString(int digits) const
+{
+    if (std::isinf(log10) && log10 < 0) return "0";
+    if (std::isnan(log10)) return "nan";
+    const double e = std::floor(log10);
+    const double m = std::pow(10.0, log10 - e);
+    std::ostringstream os;
+    os << std::setprecision(digits) << m << "e" << (e >= 0 ? "+" : "") << (long long)e;
+    return os.str();
+}
+
+namespace {
+const Json &params_of(const Json &cfg) { return cfg.at("experiment").at("experimentParams"); }
+
+std::string time_for_filename()
+{
+    std::time_t t = std::time(nullptr);
+    std::tm tmv {};
+    localtime_r(&t, &tmv);
+    char buf[64];
+    std::strftime(buf, sizeof(buf), "%F_%H-%M-%S", &tmv);
+    return buf;
+}
+}  // namespace
+
+// ExperimentRunner.cpp:48-177 (field names and derived values as there:
+// maxBound = 10 / eps, absorption forced to 0 for the simplified model,
+// seeds read as ints).
+ExperimentRunner::ExperimentParameters ExperimentRunner::ParseExperimentParamsFromConfig(const Json &cfg)
+{
+    ExperimentParameters ep;
+    const Json &p = params_of(cfg);
+    ep.numPathsInExperiment = (uint64_t)p.at("pathsToGenerate").number();
+    ep.numPathsToSkip = (uint64_t)p.at("pathsToSkip").number();
+    ep.experimentName = p.at("name").string();
+    ep.experimentDirPath = p.at("experimentDir").string() + "/" + ep.experimentName + "/" + time_for_filename() + "/";
+    ep.maxPerturbThreads = (int32_t)p.at("maxPerturbThreads").number();
+    ep.maxWeightThreads = (int32_t)p.at("maxWeightThreads").number();
+    ep.outputBigFloatWeights = p.at("outputBigFloatWeights").boolean();
+    ep.outputPathBatches = p.at("outputPathBatches").boolean();
+    ep.useGpu = p.at("useGpu").boolean();
+    ep.numSegmentsPerCurve = (uint32_t)p.at("numSegments").number();
+    ep.bootstrapSeed = (uint32_t)(int32_t)p.at("random").at("bootstrapSeed").number();
+    ep.curvePurturbSeed = (uint32_t)(int32_t)p.at("random").at("perturbSeed").number();
+    const Json &w = p.at("weighting");
+    ep.weightingParameters.weightingMethod =
+          ((int)w.at("weightFunction").number() == 1) ? WeightingMethod::SimplifiedModel : WeightingMethod::RadiativeTransfer;
+    const int pm = (int)p.at("perturbMethod").number();
+    ep.perturbMethod = pm == 1 ? PerturbMethod::GeometricMinCurvature
+                     : pm == 2 ? PerturbMethod::GeometricCombined
+                               : PerturbMethod::GeometricRandom;
+    ep.weightingParameters.mu = (float)w.at("mu").number();
+    ep.weightingParameters.eps = (float)w.at("eps").number();
+    ep.weightingParameters.numStepsInt = (uint32_t)w.at("numStepsInt").number();
+    ep.weightingParameters.numCurvatureSteps = (uint32_t)w.at("numCurvatureSteps").number();
+    ep.weightingParameters.absorption = (float)w.at("absorption").number();
+    if (ep.weightingParameters.weightingMethod == WeightingMethod::SimplifiedModel)
+        ep.weightingParameters.absorption = 0.0f;
+    ep.weightingParameters.scatter = (float)w.at("scatter").number();
+    if (w.contains("bias")) ep.weightingParameters.bias = (float)w.at("bias").number();
+    ep.weightingParameters.minBound = 0.0f;
+    ep.weightingParameters.maxBound = 10.0f / ep.weightingParameters.eps;
+    if (p.contains("twisty_hip")) {
+        const Json &h = p.at("twisty_hip");
+        if (h.contains("perturbationsPerPath")) ep.perturbationsPerPath = (uint32_t)h.at("perturbationsPerPath").number();
+        if (h.contains("chainLength")) ep.chainLength = (uint32_t)h.at("chainLength").number();
+        if (h.contains("device")) ep.device = (int)h.at("device").number();
+        if (h.contains("writeFiles")) ep.writeFiles = h.at("writeFiles").boolean();
+    }
+    return ep;
+}
+
+// Experiments/FullExperiment.cpp:162-191 (directions normalised there).
+PerturbUtils::BoundaryConditions ParseBasicExperiment(const Json &cfg)
+{
+    const Json &b = cfg.at("experiment").at("basicExperiment");
+    auto v3 = [&](const char *k) {
+        const Json &a = b.at(k);
+        return Vec3((float)a.at(0).number(), (float)a.at(1).number(), (float)a.at(2).number());
+    };
+    PerturbUtils::BoundaryConditions bc;
+    bc.m_startPos = v3("startPos");
+    bc.m_startDir = v3("startDir").Normalized();
+    bc.m_endPos = v3("endPos");
+    bc.m_endDir = v3("endDir").Normalized();
+    bc.arclength = (float)b.at("arclength").number();
+    return bc;
+}
+
+ExperimentRunner::ExperimentRunner(ExperimentParameters &experimentParams)
+    : m_experimentParams(experimentParams), m_experimentDirPath(experimentParams.experimentDirPath)
+{
+    if (!experimentParams.perExperimentDirSubfolder.empty())
+        m_experimentDirPath += "/" + experimentParams.perExperimentDirSubfolder;
+    if (experimentParams.writeFiles) std::filesystem::create_directories(m_experimentDirPath);
+}
+
+ExperimentRunner::~ExperimentRunner() = default;
+
+// ExperimentRunner.cpp:196-301 without the path-batch export.
+std::optional<ExperimentRunner::ExperimentResults> ExperimentRunner::RunExperiment(
+      PerturbUtils::BoundaryConditions boundaryConditions)
+{
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    RunnerSpecificResults r = RunnerSpecificRunExperiment(boundaryConditions);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    if (!r.experimentResults) return std::nullopt;
+    r.experimentResults->totalExperimentMs = std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+    r.experimentResults->setupExperimentMs = r.setupMs;
+    r.experimentResults->perturbExperimentMs = r.runExperimentMs;
+    r.experimentResults->weightingExperimentMs = r.weightingMs;
+    return r.experimentResults;
+}
+
+FullExperimentRunnerOptimalPerturb_HIP::FullExperimentRunnerOptimalPerturb_HIP(ExperimentParameters &ep)
+    : ExperimentRunner(ep)
+{
+}
+
+FullExperimentRunnerOptimalPerturb_HIP::~FullExperimentRunnerOptimalPerturb_HIP() = default;
+
+// The body of FullExperimentRunnerOptimalPerturb::RunnerSpecificRunExperiment
+// (FullExperimentRunnerOptimalPerturb.cpp:48-435) with the per-thread chains
+// replaced by one twisty_hip_run: build the weight table (initialised, unlike
+// :59-69 which never calls Initialize), the initial curve and the path
+// normaliser, then run every path on the device.
+ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::RunnerSpecificRunExperiment(
+      PerturbUtils::BoundaryConditions bc)
+{
+    RunnerSpecificResults out;
+    m_lastError.clear();
+    auto fail = [&](const std::string &msg) {
+        m_lastError = msg;
+        std::cerr << "FullExperimentRunnerOptimalPerturb_HIP: " << msg << std::endl;
+        return out;
+    };
+    ExperimentParameters &ep = m_experimentParams;
+    const auto setupStart = std::chrono::high_resolution_clock::now();
+    if (!(bc.arclength > 0.0f)) return fail("arclength must be positive");
+    if (ep.weightingParameters.weightingMethod != WeightingMethod::RadiativeTransfer)
+        return fail("only the radiative-transfer weighting (weightFunction 0) runs on the HIP kernels");
+    if (ep.perturbMethod != PerturbMethod::GeometricRandom)
+        return fail("Invalid Perturb Method");  // FullExperimentRunnerOptimalPerturb.cpp:320-325
+    if (ep.numPathsToSkip != 0) return fail("pathsToSkip (burn-in) is not supported by the HIP runner");
+    const uint32_t N = ep.numSegmentsPerCurve;
+    if (N < 4) return fail("numSegments must be >= 4");
+    const float ds = bc.arclength / (float)N;
+
+    PathWeighting::WeightLookupTableIntegral table(ep.weightingParameters, ds);
+    try {
+        table.Initialize(ep.maxWeightThreads > 0 ? ep.maxWeightThreads : 0);
+        if (ep.writeFiles) table.ExportValues(m_experimentDirPath);
+    } catch (const std::exception &e) {
+        return fail(e.what());
+    }
+    std::vector<Vec3> positions;
+    if (!PathGeneration::InitialArc(bc, N, positions))
+        return fail("no initial curve: arclength " + std::to_string(bc.arclength) + " is below the minimum " +
+                    std::to_string(PathGeneration::CalculateMinimumArclength(bc, N)));
+    const double normLog10 = PathWeighting::NormLog10((int)N, ds, bc);
+
+    twisty_hip_ctx *ctx = nullptr;
+    int rc = twisty_hip_create(ep.device, &ctx);
+    if (rc != TWISTY_HIP_OK) return fail("twisty_hip_create failed (device " + std::to_string(ep.device) + ")");
+    struct Guard {
+        twisty_hip_ctx *c;
+        ~Guard() { twisty_hip_destroy(c); }
+    } guard { ctx };
+    const auto &tv = table.AccessLookupTable();
+    rc = twisty_hip_set_tables(ctx, tv.data(), (uint32_t)tv.size(), 1, table.GetMinCurvature(),
+          table.GetMaxCurvature(), table.GetCurvatureStepSize(), table.GetDs(), ep.weightingParameters.absorption);
+    if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    const auto bc13 = PerturbUtils::ToCudaSafe(bc);
+    rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(positions.data()), N, bc13.data());
+    if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    const auto setupEnd = std::chrono::high_resolution_clock::now();
+
+    // Seed 0 means "from the clock" (ExperimentRunner.cpp:40-46).
+    uint64_t seed = ep.curvePurturbSeed;
+    if (seed == 0) seed = (uint64_t)std::time(nullptr);
+    const uint64_t nPaths = ep.numPathsInExperiment;
+    // Chain length 0: as many chains as 65536 groups can run side by side.
+    uint32_t L = ep.chainLength;
+    if (L == 0) L = (uint32_t)std::max<uint64_t>(1, (nPaths + 65535) / 65536);
+    twisty_hip_partials part {};
+    if (nPaths > 0) {
+        rc = twisty_hip_run(ctx, nPaths, 0, ep.perturbationsPerPath, L, seed, &part, nullptr, nullptr);
+        if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    }
+    const auto runEnd = std::chrono::high_resolution_clock::now();
+    float fwdMs = 0.0f, totMs = 0.0f;
+    twisty_hip_last_timing(ctx, &fwdMs, &totMs);
+
+    ExperimentResults res;
+    res.experimentWeight.log10 = part.log10_sum_w + normLog10;
+    res.totalPathsGenerated = nPaths;
+    res.numFailedPaths = part.num_invalid;
+    res.log10SumSquares = part.log10_sum_w2 + 2.0 * normLog10;
+    res.kernelMs = totMs;
+    res.numClamped = part.num_clamped;
+    out.experimentResults = res;
+    out.setupMs = std::chrono::duration_cast<std::chrono::milliseconds>(setupEnd - setupStart).count();
+    out.runExperimentMs = std::chrono::duration_cast<std::chrono::milliseconds>(runEnd - setupEnd).count();
+    out.weightingMs = 0;
+    return out;
+}
+
+}  // namespace twisty

@@ -1,0 +1,289 @@
+// twisty_host.hpp -- C++ host layer: Twisty's runner / weighting / config API
+// in front of libtwisty_hip.so.
+//
+// Class and field names follow the reference so a driver written against
+// Twisty/ExperimentRunner.h, PathWeightUtils.h and CurvePerturbUtils.h reads
+// the same.  Differences, each forced by a dependency the reference fetches at
+// configure time (FMath, Boost.Multiprecision, nlohmann::json):
+//   * Farlor::Vector3 -> twisty::Vec3 (three floats, same layout);
+//   * boost cpp_dec_float_100 experimentWeight -> twisty::BigWeight, a base-10
+//     logarithm in double with the same printed form;
+//   * nlohmann::json -> twisty::Json, a small reader of the same documents.
+#pragma once
+
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+struct twisty_hip_ctx;
+
+namespace twisty {
+
+// ---------------------------------------------------------------------------
+// Small vector (Farlor::Vector3 stand-in: three floats).
+// ---------------------------------------------------------------------------
+struct Vec3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    Vec3() = default;
+    Vec3(float a, float b, float c) : x(a), y(b), z(c) { }
+    Vec3 operator+(const Vec3 &o) const { return { x + o.x, y + o.y, z + o.z }; }
+    Vec3 operator-(const Vec3 &o) const { return { x - o.x, y - o.y, z - o.z }; }
+    Vec3 operator*(float s) const { return { x * s, y * s, z * s }; }
+    float Dot(const Vec3 &o) const { return x * o.x + y * o.y + z * o.z; }
+    float Magnitude() const { return std::sqrt(Dot(*this)); }
+    Vec3 Normalized() const
+    {
+        const float m = Magnitude();
+        return { x / m, y / m, z / m };
+    }
+};
+
+namespace PerturbUtils {
+// CurvePerturbUtils.h:62-68.
+struct BoundaryConditions {
+    Vec3 m_startPos = Vec3(0.0f, 0.0f, 0.0f);
+    Vec3 m_startDir = Vec3(1.0f, 0.0f, 0.0f);
+    Vec3 m_endPos = Vec3(0.0f, 0.0f, 0.0f);
+    Vec3 m_endDir = Vec3(1.0f, 0.0f, 0.0f);
+    float arclength = 0.0f;
+};
+// BoundaryConditions_CudaSafe layout (CurvePerturbUtils.h:70-76).
+std::array<float, 13> ToCudaSafe(const BoundaryConditions &bc);
+}  // namespace PerturbUtils
+
+// ---------------------------------------------------------------------------
+// Weighting (PathWeightUtils.h:33-107).
+// ---------------------------------------------------------------------------
+enum class WeightingMethod : int32_t { RadiativeTransfer = 0, SimplifiedModel = 1 };
+
+struct WeightingParameters {
+    float mu = 0.1f;
+    uint32_t numStepsInt = 2000;
+    float minBound = 0.0f;
+    float maxBound = 100.0f;
+    float eps = 0.01f;
+    float bias = 1.0f;
+    float scatter = 0.0f;
+    float absorption = 0.0f;
+    uint32_t numCurvatureSteps = 10000;
+    WeightingMethod weightingMethod = WeightingMethod::RadiativeTransfer;
+    // Same field order and hex encoding as the reference's UUID string
+    // (PathWeightUtils.h:50-106); the hash is FNV-1a, not std::hash.
+    std::pair<std::string, uint64_t> GenerateStringUUID() const;
+};
+
+namespace PathWeighting {
+
+float GaussianPhase(float evalLocation, float mu);
+double IntegrandRT(double p, double kds, double bds, double eps, double mu);
+
+// PathWeightUtils.h:118-165.
+class BaseWeightLookupTable {
+   public:
+    BaseWeightLookupTable(const WeightingParameters &wp, float ds, float minCurvature, float maxCurvature);
+    virtual ~BaseWeightLookupTable() = default;
+    virtual void Initialize() = 0;
+
+    const std::vector<float> &AccessLookupTable() const { return m_lookupTable; }
+    float GetMinSegmentWeight() const { return m_minSegmentWeight; }
+    float GetMaxSegmentWeight() const { return m_maxSegmentWeight; }
+    float GetMinCurvature() const { return m_minCurvature; }
+    float GetMaxCurvature() const { return m_maxCurvature; }
+    float GetDs() const { return m_ds; }
+    float GetCurvatureStepSize() const { return m_curvatureStepSize; }
+    WeightingParameters GetWeightingParams() const { return m_weightingParams; }
+
+    // Binary cache record: table floats then min/max segment weight
+    // (PathWeightUtils.cpp:90-107).
+    void StreamOut(std::ostream &os) const;
+    void InitializeFromStream(std::istream &is);
+    // "curvature, weight" CSV named ds_0x<hex ds>_<hash>_<ExportFilename()>
+    // (PathWeightUtils.cpp:55-88).
+    void ExportValues(const std::string &directory) const;
+    virtual std::string ExportFilename() const = 0;
+    std::string UUID() const;
+
+   protected:
+    WeightingParameters m_weightingParams;
+    float m_ds = 0.0f;
+    float m_minCurvature = 0.0f;
+    float m_maxCurvature = 0.0f;
+    float m_curvatureStepSize = 0.0f;
+    float m_minSegmentWeight = 0.0f;
+    float m_maxSegmentWeight = 0.0f;
+    std::vector<float> m_lookupTable;
+};
+
+// WeightLookupTableIntegral (PathWeightUtils.h:195-207, .cpp:281-433): FP64
+// Simpson integral of the radiative-transfer integrand per curvature bin.
+class WeightLookupTableIntegral : public BaseWeightLookupTable {
+   public:
+    WeightLookupTableIntegral(const WeightingParameters &wp, float ds);
+    void Initialize() override;  // all hardware threads
+    void Initialize(int threads);
+    std::string ExportFilename() const override { return "RT.csv"; }
+    static float Integrate(float curvature, const WeightingParameters &wp, float ds, std::vector<double> &scratch);
+};
+
+// SimpleWeightLookupTable (PathWeightUtils.h:210-218, .cpp:436-485).
+class SimpleWeightLookupTable : public BaseWeightLookupTable {
+   public:
+    SimpleWeightLookupTable(const WeightingParameters &wp, float ds);
+    void Initialize() override;
+    std::string ExportFilename() const override { return "Simple.csv"; }
+};
+
+// NormalizerStuff::Norm (PathWeightUtils.cpp:721-752): K4 for N = 4, the nested
+// quadrature for N = 5, 6 and 1.0 otherwise.
+double NormLog10(int numberOfSegments, float ds, const PerturbUtils::BoundaryConditions &bc);
+
+}  // namespace PathWeighting
+
+namespace PathGeneration {
+// PathGeneration.cpp:12-43.
+float CalculateMinimumArclength(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegmentsPerCurve);
+// Initial curve: hinges of Curve::SetBoundaryConditions (Curve.cpp:20-30) and
+// a planar circular arc of N-2 chords of length ds between them.  Returns false
+// when the boundary conditions admit no such arc (L below the minimum).
+bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments, std::vector<Vec3> &positions);
+}  // namespace PathGeneration
+
+// ---------------------------------------------------------------------------
+// Results (ExperimentRunner.h:73-82).  experimentWeight is carried as log10.
+// ---------------------------------------------------------------------------
+struct BigWeight {
+    double log10 = -INFINITY;
+    std::string ToString(int digits = 17) const;  // "m.mmmmeE" scientific form
+    BigWeight operator/(double d) const { return { log10 - std::log10(d) }; }
+};
+
+class Json;
+
+class ExperimentRunner {
+   public:
+    enum class PerturbMethod { GeometricRandom, GeometricMinCurvature, GeometricCombined };
+
+    // ExperimentRunner.h:42-68, plus the settings of this implementation that
+    // the reference fixes at compile time (read from "twisty_hip" in the
+    // config's experimentParams).
+    struct ExperimentParameters {
+        uint64_t numPathsInExperiment = 0;
+        uint64_t numPathsPerBatch = 1000000;
+        uint64_t numPathsToSkip = 0;
+        uint32_t numSegmentsPerCurve = 4;
+        bool allowSaturatePathCount = true;
+        float arclength = 0.0f;
+        uint32_t exportPathBatchSize = 1000;
+        bool exportGeneratedCurves = false;
+        std::string experimentName = "Default_Experiment";
+        std::string experimentDirPath = "./Default_Experiment/";
+        std::string perExperimentDirSubfolder = "";
+        std::string pathBatchPrepend = "";
+        uint32_t bootstrapSeed = 0;
+        uint32_t curvePurturbSeed = 0;
+        WeightingParameters weightingParameters;
+        PerturbMethod perturbMethod = PerturbMethod::GeometricRandom;
+        int32_t maxPerturbThreads = 0;
+        int32_t maxWeightThreads = 0;
+        bool outputBigFloatWeights = false;
+        bool outputPathBatches = false;
+        bool useGpu = true;
+        // twisty_hip settings
+        uint32_t perturbationsPerPath = 1;  // K; the reference applies 1 between weightings
+        uint32_t chainLength = 1000000;     // paths sharing one carried state (reference: 1e6/thread)
+        int device = 0;
+        bool writeFiles = true;             // table CSV, Results-style files
+    };
+
+    struct ExperimentResults {
+        BigWeight experimentWeight;
+        uint64_t totalPathsGenerated = 0;
+        uint64_t numFailedPaths = 0;
+        std::chrono::milliseconds::rep totalExperimentMs = 0;
+        std::chrono::milliseconds::rep setupExperimentMs = 0;
+        std::chrono::milliseconds::rep perturbExperimentMs = 0;
+        std::chrono::milliseconds::rep weightingExperimentMs = 0;
+        // extras: log10 sum of squared weights, kernel time, clamp count
+        double log10SumSquares = -INFINITY;
+        double kernelMs = 0.0;
+        uint64_t numClamped = 0;
+    };
+
+    // ExperimentRunner.cpp:48-177: reads experiment.experimentParams.*.
+    // Unlike the reference, a missing or mistyped field is an error (the
+    // reference prints and returns defaults).
+    static ExperimentParameters ParseExperimentParamsFromConfig(const Json &experimentConfig);
+
+    explicit ExperimentRunner(ExperimentParameters &experimentParams);
+    virtual ~ExperimentRunner();
+    std::optional<ExperimentResults> RunExperiment(PerturbUtils::BoundaryConditions boundaryConditions);
+
+   protected:
+    struct RunnerSpecificResults {
+        std::optional<ExperimentResults> experimentResults;
+        std::chrono::milliseconds::rep experimentRuntimeTotalMs = 0;
+        std::chrono::milliseconds::rep setupMs = 0;
+        std::chrono::milliseconds::rep runExperimentMs = 0;
+        std::chrono::milliseconds::rep weightingMs = 0;
+    };
+    virtual RunnerSpecificResults RunnerSpecificRunExperiment(PerturbUtils::BoundaryConditions boundaryConditions) = 0;
+
+    ExperimentParameters &m_experimentParams;
+    std::string m_experimentDirPath;
+};
+
+// The HIP runner: the drop-in for FullExperimentRunnerOptimalPerturb
+// (FullExperimentRunnerOptimalPerturb.h:26-74) and its CUDA variant
+// (FullExperimentRunnerOptimalPerturbOptimized_GPU.h).
+class FullExperimentRunnerOptimalPerturb_HIP : public ExperimentRunner {
+   public:
+    explicit FullExperimentRunnerOptimalPerturb_HIP(ExperimentParameters &experimentParams);
+    ~FullExperimentRunnerOptimalPerturb_HIP() override;
+    // Last error reported by the HIP library (empty when none).
+    const std::string &LastError() const { return m_lastError; }
+
+   protected:
+    RunnerSpecificResults RunnerSpecificRunExperiment(PerturbUtils::BoundaryConditions boundaryConditions) override;
+
+   private:
+    std::string m_lastError;
+};
+
+// ---------------------------------------------------------------------------
+// Minimal JSON value (objects, arrays, numbers, strings, booleans, null).
+// ---------------------------------------------------------------------------
+class Json {
+   public:
+    enum class Type { Null, Bool, Number, String, Array, Object };
+    Json() = default;
+    static Json Parse(const std::string &text);  // throws std::runtime_error
+    static Json ParseFile(const std::string &path);
+    Type type() const { return m_type; }
+    bool contains(const std::string &key) const;
+    const Json &at(const std::string &key) const;  // throws when missing
+    const Json &at(size_t i) const;
+    size_t size() const;
+    double number() const;
+    bool boolean() const;
+    const std::string &string() const;
+
+   private:
+    Type m_type = Type::Null;
+    bool m_bool = false;
+    double m_number = 0.0;
+    std::string m_string;
+    std::vector<Json> m_array;
+    std::map<std::string, Json> m_object;
+    friend class JsonParser;
+};
+
+// BoundaryConditions from experiment.basicExperiment (Experiments/FullExperiment.cpp:162-191).
+PerturbUtils::BoundaryConditions ParseBasicExperiment(const Json &experimentConfig);
+
+}  // namespace twisty

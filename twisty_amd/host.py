@@ -1,0 +1,108 @@
+"""ctypes binding of libtwisty_host.so (include/twisty_host.h): weight tables,
+initial curve and the ExperimentRunner flow of the C++ host layer."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from . import capi
+
+LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
+EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
+            "twisty_host_norm_log10", "twisty_host_run_config"]
+
+_lib = None
+
+
+class HostResult(C.Structure):
+    _fields_ = [
+        ("log10_experiment_weight", C.c_double),
+        ("log10_avg_path_weight", C.c_double),
+        ("log10_sum_squares", C.c_double),
+        ("total_paths", C.c_uint64),
+        ("failed_paths", C.c_uint64),
+        ("clamped_joints", C.c_uint64),
+        ("setup_ms", C.c_double),
+        ("perturb_ms", C.c_double),
+        ("weighting_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m twisty_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        f32, u32, f64, vp = C.c_float, C.c_uint32, C.c_double, C.c_void_p
+        L.twisty_host_build_rt_table.argtypes = [f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, C.c_int, vp, vp]
+        L.twisty_host_initial_arc.argtypes = [vp, u32, vp]
+        L.twisty_host_minimum_arclength.argtypes = [vp, u32]
+        L.twisty_host_minimum_arclength.restype = f32
+        L.twisty_host_norm_log10.argtypes = [u32, f32, vp]
+        L.twisty_host_norm_log10.restype = f64
+        L.twisty_host_run_config.argtypes = [C.c_char_p, C.POINTER(HostResult), C.c_char_p, u32]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class WeightTable:
+    """A built WeightLookupTableIntegral: table values and its curvature grid."""
+
+    def __init__(self, table, meta):
+        self.table = table
+        self.ds, self.minK, self.maxK, self.step, self.minW, self.maxW = [float(v) for v in meta]
+
+
+def build_rt_table(ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000, numCurvatureSteps=4096,
+                   minBound=0.0, maxBound=None, threads=0):
+    """WeightLookupTableIntegral(params, ds).Initialize(); maxBound defaults to
+    10/eps as ExperimentRunner.cpp:168-171 sets it."""
+    if maxBound is None:
+        maxBound = float(np.float32(10.0) / np.float32(eps))
+    t = np.zeros(numCurvatureSteps, np.float32)
+    meta = np.zeros(6, np.float32)
+    rc = lib().twisty_host_build_rt_table(mu, eps, scatter, absorption, bias, numStepsInt, numCurvatureSteps,
+                                          minBound, maxBound, ds, threads, _p(t), _p(meta))
+    if rc != 0:
+        raise ValueError("twisty_host_build_rt_table rejected its arguments")
+    return WeightTable(t, meta)
+
+
+def boundary13(start, start_dir, end, end_dir, arclength):
+    return np.array(list(start) + list(start_dir) + list(end) + list(end_dir) + [arclength], np.float32)
+
+
+def initial_arc(boundary, N):
+    b = np.ascontiguousarray(boundary, np.float32)
+    out = np.zeros((N + 1) * 3, np.float32)
+    if lib().twisty_host_initial_arc(_p(b), N, _p(out)) != 0:
+        raise ValueError("no initial arc: arclength below the minimum for these boundary conditions")
+    return out.reshape(N + 1, 3)
+
+
+def minimum_arclength(boundary, N):
+    b = np.ascontiguousarray(boundary, np.float32)
+    return lib().twisty_host_minimum_arclength(_p(b), N)
+
+
+def norm_log10(N, ds, boundary):
+    b = np.ascontiguousarray(boundary, np.float32)
+    return lib().twisty_host_norm_log10(N, ds, _p(b))
+
+
+def run_config(config):
+    """Runs a reference-schema JSON config (dict or text) on the HIP runner."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    res = HostResult()
+    err = C.create_string_buffer(1024)
+    if lib().twisty_host_run_config(text.encode(), C.byref(res), err, 1024) != 0:
+        raise RuntimeError(f"run_config failed: {err.value.decode()}")
+    return {k: getattr(res, k) for k, _ in res._fields_}

@@ -64,6 +64,9 @@ def build(asm=False, verbose=True):
     run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
     build_host()
+    calib = os.path.join(LIBDIR, "twisty_copy_calib")
+    run([cc, *HIPCC_FLAGS, "-o", calib + ".tmp", os.path.join(CSRC, "copy_calib.hip")])
+    os.replace(calib + ".tmp", calib)
     return out
 
 

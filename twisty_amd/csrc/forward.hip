@@ -75,14 +75,27 @@ __device__ __forceinline__ DrawRec load_rec(const uint32_t *p)
 // the sub-chain [left, right), sum them across the group into the axis
 // p_right - p_left, build the rotation and write the rotated segments back.
 // Window slots past this lane's count are redirected to the lane's zeroed
-// scratch row (scr = lane's element of that row): they read zeros, add nothing
-// to the axis and write zeros back, so every load and store is unconditional
-// with a compile-time offset.
+// scratch row: they read zeros, add nothing to the axis and write zeros back,
+// so every load and store is unconditional with a compile-time offset.
+// scrB[w] is the byte address that lands on the scratch row after slot w's
+// offset is added.  (Skipping the last slot behind a wave-uniform branch when
+// no path of the wave needs it measured slower: the compiler copies the window
+// registers around the branch.)
+// LDS accesses by byte address.  The dynamic-LDS block starts right after the
+// static LDS, at __builtin_amdgcn_groupstaticsize(); addresses built from it are
+// plain integers, so a slot's row offset stays an instruction immediate.
+typedef __attribute__((address_space(3))) float lds_f32;
+
+__device__ __forceinline__ float lds_ld(uint32_t byteAddr) { return *(lds_f32 *)(size_t)byteAddr; }
+__device__ __forceinline__ void lds_st(uint32_t byteAddr, float v) { *(lds_f32 *)(size_t)byteAddr = v; }
+
 template<int G, int BLOCK>
-__device__ __forceinline__ void perturb_one(float *smem, uint32_t lane, uint32_t j, uint32_t scr, const DrawRec &rec)
+__device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
+      const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
 {
-    constexpr int WMAX = (25 + G - 1) / G;  // sub-chain slots per lane (diff <= 25)
-    constexpr uint32_t SLOT = 3u * BLOCK;   // floats per slot row
+    constexpr int WMAX = (25 + G - 1) / G;       // sub-chain slots per lane (diff <= 25)
+    constexpr uint32_t SLOTB = 12u * BLOCK;      // bytes per slot row
+    constexpr uint32_t COMPB = 4u * BLOCK;       // bytes between components
     const uint32_t diff = rec.packed & 0xFFFFu;
     const uint32_t left = rec.packed >> 16;
     const uint32_t right = left + diff;
@@ -90,15 +103,16 @@ __device__ __forceinline__ void perturb_one(float *smem, uint32_t lane, uint32_t
     // lane's segments lie in [left, right).
     const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
     const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
-    const uint32_t base = (i0 / G) * SLOT + lane;
+    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
     uint32_t addr[WMAX];
     float dx[WMAX], dy[WMAX], dz[WMAX];
 #pragma unroll
     for (int w = 0; w < WMAX; ++w) {
-        addr[w] = ((uint32_t)w < cnt) ? base : scr - (uint32_t)w * SLOT;
-        dx[w] = smem[addr[w] + w * SLOT];
-        dy[w] = smem[addr[w] + w * SLOT + BLOCK];
-        dz[w] = smem[addr[w] + w * SLOT + 2 * BLOCK];
+        addr[w] = ((uint32_t)w < cnt) ? baseB : scrB[w];
+        const uint32_t a = addr[w] + w * SLOTB;
+        dx[w] = lds_ld(a);
+        dy[w] = lds_ld(a + COMPB);
+        dz[w] = lds_ld(a + 2 * COMPB);
     }
     float ax = dx[0], ay = dy[0], az = dz[0];
 #pragma unroll
@@ -118,9 +132,10 @@ __device__ __forceinline__ void perturb_one(float *smem, uint32_t lane, uint32_t
 #pragma unroll
     for (int w = 0; w < WMAX; ++w) {
         const float x = dx[w], y = dy[w], z = dz[w];
-        smem[addr[w] + w * SLOT] = fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z));
-        smem[addr[w] + w * SLOT + BLOCK] = fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z));
-        smem[addr[w] + w * SLOT + 2 * BLOCK] = fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z));
+        const uint32_t a = addr[w] + w * SLOTB;
+        lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
+        lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
+        lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
     }
 }
 
@@ -148,6 +163,12 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
     smem[scr] = 0.0f;
     smem[scr + BLOCK] = 0.0f;
     smem[scr + 2 * BLOCK] = 0.0f;
+    const uint32_t ldsBase = __builtin_amdgcn_groupstaticsize();
+    const uint32_t laneB = ldsBase + lane * 4u;
+    constexpr int WMAX = (25 + G - 1) / G;
+    uint32_t scrB[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) scrB[w] = ldsBase + (scr - (uint32_t)w * SLOT) * 4u;
 
     LseAcc acc;
     lse_init(acc);
@@ -173,7 +194,7 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
             for (uint32_t kb = 0; kb < K; kb += G) {
                 const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
                 if constexpr (G == 1) {
-                    perturb_one<G, BLOCK>(smem, lane, j, scr, pack_draw(make_draw(p.seed, p.stream, path, kb, N, p.maxDiff)));
+                    perturb_one<G, BLOCK>(laneB, j, scrB, pack_draw(make_draw(p.seed, p.stream, path, kb, N, p.maxDiff)));
                 } else {
                     // Each lane draws one perturbation of the batch into the
                     // group's slice of drawbuf.
@@ -192,11 +213,11 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
 #pragma unroll
                         for (int kk = 0; kk < G; ++kk) {
                             if ((uint32_t)kk >= kcount) break;
-                            perturb_one<G, BLOCK>(smem, lane, j, scr, recs[kk]);
+                            perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
                         }
                     } else {
                         for (uint32_t kk = 0; kk < kcount; ++kk)
-                            perturb_one<G, BLOCK>(smem, lane, j, scr, load_rec(drawbuf + (groupBase + kk) * 4));
+                            perturb_one<G, BLOCK>(laneB, j, scrB, load_rec(drawbuf + (groupBase + kk) * 4));
                         wave_sync_lds();
                     }
                 }

@@ -115,8 +115,49 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
       uint32_t *blocksPerCU, uint32_t *ldsBytes);
 
 /* Device time of the last run measured with HIP events on the context's
- * stream: the forward kernel alone and forward + FP64 finalize. */
+ * stream: the forward kernel alone, and the whole call's device work (forward,
+ * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
+
+/* Exit-direction histogram of the following runs: nTheta x nPhi bins of the
+ * last free tangent t_{N-2} in (cos theta, phi) about the end direction of the
+ * boundary conditions (end frame: f1 = the coordinate axis least aligned with
+ * e, made orthogonal to e; f2 = e x f1; phi = atan2(t.f2, t.f1)).  Bin
+ * (it, ip) is it*nPhi + ip.  Each bin holds the share of the estimator's
+ * weight sum carried by its paths (FP32 compensated bin sums, FP64
+ * normaliser), so the bins add up to 1.  0, 0 switches it off.
+ * Build-defined: the reference's only histogram bins weight magnitudes
+ * (FullExperimentRunnerOptimalPerturb.cpp:26-37, 235-246, 394-422). */
+int twisty_hip_set_exit_histogram(twisty_hip_ctx *ctx, uint32_t nTheta, uint32_t nPhi);
+
+/* Copies the histogram of the last run (nTheta*nPhi doubles) and, optionally,
+ * every path's bin index (numPaths must equal the last run's path count). */
+int twisty_hip_get_exit_histogram(twisty_hip_ctx *ctx, double *histogram, uint32_t *perPathBin,
+      uint64_t numPaths);
+
+/* Forward pass plus reverse-mode derivative of the log-estimator
+ * E = log10(sum_p w_p / numPaths) (paths restart from the initial curve,
+ * chainLength 1).  dLog10EstimateDTable (numBins doubles) receives dE/dT[b]
+ * for the combined table; with several tables set, it is also the derivative
+ * for each of them.  dLog10EstimateDSegments (3*N doubles, NULL to skip the
+ * reverse sweep) receives dE/dd_i for the initial segment vectors
+ * d_i = p_{i+1} - p_i; the host maps it onto boundary parameters (exit
+ * direction).  The curvature derivative of the segment weight uses the
+ * bin-average slope (T[r]-T[l])/step of the reference's lerp
+ * (PathWeighters.cpp:67-82).  The reference has no derivative; the call it
+ * extends is RunnerSpecificRunExperiment (ExperimentRunner.h:99-101). */
+int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset,
+      uint32_t perturbationsPerPath, uint64_t seed, twisty_hip_partials *out, double *dLog10EstimateDTable,
+      double *dLog10EstimateDSegments);
+
+/* Launch schedule of the adjoint kernel (same meaning as
+ * twisty_hip_set_schedule; 0 = built-in choice). */
+int twisty_hip_set_adjoint_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
+      uint32_t blocksPerCU);
+
+/* Device time of the adjoint part (kernel + gradient reductions) of the last
+ * twisty_hip_run_adjoint; 0 after a plain run. */
+int twisty_hip_last_adjoint_timing(twisty_hip_ctx *ctx, float *adjointMs);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,22 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
  * them; (numSegments+1)*3 floats.  Returns 1 when no arc exists. */
 int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut);
 
+/* d T[i] / d eps of the table twisty_host_build_rt_table builds, in FP64, for
+ * the config's eps slot (the Gaussian phase width the reference actually uses,
+ * PathWeightUtils.cpp:401-420 with its (mu, eps) swap).  maxBoundFollowsEps != 0
+ * differentiates maxBound = 10/eps as well (ExperimentRunner.cpp:168-171).
+ * The reference has no derivative; this feeds the phase-width gradient. */
+int twisty_host_rt_table_eps_derivative(float mu, float eps, float scatter, float bias, uint32_t numStepsInt,
+      uint32_t numCurvatureSteps, float minBound, float maxBound, int maxBoundFollowsEps, float ds, int threads,
+      double *dTableDEps);
+
+/* d positions / d theta and d phi ((numSegments+1)*3 doubles each) of the
+ * initial curve of twisty_host_initial_arc when the end direction is
+ * (sin theta cos phi, sin theta sin phi, cos theta), at the boundary's own
+ * (normalised) end direction.  Feeds the exit-direction gradient. */
+int twisty_host_initial_arc_exit_jacobian(const float boundary[13], uint32_t numSegments, double *dPosDTheta,
+      double *dPosDPhi);
+
 /* PathGeneration::CalculateMinimumArclength (PathGeneration.cpp:12-43). */
 float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegments);
 

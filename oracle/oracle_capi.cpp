@@ -169,6 +169,21 @@ int oracle_initial_arc(const double bc12[12], float arclength, uint32_t N, doubl
     return 0;
 }
 
+// InitialArc with the bulge direction taken from `bulge` instead of startDir.
+int oracle_initial_arc_bulge(const double bc12[12], float arclength, uint32_t N, const double bulge[3],
+      double *posOut)
+{
+    std::vector<V3<double>> pos;
+    const V3<double> b { bulge[0], bulge[1], bulge[2] };
+    if (!InitialArc(MakeBC(bc12, arclength), N, pos, nullptr, &b)) return 1;
+    for (uint32_t i = 0; i <= N; ++i) {
+        posOut[i * 3 + 0] = pos[i].x;
+        posOut[i * 3 + 1] = pos[i].y;
+        posOut[i * 3 + 2] = pos[i].z;
+    }
+    return 0;
+}
+
 float oracle_minimum_arclength(const double bc12[12], uint32_t N)
 {
     return CalculateMinimumArclength(MakeBC(bc12, 0.0f), N);

@@ -45,6 +45,8 @@ def _declare(L):
     L.oracle_inv_ds.restype = f32
     L.oracle_initial_arc.argtypes = [_p(np.float64), f32, u32, _p(np.float64), C.POINTER(f64)]
     L.oracle_initial_arc.restype = C.c_int
+    L.oracle_initial_arc_bulge.argtypes = [_p(np.float64), f32, u32, _p(np.float64), _p(np.float64)]
+    L.oracle_initial_arc_bulge.restype = C.c_int
     L.oracle_minimum_arclength.argtypes = [_p(np.float64), u32]
     L.oracle_minimum_arclength.restype = f32
     L.oracle_norm.argtypes = [u32, f32, _p(np.float64), f32, u32, u32]
@@ -72,6 +74,9 @@ def _declare(L):
     L.oracle_simulate.restype = C.c_int
     L.oracle_exit_bin.argtypes = [_p(np.float64), _p(np.float64), u32, u32, C.POINTER(u32)]
     L.oracle_end_frame.argtypes = [_p(np.float64), _p(np.float64), _p(np.float64)]
+    L.oracle_adjoint.argtypes = [_p(np.float64), u32, f32, u32, u64, u64, _p(np.float32), i32, f32, f32, f32, f32,
+                                 f32, C.c_int, C.c_int, C.POINTER(f64), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.oracle_adjoint.restype = C.c_int
 
 
 # ---------------------------------------------------------------------------
@@ -102,6 +107,13 @@ def initial_arc(N, arclength, bc=None):
     if rc != 0:
         raise ValueError("infeasible initial arc")
     return pos.reshape(N + 1, 3), alpha.value
+
+
+def initial_arc_bulge(N, arclength, bc, bulge):
+    pos = np.zeros((N + 1) * 3, np.float64)
+    if lib().oracle_initial_arc_bulge(np.asarray(bc, np.float64), arclength, N, np.asarray(bulge, np.float64), pos):
+        raise ValueError("infeasible initial arc")
+    return pos.reshape(N + 1, 3)
 
 
 def table_range(nbins, ds):
@@ -162,4 +174,44 @@ def simulate(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, stre
         out["curv"] = cv.reshape(n_paths, N - 1)
     if want_exit:
         out["exit"] = ex.reshape(n_paths, 3)
+    return out
+
+
+def adjoint(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, lerp_mode=0, want_table=True,
+            want_seg=True, want_w=False, threads=None, table_values=None):
+    """FP64 estimator E = log10(sum w / n_paths) and its derivatives (oracle_adjoint.cpp).
+
+    table_values overrides table.table (for finite differences in the entries).
+    Returns dict: log10_sum_w, d_table (bins), d_seg (N, 3), log10w (paths).
+    """
+    threads = threads or os.cpu_count() or 1
+    init = np.ascontiguousarray(init_pos, np.float64).reshape(-1)
+    tv = np.ascontiguousarray(table.table if table_values is None else table_values, np.float32)
+    out = C.c_double()
+    dT = np.zeros(len(tv), np.float64) if want_table else None
+    dS = np.zeros(3 * N, np.float64) if want_seg else None
+    w = np.zeros(n_paths, np.float64) if want_w else None
+    ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    rc = lib().oracle_adjoint(init, N, arclength, K, seed, n_paths, tv, len(tv), table.ds, table.minK, table.maxK,
+                              table.step, absorption, lerp_mode, threads, C.byref(out), ptr(dT), ptr(dS), ptr(w))
+    if rc != 0:
+        raise ValueError("oracle_adjoint rejected its arguments")
+    res = {"log10_sum_w": out.value}
+    if want_table:
+        res["d_table"] = dT
+    if want_seg:
+        res["d_seg"] = dS.reshape(N, 3)
+    if want_w:
+        res["log10w"] = w
+    return res
+
+
+def exit_bins(tangents, e, n_theta, n_phi):
+    """Exit-direction bin of each FP64 tangent (rows of `tangents`)."""
+    e = np.ascontiguousarray(e, np.float64)
+    out = np.zeros(len(tangents), np.uint32)
+    b = C.c_uint32()
+    for i, t in enumerate(np.ascontiguousarray(tangents, np.float64)):
+        lib().oracle_exit_bin(np.ascontiguousarray(t), e, n_theta, n_phi, C.byref(b))
+        out[i] = b.value
     return out

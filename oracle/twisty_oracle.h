@@ -608,7 +608,7 @@ inline V3<double> LeastAlignedAxis(V3<double> u)
 // Returns false when the arc does not exist (c > M ds or c == 0).
 // ---------------------------------------------------------------------------
 inline bool InitialArc(const BoundaryConditions &bc, uint32_t N, std::vector<V3<double>> &pos,
-      double *turningAngle = nullptr)
+      double *turningAngle = nullptr, const V3<double> *bulge = nullptr)
 {
     if (N < 3) return false;
     const double ds = (double)(bc.arclength / (float)N);
@@ -628,7 +628,10 @@ inline bool InitialArc(const BoundaryConditions &bc, uint32_t N, std::vector<V3<
     if (c > (double)M * ds * (1.0 + 1e-12)) return false;
     const V3<double> u = scale(chordVec, 1.0 / c);
     // bulge direction
-    V3<double> v = sub(sd, scale(u, dot(sd, u)));
+    // (a given bulge direction replaces startDir here: the derivative in the end
+    // direction is taken with the bulge held fixed when startDir lies on the chord)
+    const V3<double> bsrc = bulge ? *bulge : sd;
+    V3<double> v = sub(bsrc, scale(u, dot(bsrc, u)));
     if (norm(v) < 1e-9) {
         const V3<double> axis = LeastAlignedAxis(u);
         v = sub(axis, scale(u, dot(axis, u)));

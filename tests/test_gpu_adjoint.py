@@ -1,0 +1,163 @@
+"""GPU adjoint (twisty_hip_run_adjoint) against the FP64 oracle.
+
+The oracle adjoint (oracle/oracle_adjoint.cpp, pinned by central differences in
+test_gradients_cpu) computes the same derivative in FP64 with the reference's
+lookup: dE/dT exactly, dE/d(initial segments) with the bin-average slope.
+Tolerances, FP32 kernel vs FP64 oracle on the same draws:
+  * table gradient: ||g - r|| <= 2e-3 ||r|| and, along random directions,
+    |g.v - r.v| <= 2e-3 |r.v|.  Per-path weights agree to 1e-5 in log10 (about
+    1e-3 relative in weight at log10 w ~ -40), and a joint whose FP32 curvature
+    falls into the neighbouring bin moves its share to the next entry;
+  * segment gradient: ||g - r|| <= 5e-3 ||r|| (the same, plus the FP32 rewind
+    of K rotations);
+  * the table gradient also against central differences of the oracle's
+    estimator in the reference lookup, where E is exactly a function of the
+    table entries;
+  * sigma and the exit angles through torch.autograd against the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from twisty_amd import capi, host
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_norm(g, r):
+    return np.linalg.norm(g - r) / np.linalg.norm(r)
+
+
+def oracle_adjoint(case, oracle, paths, table=None, pos=None, K=None, mode=0):
+    t, pos32 = cases.setup(case, oracle)
+    t = t if table is None else table
+    pos = pos32.astype(np.float64) if pos is None else pos
+    return oracle.adjoint(pos, case.N, case.L, case.K if K is None else K, paths, t, case.absorption,
+                          seed=case.seed, lerp_mode=mode)
+
+
+@pytest.mark.parametrize("case,paths", [(cases.C4, 1 << 14), (cases.C1, 1 << 14), (cases.MID, 1 << 13),
+                                        (cases.SMALL, 1 << 12)])
+def test_adjoint_matches_oracle(hip_device, oracle, case, paths):
+    print(f"[adjoint {case.name} paths={paths}]", flush=True)
+    ctx = cases.hip_context(case, oracle)
+    g = ctx.run_adjoint(paths, case.K, seed=case.seed)
+    r = oracle_adjoint(case, oracle, paths)
+    e_t, e_s = rel_norm(g["d_table"], r["d_table"]), rel_norm(g["d_seg"], r["d_seg"])
+    print(f"  |dT| rel {e_t:.2e}  |dSeg| rel {e_s:.2e}  fwd {ctx.last_timing()[0]:.3f} ms "
+          f"adj {ctx.last_adjoint_ms():.3f} ms", flush=True)
+    assert abs(g["log10_sum_w"] - r["log10_sum_w"]) < 1e-3 * abs(r["log10_sum_w"])
+    assert e_t <= 2e-3
+    assert e_s <= 5e-3
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        v = rng.standard_normal(len(g["d_table"]))
+        assert abs(g["d_table"] @ v - r["d_table"] @ v) <= 2e-3 * abs(r["d_table"] @ v) + 1e-12
+    ctx.close()
+
+
+def test_adjoint_table_gradient_vs_oracle_fd(hip_device, oracle):
+    case, paths, h = cases.C4, 1 << 13, 1e-3
+    ctx = cases.hip_context(case, oracle)
+    g = ctx.run_adjoint(paths, case.K, seed=case.seed, segments=False)
+    table, pos32 = cases.setup(case, oracle)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(len(table.table)) * table.table
+    tp = (table.table + h * v).astype(np.float32)
+    tm = (table.table - h * v).astype(np.float32)
+
+    def est(tv):
+        return oracle.adjoint(pos32.astype(np.float64), case.N, case.L, case.K, paths, table, case.absorption,
+                              seed=case.seed, lerp_mode=0, want_table=False, want_seg=False,
+                              table_values=tv)["log10_sum_w"]
+
+    fd = est(tp) - est(tm)
+    an = g["d_table"] @ (tp.astype(np.float64) - tm.astype(np.float64))
+    print(f"[adjoint FD] fd={fd:.6e} adjoint={an:.6e}", flush=True)
+    assert abs(fd - an) <= 3e-3 * abs(fd)
+    ctx.close()
+
+
+@pytest.mark.parametrize("sched", [(4, 256, 0), (8, 512, 0), (16, 1024, 0), (64, 512, 0)])
+def test_adjoint_schedules_agree(hip_device, oracle, sched):
+    case, paths = cases.C2, 1 << 13
+    ctx = cases.hip_context(case, oracle)
+    base = ctx.run_adjoint(paths, case.K, seed=case.seed)
+    ctx.set_adjoint_schedule(*sched)
+    other = ctx.run_adjoint(paths, case.K, seed=case.seed)
+    print(f"[adjoint schedule {sched}] dT {rel_norm(other['d_table'], base['d_table']):.2e} "
+          f"dSeg {rel_norm(other['d_seg'], base['d_seg']):.2e}", flush=True)
+    assert rel_norm(other["d_table"], base["d_table"]) < 1e-5
+    assert rel_norm(other["d_seg"], base["d_seg"]) < 1e-4
+    no_sweep = ctx.run_adjoint(paths, case.K, seed=case.seed, segments=False)
+    assert rel_norm(no_sweep["d_table"], base["d_table"]) < 1e-5
+    ctx.close()
+
+
+def test_adjoint_large_n(hip_device, oracle):
+    case, paths = cases.C5, 1 << 10
+    print(f"[adjoint {case.name} paths={paths}]", flush=True)
+    ctx = cases.hip_context(case, oracle)
+    g = ctx.run_adjoint(paths, case.K, seed=case.seed)
+    r = oracle_adjoint(case, oracle, paths)
+    e_t, e_s = rel_norm(g["d_table"], r["d_table"]), rel_norm(g["d_seg"], r["d_seg"])
+    print(f"  |dT| rel {e_t:.2e}  |dSeg| rel {e_s:.2e}", flush=True)
+    # log10 w ~ -600 here: 1e-5 relative per path is ~1% in weight
+    assert e_t <= 5e-2 and e_s <= 5e-2
+    ctx.close()
+
+
+def test_autograd_sigma_and_exit_angles(hip_device, oracle):
+    """C4 geometry: d E / d sigma and d E / d(theta, phi) of the exit direction."""
+    import torch
+
+    from twisty_amd import autograd as ag
+
+    case, paths = cases.C4, 1 << 13
+    N, L, K = case.N, case.L, case.K
+    ds = cases.ds_of(case)
+    for edir, bulge in (((0.0, 0.0, 1.0), (1.0, 0.0, 0.0)), ((0.6, 0.0, 0.8), None)):
+        bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), edir, L)
+        th0, ph0 = math.acos(edir[2]), math.atan2(edir[1], edir[0])
+        sigma = torch.tensor(case.sigma, dtype=torch.float64, requires_grad=True)
+        ang = torch.tensor([th0, ph0], dtype=torch.float64, requires_grad=True)
+        hctx = capi.Context(0)
+        table, grid = ag.rt_table_with_grid(sigma, ds, numStepsInt=case.numStepsInt, bins=case.bins)
+        E = ag.log10_estimate(table, ag.exit_curve(ang, bc, N), hctx, grid, bc, paths, K, seed=case.seed)
+        E.backward()
+        hctx.close()
+
+        def arc(t_, p_):
+            e = (math.sin(t_) * math.cos(p_), math.sin(t_) * math.sin(p_), math.cos(t_))
+            if bulge is not None:
+                return oracle.initial_arc_bulge(N, L, oracle.bc12(edir=e), bulge)
+            return oracle.initial_arc(N, L, oracle.bc12(edir=e))[0]
+
+        def otable(s):
+            return oracle.build_table(ds, eps=float(np.float32(s) ** 2), numCurvatureSteps=case.bins,
+                                      numStepsInt=case.numStepsInt)
+
+        pos0 = host.initial_arc(bc, N).astype(np.float64)
+        # sigma: central difference of the oracle estimator over rebuilt tables
+        hs = 1e-3 * case.sigma
+
+        def est_sigma(s):
+            return oracle.adjoint(pos0, N, L, K, paths, otable(s), case.absorption, seed=case.seed, lerp_mode=0,
+                                  want_table=False, want_seg=False)["log10_sum_w"]
+
+        fd_sigma = (est_sigma(case.sigma + hs) - est_sigma(case.sigma - hs)) / (2 * hs)
+        # exit angles: the oracle adjoint's dE/d(segments) chained with the FP64
+        # Jacobian of the oracle's own arc (central differences)
+        r = oracle.adjoint(pos0, N, L, K, paths, otable(case.sigma), case.absorption, seed=case.seed, lerp_mode=0)
+        h = 1e-6
+        jt = (arc(th0 + h, ph0) - arc(th0 - h, ph0)) / (2 * h)
+        jp = (arc(th0, ph0 + h) - arc(th0, ph0 - h)) / (2 * h)
+        ref_ang = np.array([np.sum(r["d_seg"] * np.diff(jt, axis=0)), np.sum(r["d_seg"] * np.diff(jp, axis=0))])
+        gs, ga = sigma.grad.item(), ang.grad.numpy()
+        print(f"[autograd edir={edir}] dE/dsigma gpu={gs:.6e} fd={fd_sigma:.6e}; dE/dangles gpu={ga} ref={ref_ang}",
+              flush=True)
+        assert abs(gs - fd_sigma) <= 5e-3 * abs(fd_sigma)
+        scale = np.abs(ref_ang).max()
+        assert np.all(np.abs(ga - ref_ang) <= 5e-3 * scale + 1e-9)

@@ -1,0 +1,141 @@
+"""GPU exit-direction histogram against the oracle.
+
+The histogram is build-defined (the reference bins only weight magnitudes):
+bin (it, ip) of the last free tangent t_{N-2}, cos theta = t.e over nTheta bins
+of [-1, 1], phi = atan2(t.f2, t.f1) over nPhi bins of [-pi, pi), each bin the
+share of the weight sum of its paths.
+
+Checks:
+  * per-path bins equal the oracle's bins of the FP64 exit tangent, except for
+    paths whose FP64 coordinate lies within 1e-3 of a bin edge (FP32 tangents
+    differ from FP64 by ~1e-6);
+  * bins: the GPU histogram equals the FP64 sum over the oracle's per-path
+    weights binned with the GPU's own bins, |dh| <= 1e-4 h + 1e-9 (per-path
+    weights agree to 1e-5 relative in log10, about 1e-3 in weight at log10 w
+    ~ -40; most bins hold many paths);
+  * invariants: sum of bins = 1, bins >= 0, a constant table gives the path
+    fraction of every bin.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from twisty_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def frame(e):
+    e = np.asarray(e, np.float64)
+    e = e / np.linalg.norm(e)
+    k = 0
+    if abs(e[1]) < abs(e[0]):
+        k = 1
+    if abs(e[2]) < min(abs(e[0]), abs(e[1])):
+        k = 2
+    a = np.zeros(3)
+    a[k] = 1.0
+    f1 = a - (a @ e) * e
+    f1 /= np.linalg.norm(f1)
+    return e, f1, np.cross(e, f1)
+
+
+def edge_distance(t, e, nt, nph):
+    """Distance (in bins) of each FP64 tangent's coordinates to the nearest bin edge."""
+    e, f1, f2 = frame(e)
+    t = t / np.linalg.norm(t, axis=1, keepdims=True)
+    u = (t @ e + 1.0) * 0.5 * nt
+    v = (np.arctan2(t @ f2, t @ f1) + math.pi) / (2 * math.pi) * nph
+    return np.minimum(np.abs(u - np.round(u)), np.abs(v - np.round(v)))
+
+
+def reference_histogram(log10w, bins, nb):
+    ok = np.isfinite(log10w)
+    w = np.zeros(len(log10w))
+    m = log10w[ok].max()
+    w[ok] = 10.0 ** (log10w[ok] - m)
+    h = np.bincount(bins[ok], weights=w[ok], minlength=nb)
+    return h / h.sum()
+
+
+@pytest.mark.parametrize("case,nt,nph,paths", [(cases.C2, 64, 64, 1 << 15), (cases.MID, 32, 16, 1 << 14),
+                                               (cases.C1, 256, 256, 1 << 15)])
+def test_exit_histogram_matches_oracle(hip_device, oracle, case, nt, nph, paths):
+    print(f"[hist {case.name} {nt}x{nph} paths={paths}]", flush=True)
+    ctx = cases.hip_context(case, oracle)
+    ctx.set_exit_histogram(nt, nph)
+    res = ctx.run(paths, case.K, seed=case.seed)
+    h, gbins = ctx.exit_histogram(paths)
+    table, pos32 = cases.setup(case, oracle)
+    ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, paths, table, case.absorption,
+                          seed=case.seed, want_exit=True)
+    e = (0.0, 0.0, 1.0)
+    obins = oracle.exit_bins(ref["exit"].astype(np.float64), e, nt, nph)
+    near = edge_distance(ref["exit"].astype(np.float64), e, nt, nph) < 1e-3
+    mism = (gbins != obins) & ref["valid"]
+    print(f"  bin mismatches {mism.sum()} (near-edge paths {near.sum()})", flush=True)
+    assert not np.any(mism & ~near)
+    assert h.shape == (nt, nph) and np.all(h >= 0)
+    assert abs(h.sum() - 1.0) < 1e-5
+    href = reference_histogram(ref["log10w"], gbins.astype(np.int64), nt * nph).reshape(nt, nph)
+    err = np.abs(h - href)
+    print(f"  max |dh| {err.max():.3e}, max h {h.max():.3e}", flush=True)
+    assert np.all(err <= 1e-4 * href + 1e-9)
+    ctx.close()
+
+
+def test_exit_histogram_constant_table_counts(hip_device, oracle):
+    """Equal path weights: every bin is its path fraction."""
+    case = cases.MID
+    table, pos32 = cases.setup(case, oracle)
+    ctx = capi.Context(0)
+    ctx.set_tables(np.full(len(table.table), 0.5, np.float32), table.minK, table.maxK, table.step, table.ds,
+                   case.absorption)
+    ctx.set_curve(pos32, cases.bc13(case))
+    ctx.set_exit_histogram(16, 8)
+    paths = 1 << 14
+    ctx.run(paths, case.K, seed=3)
+    h, b = ctx.exit_histogram(paths)
+    counts = np.bincount(b, minlength=128).reshape(16, 8) / paths
+    assert np.abs(h - counts).max() < 1e-6
+    ctx.close()
+
+
+def test_exit_histogram_c5_three_tables(hip_device, oracle):
+    """N=1024, K=200, the sum of 3 tables (sigma 0.02/0.05/0.1), 256x256 bins."""
+    case = cases.C5
+    ds = cases.ds_of(case)
+    tabs = [oracle.build_table(ds, eps=float(np.float32(s) ** 2), numCurvatureSteps=4096) for s in (0.02, 0.05, 0.1)]
+    summed = np.sum([t.table.astype(np.float64) for t in tabs], axis=0).astype(np.float32)
+    _, pos32 = cases.setup(case, oracle)
+    ctx = capi.Context(0)
+    ctx.set_tables(np.stack([t.table for t in tabs]), tabs[0].minK, tabs[0].maxK, tabs[0].step, ds, case.absorption)
+    ctx.set_curve(pos32, cases.bc13(case))
+    ctx.set_exit_histogram(256, 256)
+    paths = 1 << 12
+    print(f"[hist C5 3 tables paths={paths}]", flush=True)
+    res = ctx.run(paths, case.K, seed=case.seed, per_path=True)
+    h, gbins = ctx.exit_histogram(paths)
+
+    class T:
+        pass
+
+    t = T()
+    t.table, t.ds, t.minK, t.maxK, t.step = summed, ds, tabs[0].minK, tabs[0].maxK, tabs[0].step
+    ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, paths, t, case.absorption,
+                          seed=case.seed, want_exit=True)
+    g = res["log10w"].astype(np.float64)
+    ok = ref["valid"]
+    rel = np.abs(g[ok] - ref["log10w"][ok]) / np.abs(ref["log10w"][ok])
+    print(f"  per-path median rel {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
+    assert np.median(rel) < 1e-5
+    href = reference_histogram(ref["log10w"], gbins.astype(np.int64), 256 * 256).reshape(256, 256)
+    # weights this wide put most of the mass on a few paths; compare the mass
+    # each bin carries with the per-path error allowance of those paths
+    err = np.abs(h - href).sum()
+    print(f"  L1 histogram difference {err:.3e}", flush=True)
+    assert err < 5e-2
+    assert abs(h.sum() - 1.0) < 1e-5
+    ctx.close()

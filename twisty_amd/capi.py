@@ -19,6 +19,8 @@ EXPORTED = [
     "twisty_hip_version", "twisty_hip_device_count", "twisty_hip_create", "twisty_hip_destroy",
     "twisty_hip_last_error", "twisty_hip_set_tables", "twisty_hip_set_curve", "twisty_hip_run",
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
+    "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
+    "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
 ]
 
 
@@ -71,6 +73,11 @@ def _declare(L):
     L.twisty_hip_set_schedule.argtypes = [vp, u32, u32, u32]
     L.twisty_hip_get_schedule.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
+    L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
+    L.twisty_hip_get_exit_histogram.argtypes = [vp, vp, vp, u64]
+    L.twisty_hip_run_adjoint.argtypes = [vp, u64, u64, u32, u64, C.POINTER(Partials), vp, vp]
+    L.twisty_hip_set_adjoint_schedule.argtypes = [vp, u32, u32, u32]
+    L.twisty_hip_last_adjoint_timing.argtypes = [vp, C.POINTER(f32)]
 
 
 def device_count():
@@ -119,6 +126,7 @@ class Context:
             t = t[None, :]
         self._check(lib().twisty_hip_set_tables(self._h, _ptr(t), t.shape[1], t.shape[0], minK, maxK, step, ds,
                                                 absorption))
+        self.table_bins = t.shape[1]
 
     def set_curve(self, positions, boundary):
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
@@ -152,6 +160,39 @@ class Context:
         a, b = C.c_float(), C.c_float()
         self._check(lib().twisty_hip_last_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_exit_histogram(self, n_theta, n_phi):
+        self._check(lib().twisty_hip_set_exit_histogram(self._h, n_theta, n_phi))
+        self.hist_shape = (n_theta, n_phi)
+
+    def exit_histogram(self, num_paths=None):
+        """(histogram (n_theta, n_phi) float64, per-path bins or None) of the last run."""
+        nt, nph = self.hist_shape
+        h = np.empty(nt * nph, np.float64)
+        bins = np.empty(num_paths, np.uint32) if num_paths else None
+        self._check(lib().twisty_hip_get_exit_histogram(self._h, _ptr(h), _ptr(bins), num_paths or 0))
+        return h.reshape(nt, nph), bins
+
+    def set_adjoint_schedule(self, lanes_per_path=0, block_size=0, blocks_per_cu=0):
+        self._check(lib().twisty_hip_set_adjoint_schedule(self._h, lanes_per_path, block_size, blocks_per_cu))
+
+    def run_adjoint(self, num_paths, K, seed=1, path_offset=0, segments=True, table_bins=None):
+        """Forward + adjoint: partials, dE/dT (bins) and dE/d(initial segments) (N, 3) or None."""
+        out = Partials()
+        nb = table_bins if table_bins is not None else self.table_bins
+        dT = np.empty(nb, np.float64)
+        dS = np.empty(3 * self.N, np.float64) if segments else None
+        self._check(lib().twisty_hip_run_adjoint(self._h, num_paths, path_offset, K, seed, C.byref(out), _ptr(dT),
+                                                 _ptr(dS)))
+        res = out.as_dict()
+        res["d_table"] = dT
+        res["d_seg"] = None if dS is None else dS.reshape(self.N, 3)
+        return res
+
+    def last_adjoint_ms(self):
+        a = C.c_float()
+        self._check(lib().twisty_hip_last_adjoint_timing(self._h, C.byref(a)))
+        return a.value
 
 
 def estimator(res, norm=1.0):

@@ -1,0 +1,379 @@
+// adjoint.hip -- reverse-mode derivative of the log-estimator (gfx950).
+//
+// E = log10(sum_p w_p / P) over the valid paths.  With omega_p = w_p / sum w,
+//   dE/dx = sum_p omega_p d(ln w_p)/dx / ln 10,
+// and ln w_p = sum_joints ln f(kappa_q) + const, f the lerped table weight
+// (PathWeighters.cpp:41-85).  The kernel reads what the forward pass left in
+// HBM -- each path's final segment vectors and its log2 weight -- plus the
+// normaliser (M, S) of the finalize kernel, so omega_p = 2^(log2 w_p - M) / S
+// needs no second pass over the paths.
+//
+// Table entries.  f is linear in T[left], T[right] with coefficients
+// (1 - frac, frac), so dE/dT[b] collects omega_p (1 - frac)/f and omega_p frac/f
+// over every joint: a scatter-add into a block-private copy of the gradient in
+// LDS (LDS float atomics), written out per block and summed in a second kernel.
+//
+// Initial segments (sweep instances).  dE/dd_i^0 is the adjoint of the K
+// perturbations run backwards.  A rotation about its own chord leaves the chord
+// unchanged, so each step's axis is recomputed from the post-rotation segments;
+// the pre-rotation segments are R^T d, so the state is rewound in place instead
+// of being stored.  With lambda = dE/dd after the step,
+//   lambda_pre = R^T lambda + dE/da   (segments of the sub-chain),
+//   dE/dR = sum lambda d_pre^T,  R = c I + s [n]x + C n n^T,
+//   dE/dn_k = s (G_ji - G_ij)_{(i,j,k) cyclic} + C ((G n)_k + (G^T n)_k),
+//   dE/da = (I - n n^T) dE/dn / |a|.
+// Every path starts from the same initial curve, so the per-path lambda^0 are
+// summed over paths (LDS atomics per block).
+//
+// The weight's derivative in the curvature.  The reference's lerp fraction is
+// in curvature units, not bin units (PathWeighters.cpp:79), so inside a bin f
+// moves by (T[r] - T[l]) per unit curvature and jumps by about T[r] - T[l] at
+// the next bin edge.  The pathwise slope alone would miss the jumps, which
+// carry almost all of the change; the adjoint uses the bin-average slope
+// (T[r] - T[l]) / step, the derivative of the weight averaged over the
+// position inside the bin (in-bin slope plus edge jumps per unit curvature).
+// For table entries no such choice arises: f is exactly linear in them.
+#include "adjoint.hpp"
+
+namespace twisty_hip {
+
+// One perturbation undone by lane j of its group: rewinds this lane's
+// segments of the sub-chain and their adjoints (same window and scratch-row
+// redirection as the forward's perturb_one; the adjoint rows sit lamB bytes
+// after the segment rows).
+template<int G, int BLOCK>
+__device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uint32_t j,
+      const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
+{
+    constexpr int WMAX = (25 + G - 1) / G;
+    constexpr uint32_t SLOTB = 12u * BLOCK;
+    constexpr uint32_t COMPB = 4u * BLOCK;
+    const uint32_t diff = rec.packed & 0xFFFFu;
+    const uint32_t left = rec.packed >> 16;
+    const uint32_t right = left + diff;
+    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
+    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
+    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
+    uint32_t addr[WMAX];
+    float dx[WMAX], dy[WMAX], dz[WMAX], lx[WMAX], ly[WMAX], lz[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        addr[w] = ((uint32_t)w < cnt) ? baseB : scrB[w];
+        const uint32_t a = addr[w] + w * SLOTB;
+        dx[w] = lds_ld(a);
+        dy[w] = lds_ld(a + COMPB);
+        dz[w] = lds_ld(a + 2 * COMPB);
+        lx[w] = lds_ld(a + lamB);
+        ly[w] = lds_ld(a + lamB + COMPB);
+        lz[w] = lds_ld(a + lamB + 2 * COMPB);
+    }
+    float ax = dx[0], ay = dy[0], az = dz[0];
+#pragma unroll
+    for (int w = 1; w < WMAX; ++w) {
+        ax += dx[w];
+        ay += dy[w];
+        az += dz[w];
+    }
+    if constexpr (G > 1) {
+        ax = group_sum<G>(ax);
+        ay = group_sum<G>(ay);
+        az = group_sum<G>(az);
+    }
+    const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+    const float nx = ax * inv, ny = ay * inv, nz = az * inv;
+    const Mat3 R = rotation_from_axis(nx, ny, nz, rec.c, rec.s, rec.C);
+    float g00 = 0.f, g01 = 0.f, g02 = 0.f, g10 = 0.f, g11 = 0.f, g12 = 0.f, g20 = 0.f, g21 = 0.f, g22 = 0.f;
+    float px[WMAX], py[WMAX], pz[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        // d_pre = R^T d_post
+        px[w] = fmaf(R.m00, dx[w], fmaf(R.m10, dy[w], R.m20 * dz[w]));
+        py[w] = fmaf(R.m01, dx[w], fmaf(R.m11, dy[w], R.m21 * dz[w]));
+        pz[w] = fmaf(R.m02, dx[w], fmaf(R.m12, dy[w], R.m22 * dz[w]));
+        g00 = fmaf(lx[w], px[w], g00);
+        g01 = fmaf(lx[w], py[w], g01);
+        g02 = fmaf(lx[w], pz[w], g02);
+        g10 = fmaf(ly[w], px[w], g10);
+        g11 = fmaf(ly[w], py[w], g11);
+        g12 = fmaf(ly[w], pz[w], g12);
+        g20 = fmaf(lz[w], px[w], g20);
+        g21 = fmaf(lz[w], py[w], g21);
+        g22 = fmaf(lz[w], pz[w], g22);
+    }
+    if constexpr (G > 1) {
+        g00 = group_sum<G>(g00);
+        g01 = group_sum<G>(g01);
+        g02 = group_sum<G>(g02);
+        g10 = group_sum<G>(g10);
+        g11 = group_sum<G>(g11);
+        g12 = group_sum<G>(g12);
+        g20 = group_sum<G>(g20);
+        g21 = group_sum<G>(g21);
+        g22 = group_sum<G>(g22);
+    }
+    // dE/dn, then project out the axis direction and divide by |a|.
+    const float gnx = fmaf(g00, nx, fmaf(g01, ny, g02 * nz)) + fmaf(g00, nx, fmaf(g10, ny, g20 * nz));
+    const float gny = fmaf(g10, nx, fmaf(g11, ny, g12 * nz)) + fmaf(g01, nx, fmaf(g11, ny, g21 * nz));
+    const float gnz = fmaf(g20, nx, fmaf(g21, ny, g22 * nz)) + fmaf(g02, nx, fmaf(g12, ny, g22 * nz));
+    const float hx = fmaf(rec.s, g21 - g12, rec.C * gnx);
+    const float hy = fmaf(rec.s, g02 - g20, rec.C * gny);
+    const float hz = fmaf(rec.s, g10 - g01, rec.C * gnz);
+    const float hn = fmaf(hx, nx, fmaf(hy, ny, hz * nz));
+    const float dax = fmaf(-hn, nx, hx) * inv;
+    const float day = fmaf(-hn, ny, hy) * inv;
+    const float daz = fmaf(-hn, nz, hz) * inv;
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        const uint32_t a = addr[w] + w * SLOTB;
+        float qx = fmaf(R.m00, lx[w], fmaf(R.m10, ly[w], R.m20 * lz[w]));
+        float qy = fmaf(R.m01, lx[w], fmaf(R.m11, ly[w], R.m21 * lz[w]));
+        float qz = fmaf(R.m02, lx[w], fmaf(R.m12, ly[w], R.m22 * lz[w]));
+        // Redirected slots read zeros and must write zeros back.
+        const bool in = (uint32_t)w < cnt;
+        qx = in ? qx + dax : qx;
+        qy = in ? qy + day : qy;
+        qz = in ? qz + daz : qz;
+        lds_st(a, px[w]);
+        lds_st(a + COMPB, py[w]);
+        lds_st(a + 2 * COMPB, pz[w]);
+        lds_st(a + lamB, qx);
+        lds_st(a + lamB + COMPB, qy);
+        lds_st(a + lamB + 2 * COMPB, qz);
+    }
+}
+
+// d ln f / d kappa and the joint's curvature-direction vector for the joint
+// between unit tangents (ta, tb), plus the table scatter when requested.
+struct JointAdj {
+    float vx, vy, vz;  // (d ln f / d kappa) * d kappa / d t_b  (= -d/d t_a)
+};
+
+template<bool SCATTER>
+__device__ __forceinline__ JointAdj joint_adjoint(const TableView &tv, float invDs, float invStep, float ta0,
+      float ta1, float ta2, float tb0, float tb1, float tb2, float wp, float *gradT)
+{
+    const float e0 = (tb0 - ta0) * invDs;
+    const float e1 = (tb1 - ta1) * invDs;
+    const float e2 = (tb2 - ta2) * invDs;
+    const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+    const LookupTerms lt = lookup_terms(tv, kappa);
+    if constexpr (SCATTER) {
+        const float cw = wp / lt.value;
+        atomicAdd(&gradT[lt.left], cw * (1.0f - lt.frac));
+        atomicAdd(&gradT[lt.right], cw * lt.frac);
+    }
+    const float slope = lt.clamped ? 0.0f : (tv.w[lt.right] - tv.w[lt.left]) * invStep;
+    const float g = (kappa > 0.0f) ? slope / lt.value * (invDs / kappa) : 0.0f;
+    return { g * e0, g * e1, g * e2 };
+}
+
+template<int G, int BLOCK, bool SWEEP>
+__global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
+{
+    constexpr uint32_t SLOT = 3u * BLOCK;
+    constexpr int WMAX = (25 + G - 1) / G;
+    extern __shared__ float smem[];
+    const uint32_t N = p.N, S = p.S, K = p.K;
+    const uint32_t rows = p.scratchRow + 1;
+    const uint32_t lamOff = SWEEP ? rows * SLOT : 0u;
+    const uint32_t tabOff = rows * SLOT + lamOff;
+    const uint32_t gtOff = tabOff + p.tableStride;
+    const uint32_t gsOff = gtOff + p.tableStride;
+    const uint32_t drawOff = gsOff + (SWEEP ? p.segStride : 0u);
+    uint32_t *drawbuf = reinterpret_cast<uint32_t *>(smem + drawOff);
+    float *gradT = smem + gtOff;
+    float *gradS = smem + gsOff;
+
+    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) {
+        smem[tabOff + i] = p.table[i];
+        gradT[i] = 0.0f;
+    }
+    if constexpr (SWEEP)
+        for (uint32_t i = threadIdx.x; i < 3 * N; i += BLOCK) gradS[i] = 0.0f;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t j = lane & (G - 1);
+    const uint32_t groupBase = lane - j;
+    constexpr uint32_t groupsPerBlock = BLOCK / G;
+    const uint32_t scr = p.scratchRow * SLOT + lane;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        smem[scr + c * BLOCK] = 0.0f;
+        if constexpr (SWEEP) smem[lamOff + scr + c * BLOCK] = 0.0f;
+    }
+    __syncthreads();
+    const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    const uint32_t ldsBase = __builtin_amdgcn_groupstaticsize();
+    const uint32_t laneB = ldsBase + lane * 4u;
+    const uint32_t lamB = lamOff * 4u;
+    uint32_t scrB[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) scrB[w] = ldsBase + (scr - (uint32_t)w * SLOT) * 4u;
+    const float M = (float)p.fin->maxLog2;
+    const bool finiteM = M > -INFINITY && M < INFINITY;
+
+    for (uint64_t local = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; local < p.numPaths;
+          local += (uint64_t)gridDim.x * groupsPerBlock) {
+        const float2 lw = p.pathLog2[local];
+        // Scaled weight 2^(log2 w - M) in (0, 1]; invalid (NaN) and zero-weight
+        // paths, and those below FP32 range relative to the largest, add nothing.
+        const float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
+        if (!(wp > 0.0f)) continue;  // uniform across the path's group
+        const uint64_t path = p.pathOffset + local;
+        for (uint32_t s = 0; s < S; ++s) {
+            const uint32_t i = s * G + j;
+            if (i < N) {
+                const float *src = p.finalSeg + (uint64_t)(3 * i) * p.numPaths + local;
+                smem[s * SLOT + lane] = src[0];
+                smem[s * SLOT + BLOCK + lane] = src[p.numPaths];
+                smem[s * SLOT + 2 * BLOCK + lane] = src[2 * p.numPaths];
+            }
+        }
+        wave_sync_lds();
+
+        // Adjoint of the weighting: lambda_i = wp (I - t t^T) dl/dt_i / |d_i|,
+        // dl/dt_i = v_{i-1} - v_i over the joints on either side.  The owner of
+        // segment i also scatters joint i's table coefficients.
+        for (uint32_t s = 0; s < S; ++s) {
+            const uint32_t i = s * G + j;
+            if (i >= N) break;
+            float t0, t1, t2;
+            const float ia = unit_tangent(smem[s * SLOT + lane], smem[s * SLOT + BLOCK + lane],
+                  smem[s * SLOT + 2 * BLOCK + lane], t0, t1, t2);
+            float mx = 0.0f, my = 0.0f, mz = 0.0f;
+            if (i >= 1) {
+                const uint32_t q = i - 1;
+                const uint32_t qi = (q / G) * SLOT + groupBase + (q & (uint32_t)(G - 1));
+                float u0, u1, u2;
+                unit_tangent(smem[qi], smem[qi + BLOCK], smem[qi + 2 * BLOCK], u0, u1, u2);
+                const JointAdj ja = joint_adjoint<false>(tv, p.invDs, p.invStep, u0, u1, u2, t0, t1, t2, wp, gradT);
+                mx += ja.vx;
+                my += ja.vy;
+                mz += ja.vz;
+            }
+            if (i + 1 < N) {
+                const uint32_t q = i + 1;
+                const uint32_t qi = (q / G) * SLOT + groupBase + (q & (uint32_t)(G - 1));
+                float u0, u1, u2;
+                unit_tangent(smem[qi], smem[qi + BLOCK], smem[qi + 2 * BLOCK], u0, u1, u2);
+                const JointAdj ja = joint_adjoint<true>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT);
+                mx -= ja.vx;
+                my -= ja.vy;
+                mz -= ja.vz;
+            }
+            if constexpr (SWEEP) {
+                const float tm = fmaf(t0, mx, fmaf(t1, my, t2 * mz));
+                const float sc = wp * ia;
+                smem[lamOff + s * SLOT + lane] = fmaf(-tm, t0, mx) * sc;
+                smem[lamOff + s * SLOT + BLOCK + lane] = fmaf(-tm, t1, my) * sc;
+                smem[lamOff + s * SLOT + 2 * BLOCK + lane] = fmaf(-tm, t2, mz) * sc;
+            }
+        }
+
+        if constexpr (SWEEP) {
+            wave_sync_lds();
+            const uint32_t numBatches = (K + G - 1) / G;
+            for (uint32_t bb = numBatches; bb-- > 0;) {
+                const uint32_t kb = bb * G;
+                const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
+                if constexpr (G == 1) {
+                    unperturb_one<G, BLOCK>(laneB, lamB, j, scrB,
+                          pack_draw(make_draw(p.seed, p.stream, path, kb, N, p.maxDiff)));
+                } else {
+                    if (j < kcount) {
+                        const DrawRec r = pack_draw(make_draw(p.seed, p.stream, path, kb + j, N, p.maxDiff));
+                        *reinterpret_cast<uint4 *>(drawbuf + lane * 4) =
+                              make_uint4(r.packed, __float_as_uint(r.c), __float_as_uint(r.s), __float_as_uint(r.C));
+                    }
+                    wave_sync_lds();
+                    if constexpr (G <= 8) {
+                        DrawRec recs[G];
+#pragma unroll
+                        for (int kk = 0; kk < G; ++kk) recs[kk] = load_rec(drawbuf + (groupBase + kk) * 4);
+                        wave_sync_lds();
+#pragma unroll
+                        for (int kk = G - 1; kk >= 0; --kk) {
+                            if ((uint32_t)kk < kcount) unperturb_one<G, BLOCK>(laneB, lamB, j, scrB, recs[kk]);
+                        }
+                    } else {
+                        for (uint32_t kk = kcount; kk-- > 0;)
+                            unperturb_one<G, BLOCK>(laneB, lamB, j, scrB, load_rec(drawbuf + (groupBase + kk) * 4));
+                        wave_sync_lds();
+                    }
+                }
+            }
+            wave_sync_lds();
+            for (uint32_t s = 0; s < S; ++s) {
+                const uint32_t i = s * G + j;
+                if (i < N) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) atomicAdd(&gradS[3 * i + c], smem[lamOff + s * SLOT + c * BLOCK + lane]);
+                }
+            }
+        }
+        wave_sync_lds();  // neighbours' columns are read before the next path is loaded
+    }
+    __syncthreads();
+    float *gt = p.gradTablePartial + (uint64_t)blockIdx.x * p.tableSize;
+    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) gt[i] = gradT[i];
+    if constexpr (SWEEP) {
+        float *gs = p.gradSegPartial + (uint64_t)blockIdx.x * (3 * N);
+        for (uint32_t i = threadIdx.x; i < 3 * N; i += BLOCK) gs[i] = gradS[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) grad_reduce_kernel(const float *partial, uint32_t numBlocks,
+      uint32_t numEntries, const FinalOut *fin, double *out)
+{
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= numEntries) return;
+    FF acc { 0.0f, 0.0f };
+    for (uint32_t b = 0; b < numBlocks; ++b) acc = ff_add_f(acc, partial[(uint64_t)b * numEntries + e]);
+    const double S = fin->sumScaled;
+    const double ln10 = 2.30258509299404568402;
+    out[e] = (S > 0.0) ? ((double)acc.hi + (double)acc.lo) / (S * ln10) : 0.0;
+}
+
+#define TWISTY_ADJ_INSTANCES(X)                                                                                   \
+    X(1, 128) X(2, 256) X(4, 256) X(4, 512) X(8, 256) X(8, 512) X(16, 512) X(16, 1024) X(32, 512) X(32, 1024)    \
+    X(64, 256) X(64, 512) X(64, 1024)
+
+template<int G, int B, bool SW>
+static hipError_t launch_adj_gb(const AdjParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    auto *fn = adjoint_kernel<G, B, SW>;
+    hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, s, p);
+    return hipGetLastError();
+}
+
+int adjoint_block_sizes(uint32_t G, bool, uint32_t *out, int cap)
+{
+    int n = 0;
+#define X(g, b)                                                                                                   \
+    if (G == g && n < cap) out[n++] = b;
+    TWISTY_ADJ_INSTANCES(X)
+#undef X
+    return n;
+}
+
+hipError_t launch_adjoint(const AdjParams &p, uint32_t G, bool sweep, uint32_t grid, uint32_t block, uint32_t lds,
+      hipStream_t s)
+{
+#define X(g, b)                                                                                                   \
+    if (G == g && block == b) return sweep ? launch_adj_gb<g, b, true>(p, grid, lds, s)                         \
+                                           : launch_adj_gb<g, b, false>(p, grid, lds, s);
+    TWISTY_ADJ_INSTANCES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_grad_reduce(const float *partial, uint32_t numBlocks, uint32_t numEntries, const FinalOut *fin,
+      double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((numEntries + 255) / 256), dim3(256), 0, s, partial, numBlocks,
+          numEntries, fin, out);
+    return hipGetLastError();
+}
+
+}  // namespace twisty_hip

@@ -4,6 +4,7 @@
 // every shape on the host before a kernel is launched.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "../../include/twisty_hip.h"
+#include "adjoint.hpp"
 #include "forward.hpp"
 
 using namespace twisty_hip;
@@ -18,7 +20,7 @@ using namespace twisty_hip;
 struct twisty_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     uint32_t numCUs = 0;
     uint32_t ldsPerBlock = 0;
     uint32_t ldsPerCU = 0;
@@ -32,10 +34,11 @@ struct twisty_hip_ctx {
     float *dInitSeg = nullptr;
     uint32_t N = 0;
     float arclength = 0;
+    float endDir[3] = { 0, 0, 1 };
     bool haveCurve = false;
 
     BlockPartial *dPartials = nullptr;
-    uint32_t partialsCap = 0;
+    uint64_t partialsCap = 0;
     FinalOut *dFinal = nullptr;
 
     float *dPerPath = nullptr;
@@ -43,8 +46,34 @@ struct twisty_hip_ctx {
     float *dPerJoint = nullptr;
     uint64_t perJointCap = 0;
 
+    // Exit-direction histogram.
+    uint32_t histTheta = 0, histPhi = 0;
+    float2 *dPathLog2 = nullptr;
+    uint64_t pathLog2Cap = 0;
+    uint32_t *dPathBin = nullptr;
+    uint64_t pathBinCap = 0;
+    float *dHistPartial = nullptr;
+    uint64_t histPartialCap = 0;
+    double *dHist = nullptr;
+    uint64_t histCap = 0;
+    uint64_t lastHistPaths = 0;
+    uint32_t lastHistBins = 0;
+
+    // Adjoint.
+    float *dFinalSeg = nullptr;
+    uint64_t finalSegCap = 0;
+    float *dGradTPartial = nullptr;
+    uint64_t gradTPartialCap = 0;
+    float *dGradSPartial = nullptr;
+    uint64_t gradSPartialCap = 0;
+    double *dGradT = nullptr;
+    uint64_t gradTCap = 0;
+    double *dGradS = nullptr;
+    uint64_t gradSCap = 0;
+
     uint32_t schedG = 0, schedBlock = 0, schedBpc = 0;
-    float lastFwdMs = 0.0f, lastTotalMs = 0.0f;
+    uint32_t adjG = 0, adjBlock = 0, adjBpc = 0;
+    float lastFwdMs = 0.0f, lastTotalMs = 0.0f, lastAdjMs = 0.0f;
 };
 
 namespace {
@@ -120,6 +149,187 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
     return TWISTY_HIP_OK;
 }
 
+template<class T>
+int ensure_dev(twisty_hip_ctx *ctx, T *&ptr, uint64_t &cap, uint64_t n, const char *what)
+{
+    if (n <= cap && ptr) return TWISTY_HIP_OK;
+    hipError_t e = hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    if (e != hipSuccess) return hipfail(ctx, e, what);
+    if ((e = hipMalloc(&ptr, n * sizeof(T))) != hipSuccess) return hipfail(ctx, e, what);
+    cap = n;
+    return TWISTY_HIP_OK;
+}
+
+// End frame of the exit-direction bins: e = normalised end direction,
+// f1 = the coordinate axis least aligned with e made orthogonal to it,
+// f2 = e x f1 (FP64, rounded once).
+void end_frame(const float endDir[3], float e[3], float f1[3], float f2[3])
+{
+    double E[3] = { endDir[0], endDir[1], endDir[2] };
+    const double en = std::sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+    for (double &v : E) v /= en;
+    const double ax = std::fabs(E[0]), ay = std::fabs(E[1]), az = std::fabs(E[2]);
+    int idx = 0;
+    if (ay < ax) idx = 1;
+    if (az < std::min(ax, ay)) idx = 2;
+    double A[3] = { 0, 0, 0 };
+    A[idx] = 1.0;
+    const double d = A[0] * E[0] + A[1] * E[1] + A[2] * E[2];
+    double F1[3] = { A[0] - d * E[0], A[1] - d * E[1], A[2] - d * E[2] };
+    const double fn = std::sqrt(F1[0] * F1[0] + F1[1] * F1[1] + F1[2] * F1[2]);
+    for (double &v : F1) v /= fn;
+    const double F2[3] = { E[1] * F1[2] - E[2] * F1[1], E[2] * F1[0] - E[0] * F1[2], E[0] * F1[1] - E[1] * F1[0] };
+    for (int a = 0; a < 3; ++a) {
+        e[a] = (float)E[a];
+        f1[a] = (float)F1[a];
+        f2[a] = (float)F2[a];
+    }
+}
+
+struct FwdRequest {
+    uint64_t numPaths = 0, pathOffset = 0, seed = 0;
+    uint32_t K = 0, chainLength = 1;
+    bool perPath = false, perJoint = false, pathLog2 = false, pathBin = false, finalSeg = false;
+};
+
+// Validates a run, sizes the buffers it needs and enqueues forward + finalize
+// on the context's stream between events ev0 (start), ev1 (forward done) and
+// ev2 (finalize done).  Nothing is synchronised.
+int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
+{
+    const std::string w(who);
+    if (!ctx->haveTables || !ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, w + ": set_tables and set_curve first");
+    if (r.numPaths == 0 || r.K == 0 || r.chainLength == 0)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, w + ": numPaths, perturbationsPerPath and chainLength must be >= 1");
+    if (r.pathOffset % r.chainLength != 0)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, w + ": pathOffset must be a multiple of chainLength");
+    if (r.numPaths > (1ull << 40)) return fail(ctx, TWISTY_HIP_ERR_ARG, w + ": numPaths > 2^40");
+    const uint32_t N = ctx->N;
+    uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
+    std::string why;
+    int rc = resolve_schedule(ctx, N, G, B, bpc, lds, why);
+    if (rc) return fail(ctx, rc, w + ": " + why);
+    const uint32_t grid = ctx->numCUs * bpc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) return rc;
+    if (r.perPath && (rc = ensure_dev(ctx, ctx->dPerPath, ctx->perPathCap, r.numPaths, "per-path buffer"))) return rc;
+    const uint64_t nJoint = r.numPaths * (uint64_t)(N - 1);
+    if (r.perJoint && (rc = ensure_dev(ctx, ctx->dPerJoint, ctx->perJointCap, nJoint, "per-joint buffer"))) return rc;
+    if (r.pathLog2 && (rc = ensure_dev(ctx, ctx->dPathLog2, ctx->pathLog2Cap, r.numPaths, "path log2 buffer")))
+        return rc;
+    if (r.pathBin && (rc = ensure_dev(ctx, ctx->dPathBin, ctx->pathBinCap, r.numPaths, "path bin buffer"))) return rc;
+    if (r.finalSeg &&
+          (rc = ensure_dev(ctx, ctx->dFinalSeg, ctx->finalSegCap, r.numPaths * 3ull * N, "final segment buffer")))
+        return rc;
+
+    // Per-segment absorption factor, as PathWeighters.cpp:37 computes it
+    // (float argument -> float exp), applied to every one of the N-1 joints.
+    const double absorb = (double)std::exp((float)(-ctx->absorption * ctx->ds));
+    const double absorbLog2PerPath = (double)(N - 1) * std::log2(absorb);
+
+    FwdParams p {};
+    p.initSeg = ctx->dInitSeg;
+    p.table = ctx->dTable;
+    p.partials = ctx->dPartials;
+    p.perPathLog10 = r.perPath ? ctx->dPerPath : nullptr;
+    p.perJointCurv = r.perJoint ? ctx->dPerJoint : nullptr;
+    p.seed = r.seed;
+    p.pathOffset = r.pathOffset;
+    p.numChains = (r.numPaths + r.chainLength - 1) / r.chainLength;
+    p.numPaths = r.numPaths;
+    p.chainLength = r.chainLength;
+    p.N = N;
+    p.K = r.K;
+    p.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
+    p.stream = 0;
+    p.S = (N + G - 1) / G;
+    p.scratchRow = forward_scratch_row(G, N);
+    p.tableSize = (int32_t)ctx->tableSize;
+    p.tableStride = table_stride(ctx->tableSize);
+    p.minK = ctx->minK;
+    p.maxK = ctx->maxK;
+    p.step = ctx->step;
+    p.invDs = (float)N / ctx->arclength;  // CurvePerturbUtils.cpp:223
+    p.log10of2 = 0.30102999566398119521f;
+    p.absorbLog10PerPath = (float)(absorbLog2PerPath * 0.30102999566398119521);
+    p.pathLog2 = r.pathLog2 ? ctx->dPathLog2 : nullptr;
+    p.pathBin = r.pathBin ? ctx->dPathBin : nullptr;
+    p.finalSeg = r.finalSeg ? ctx->dFinalSeg : nullptr;
+    end_frame(ctx->endDir, p.exitE, p.exitF1, p.exitF2);
+    p.nTheta = ctx->histTheta;
+    p.nPhi = ctx->histPhi;
+
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "forward kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "finalize kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    return TWISTY_HIP_OK;
+}
+
+void copy_partials(const FinalOut &fo, uint64_t numPaths, twisty_hip_partials *out)
+{
+    out->log10_sum_w = fo.log10SumW;
+    out->log10_sum_w2 = fo.log10SumW2;
+    out->max_log10_w = fo.maxLog10W;
+    out->num_paths = numPaths;
+    out->num_valid = fo.nValid;
+    out->num_invalid = fo.nInvalid;
+    out->num_clamped = fo.nClamped;
+}
+
+// Adjoint schedule: G from N (about 8 segments per lane: the sweep keeps two
+// rows per segment), then the largest compiled block that fits.
+int resolve_adjoint_schedule(const twisty_hip_ctx *ctx, uint32_t N, bool sweep, uint32_t &G, uint32_t &block,
+      uint32_t &bpc, uint32_t &lds, std::string &why)
+{
+    const uint32_t ts = table_stride(ctx->tableSize ? ctx->tableSize : 4096);
+    if (G == 0) {
+        G = 1;
+        while (G < 64 && (N + G - 1) / G > (sweep ? 8u : 16u)) G <<= 1;
+    }
+    if (!is_pow2(G) || G > 64) {
+        why = "adjoint lanesPerPath must be a power of two <= 64";
+        return TWISTY_HIP_ERR_ARG;
+    }
+    uint32_t sizes[32];
+    const int ns = adjoint_block_sizes(G, sweep, sizes, 32);
+    if (block == 0) {
+        for (int k = ns - 1; k >= 0; --k) {
+            if (adjoint_lds_bytes(G, sizes[k], N, ts, sweep) <= ctx->ldsPerBlock) {
+                block = sizes[k];
+                break;
+            }
+        }
+        if (block == 0) {
+            why = "no compiled adjoint block size fits the LDS of one block for N=" + std::to_string(N);
+            return TWISTY_HIP_ERR_UNSUPPORTED;
+        }
+    }
+    bool compiled = false;
+    for (int k = 0; k < ns; ++k) compiled |= (sizes[k] == block);
+    if (!compiled) {
+        why = "no adjoint kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
+        return TWISTY_HIP_ERR_UNSUPPORTED;
+    }
+    lds = adjoint_lds_bytes(G, block, N, ts, sweep);
+    if (lds > ctx->ldsPerBlock) {
+        why = "adjoint schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
+              std::to_string(ctx->ldsPerBlock);
+        return TWISTY_HIP_ERR_UNSUPPORTED;
+    }
+    if (bpc == 0) {
+        bpc = ctx->ldsPerCU / lds;
+        if (bpc * block > 2048) bpc = 2048 / block;
+        if (bpc < 1) bpc = 1;
+    }
+    return TWISTY_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -166,6 +376,7 @@ int twisty_hip_create(int device, twisty_hip_ctx **out)
     if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "event");
     if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "event");
     if ((e = hipEventCreate(&ctx->ev2)) != hipSuccess) return bail(e, "event");
+    if ((e = hipEventCreate(&ctx->ev3)) != hipSuccess) return bail(e, "event");
     if ((e = hipMalloc(&ctx->dFinal, sizeof(FinalOut))) != hipSuccess) return bail(e, "final buffer");
     *out = ctx;
     return TWISTY_HIP_OK;
@@ -183,9 +394,19 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dFinal);
     (void)hipFree(ctx->dPerPath);
     (void)hipFree(ctx->dPerJoint);
+    (void)hipFree(ctx->dPathLog2);
+    (void)hipFree(ctx->dPathBin);
+    (void)hipFree(ctx->dHistPartial);
+    (void)hipFree(ctx->dHist);
+    (void)hipFree(ctx->dFinalSeg);
+    (void)hipFree(ctx->dGradTPartial);
+    (void)hipFree(ctx->dGradSPartial);
+    (void)hipFree(ctx->dGradT);
+    (void)hipFree(ctx->dGradS);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TWISTY_HIP_OK;
@@ -270,6 +491,7 @@ int twisty_hip_set_curve(twisty_hip_ctx *ctx, const float *positions, uint32_t n
     HIP_TRY(ctx, hipMemcpy(ctx->dInitSeg, tan.data(), tan.size() * sizeof(float), hipMemcpyHostToDevice));
     ctx->N = N;
     ctx->arclength = arclength;
+    for (int a = 0; a < 3; ++a) ctx->endDir[a] = boundary[9 + a];
     ctx->haveCurve = true;
     return TWISTY_HIP_OK;
 }
@@ -309,96 +531,176 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
       float *perJointCurvature)
 {
     if (!ctx) return TWISTY_HIP_ERR_ARG;
-    if (!ctx->haveTables || !ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "run: set_tables and set_curve first");
     if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run: out is NULL");
-    if (numPaths == 0 || perturbationsPerPath == 0 || chainLength == 0)
-        return fail(ctx, TWISTY_HIP_ERR_ARG, "run: numPaths, perturbationsPerPath and chainLength must be >= 1");
-    if (pathOffset % chainLength != 0)
-        return fail(ctx, TWISTY_HIP_ERR_ARG, "run: pathOffset must be a multiple of chainLength");
-    if (numPaths > (1ull << 40)) return fail(ctx, TWISTY_HIP_ERR_ARG, "run: numPaths > 2^40");
-    const uint32_t N = ctx->N;
-    uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
-    std::string why;
-    int rc = resolve_schedule(ctx, N, G, B, bpc, lds, why);
-    if (rc) return fail(ctx, rc, "run: " + why);
-    const uint32_t grid = ctx->numCUs * bpc;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (grid > ctx->partialsCap) {
-        HIP_TRY(ctx, hipFree(ctx->dPartials));
-        ctx->dPartials = nullptr;
-        HIP_TRY(ctx, hipMalloc(&ctx->dPartials, (size_t)grid * sizeof(BlockPartial)));
-        ctx->partialsCap = grid;
+    const bool hist = ctx->histTheta > 0;
+    FwdRequest r;
+    r.numPaths = numPaths;
+    r.pathOffset = pathOffset;
+    r.seed = seed;
+    r.K = perturbationsPerPath;
+    r.chainLength = chainLength;
+    r.perPath = perPathLog10 != nullptr;
+    r.perJoint = perJointCurvature != nullptr;
+    r.pathLog2 = hist;
+    r.pathBin = hist;
+    int rc = enqueue_forward(ctx, r, "run");
+    if (rc) return rc;
+    if (hist) {
+        const uint32_t nb = ctx->histTheta * ctx->histPhi;
+        const HistShape h = hist_shape(numPaths, nb);
+        if ((rc = ensure_dev(ctx, ctx->dHistPartial, ctx->histPartialCap, (uint64_t)h.numChunks * nb, "histogram partials")))
+            return rc;
+        if ((rc = ensure_dev(ctx, ctx->dHist, ctx->histCap, nb, "histogram"))) return rc;
+        hipError_t e = launch_histogram(ctx->dPathLog2, ctx->dPathBin, numPaths, ctx->dFinal, h, ctx->dHistPartial,
+              ctx->dHist, ctx->stream);
+        if (e != hipSuccess) return hipfail(ctx, e, "histogram kernel launch");
     }
-    if (perPathLog10 && numPaths > ctx->perPathCap) {
-        HIP_TRY(ctx, hipFree(ctx->dPerPath));
-        ctx->dPerPath = nullptr;
-        HIP_TRY(ctx, hipMalloc(&ctx->dPerPath, numPaths * sizeof(float)));
-        ctx->perPathCap = numPaths;
-    }
-    const uint64_t nJoint = numPaths * (uint64_t)(N - 1);
-    if (perJointCurvature && nJoint > ctx->perJointCap) {
-        HIP_TRY(ctx, hipFree(ctx->dPerJoint));
-        ctx->dPerJoint = nullptr;
-        HIP_TRY(ctx, hipMalloc(&ctx->dPerJoint, nJoint * sizeof(float)));
-        ctx->perJointCap = nJoint;
-    }
-
-    // Per-segment absorption factor, as PathWeighters.cpp:37 computes it
-    // (float argument -> float exp), applied to every one of the N-1 joints.
-    const double absorb = (double)std::exp((float)(-ctx->absorption * ctx->ds));
-    const double absorbLog2PerPath = (double)(N - 1) * std::log2(absorb);
-
-    FwdParams p {};
-    p.initSeg = ctx->dInitSeg;
-    p.table = ctx->dTable;
-    p.partials = ctx->dPartials;
-    p.perPathLog10 = perPathLog10 ? ctx->dPerPath : nullptr;
-    p.perJointCurv = perJointCurvature ? ctx->dPerJoint : nullptr;
-    p.seed = seed;
-    p.pathOffset = pathOffset;
-    p.numChains = (numPaths + chainLength - 1) / chainLength;
-    p.numPaths = numPaths;
-    p.chainLength = chainLength;
-    p.N = N;
-    p.K = perturbationsPerPath;
-    p.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
-    p.stream = 0;
-    p.S = (N + G - 1) / G;
-    p.scratchRow = forward_scratch_row(G, N);
-    p.tableSize = (int32_t)ctx->tableSize;
-    p.tableStride = table_stride(ctx->tableSize);
-    p.minK = ctx->minK;
-    p.maxK = ctx->maxK;
-    p.step = ctx->step;
-    p.invDs = (float)N / ctx->arclength;  // CurvePerturbUtils.cpp:223
-    p.log10of2 = 0.30102999566398119521f;
-    p.absorbLog10PerPath = (float)(absorbLog2PerPath * 0.30102999566398119521);
-
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
-    if (e != hipSuccess) return hipfail(ctx, e, "forward kernel launch");
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
-    if (e != hipSuccess) return hipfail(ctx, e, "finalize kernel launch");
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
     FinalOut fo;
     HIP_TRY(ctx, hipMemcpyAsync(&fo, ctx->dFinal, sizeof(FinalOut), hipMemcpyDeviceToHost, ctx->stream));
     if (perPathLog10)
         HIP_TRY(ctx, hipMemcpyAsync(perPathLog10, ctx->dPerPath, numPaths * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream));
     if (perJointCurvature)
-        HIP_TRY(ctx, hipMemcpyAsync(perJointCurvature, ctx->dPerJoint, nJoint * sizeof(float),
+        HIP_TRY(ctx, hipMemcpyAsync(perJointCurvature, ctx->dPerJoint, numPaths * (uint64_t)(ctx->N - 1) * sizeof(float),
                            hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev2));
-    out->log10_sum_w = fo.log10SumW;
-    out->log10_sum_w2 = fo.log10SumW2;
-    out->max_log10_w = fo.maxLog10W;
-    out->num_paths = numPaths;
-    out->num_valid = fo.nValid;
-    out->num_invalid = fo.nInvalid;
-    out->num_clamped = fo.nClamped;
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev3));
+    ctx->lastAdjMs = 0.0f;
+    ctx->lastHistPaths = hist ? numPaths : 0;
+    ctx->lastHistBins = hist ? ctx->histTheta * ctx->histPhi : 0;
+    copy_partials(fo, numPaths, out);
+    ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_exit_histogram(twisty_hip_ctx *ctx, uint32_t nTheta, uint32_t nPhi)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if ((nTheta == 0) != (nPhi == 0) || (uint64_t)nTheta * nPhi > (1u << 20))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_exit_histogram: need nTheta, nPhi both 0 (off) or nTheta*nPhi <= 2^20");
+    ctx->histTheta = nTheta;
+    ctx->histPhi = nPhi;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_get_exit_histogram(twisty_hip_ctx *ctx, double *histogram, uint32_t *perPathBin, uint64_t numPaths)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (ctx->lastHistBins == 0)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "get_exit_histogram: the last run had no exit histogram");
+    if (perPathBin && numPaths != ctx->lastHistPaths)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "get_exit_histogram: numPaths must equal the last run's path count");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (histogram)
+        HIP_TRY(ctx, hipMemcpy(histogram, ctx->dHist, ctx->lastHistBins * sizeof(double), hipMemcpyDeviceToHost));
+    if (perPathBin)
+        HIP_TRY(ctx, hipMemcpy(perPathBin, ctx->dPathBin, numPaths * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_adjoint_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
+      uint32_t blocksPerCU)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (lanesPerPath && (!is_pow2(lanesPerPath) || lanesPerPath > 64))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_schedule: lanesPerPath must be a power of two <= 64");
+    if (blockSize && (blockSize % 64 != 0 || blockSize > 1024))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_schedule: blockSize must be a multiple of 64 <= 1024");
+    if (blocksPerCU > 32) return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_schedule: blocksPerCU <= 32");
+    ctx->adjG = lanesPerPath;
+    ctx->adjBlock = blockSize;
+    ctx->adjBpc = blocksPerCU;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset,
+      uint32_t perturbationsPerPath, uint64_t seed, twisty_hip_partials *out, double *dLog10EstimateDTable,
+      double *dLog10EstimateDSegments)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!out || !dLog10EstimateDTable) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_adjoint: out and dTable are required");
+    const bool sweep = dLog10EstimateDSegments != nullptr;
+    FwdRequest r;
+    r.numPaths = numPaths;
+    r.pathOffset = pathOffset;
+    r.seed = seed;
+    r.K = perturbationsPerPath;
+    r.chainLength = 1;
+    r.pathLog2 = true;
+    r.finalSeg = true;
+    const uint32_t N = ctx->N;
+    if (ctx->haveCurve && ctx->haveTables) {
+        uint32_t G = ctx->adjG, B = ctx->adjBlock, bpc = ctx->adjBpc, lds = 0;
+        std::string why;
+        int rc0 = resolve_adjoint_schedule(ctx, N, sweep, G, B, bpc, lds, why);
+        if (rc0) return fail(ctx, rc0, "run_adjoint: " + why);
+    }
+    int rc = enqueue_forward(ctx, r, "run_adjoint");
+    if (rc) return rc;
+    uint32_t G = ctx->adjG, B = ctx->adjBlock, bpc = ctx->adjBpc, lds = 0;
+    std::string why;
+    rc = resolve_adjoint_schedule(ctx, N, sweep, G, B, bpc, lds, why);
+    if (rc) return fail(ctx, rc, "run_adjoint: " + why);
+    const uint32_t grid = ctx->numCUs * bpc;
+    const uint32_t nb = ctx->tableSize;
+    if ((rc = ensure_dev(ctx, ctx->dGradTPartial, ctx->gradTPartialCap, (uint64_t)grid * nb, "table gradient partials")))
+        return rc;
+    if ((rc = ensure_dev(ctx, ctx->dGradT, ctx->gradTCap, nb, "table gradient"))) return rc;
+    if (sweep) {
+        if ((rc = ensure_dev(ctx, ctx->dGradSPartial, ctx->gradSPartialCap, (uint64_t)grid * 3ull * N,
+                   "segment gradient partials")))
+            return rc;
+        if ((rc = ensure_dev(ctx, ctx->dGradS, ctx->gradSCap, 3ull * N, "segment gradient"))) return rc;
+    }
+    AdjParams a {};
+    a.finalSeg = ctx->dFinalSeg;
+    a.pathLog2 = ctx->dPathLog2;
+    a.fin = ctx->dFinal;
+    a.table = ctx->dTable;
+    a.gradTablePartial = ctx->dGradTPartial;
+    a.gradSegPartial = sweep ? ctx->dGradSPartial : nullptr;
+    a.seed = seed;
+    a.pathOffset = pathOffset;
+    a.numPaths = numPaths;
+    a.N = N;
+    a.K = perturbationsPerPath;
+    a.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
+    a.stream = 0;
+    a.S = (N + G - 1) / G;
+    a.scratchRow = forward_scratch_row(G, N);
+    a.tableSize = (int32_t)nb;
+    a.tableStride = table_stride(nb);
+    a.segStride = (3u * N + 63u) & ~63u;
+    a.minK = ctx->minK;
+    a.maxK = ctx->maxK;
+    a.step = ctx->step;
+    a.invDs = (float)N / ctx->arclength;
+    a.invStep = 1.0f / ctx->step;
+    hipError_t e = launch_adjoint(a, G, sweep, grid, B, lds, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "adjoint kernel launch");
+    e = launch_grad_reduce(ctx->dGradTPartial, grid, nb, ctx->dFinal, ctx->dGradT, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "gradient reduce launch");
+    if (sweep) {
+        e = launch_grad_reduce(ctx->dGradSPartial, grid, 3u * N, ctx->dFinal, ctx->dGradS, ctx->stream);
+        if (e != hipSuccess) return hipfail(ctx, e, "gradient reduce launch");
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
+    FinalOut fo;
+    HIP_TRY(ctx, hipMemcpyAsync(&fo, ctx->dFinal, sizeof(FinalOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dLog10EstimateDTable, ctx->dGradT, nb * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx->stream));
+    if (sweep)
+        HIP_TRY(ctx, hipMemcpyAsync(dLog10EstimateDSegments, ctx->dGradS, 3ull * N * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev3));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastAdjMs, ctx->ev2, ctx->ev3));
+    ctx->lastHistPaths = 0;
+    ctx->lastHistBins = 0;
+    copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
     return TWISTY_HIP_OK;
 }
@@ -408,6 +710,13 @@ int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (forwardMs) *forwardMs = ctx->lastFwdMs;
     if (totalMs) *totalMs = ctx->lastTotalMs;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_last_adjoint_timing(twisty_hip_ctx *ctx, float *adjointMs)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (adjointMs) *adjointMs = ctx->lastAdjMs;
     return TWISTY_HIP_OK;
 }
 

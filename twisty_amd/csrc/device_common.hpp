@@ -141,6 +141,42 @@ __device__ __forceinline__ float lookup_weight(const TableView &t, float curvatu
     return __fadd_rn(a, b);
 }
 
+// The same lookup, returning the interpolation terms the adjoint needs.  value
+// is bit-identical to lookup_weight's result for the same curvature; clamped is
+// set when the curvature was moved onto the table range (the weight is then
+// flat in the curvature).
+struct LookupTerms {
+    int32_t left, right;
+    float frac;   // fraction in curvature units (PathWeighters.cpp:79)
+    float value;  // lerped weight
+    bool clamped;
+};
+
+__device__ __forceinline__ LookupTerms lookup_terms(const TableView &t, float curvature)
+{
+    LookupTerms r;
+    r.clamped = false;
+    if (curvature < t.minK) {
+        curvature = t.minK;
+        r.clamped = true;
+    }
+    if (curvature > t.maxK) {
+        curvature = t.maxK;
+        r.clamped = true;
+    }
+    const float distanceFromMin = __fsub_rn(curvature, t.minK);
+    const float realIdx = __fdiv_rn(distanceFromMin, t.step);
+    int32_t leftIdx = (int32_t)floorf(realIdx);
+    leftIdx = min(max(leftIdx, 0), t.size - 1);
+    r.left = leftIdx;
+    r.right = (leftIdx == t.size - 1) ? leftIdx : leftIdx + 1;
+    r.frac = __fsub_rn(distanceFromMin, __fmul_rn((float)leftIdx, t.step));
+    const float a = __fmul_rn(t.w[r.left], __fsub_rn(1.0f, r.frac));
+    const float b = __fmul_rn(t.w[r.right], r.frac);
+    r.value = __fadd_rn(a, b);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Compensated float arithmetic.
 // ---------------------------------------------------------------------------
@@ -302,6 +338,42 @@ __device__ __forceinline__ FF group_sum_ff(FF v)
     if constexpr (G >= 32) v = ff_add(v, FF { __shfl_xor(v.hi, 16, 64), __shfl_xor(v.lo, 16, 64) });
     if constexpr (G >= 64) v = ff_add(v, FF { __shfl_xor(v.hi, 32, 64), __shfl_xor(v.lo, 32, 64) });
     return v;
+}
+
+// ---------------------------------------------------------------------------
+// LDS helpers shared by the path kernels.
+// ---------------------------------------------------------------------------
+// Orders one wave's LDS writes before its later LDS reads (lanes of one group
+// exchange data through LDS; a group never spans two waves).
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ DrawRec load_rec(const uint32_t *p)
+{
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return { v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w) };
+}
+
+// LDS accesses by byte address.  The dynamic-LDS block starts right after the
+// static LDS, at __builtin_amdgcn_groupstaticsize(); addresses built from it are
+// plain integers, so a slot's row offset stays an instruction immediate.
+typedef __attribute__((address_space(3))) float lds_f32;
+
+__device__ __forceinline__ float lds_ld(uint32_t byteAddr) { return *(lds_f32 *)(size_t)byteAddr; }
+__device__ __forceinline__ void lds_st(uint32_t byteAddr, float v) { *(lds_f32 *)(size_t)byteAddr = v; }
+
+// Unit tangent of a segment vector as the forward pass forms it.
+__device__ __forceinline__ float unit_tangent(float a0, float a1, float a2, float &t0, float &t1, float &t2)
+{
+    const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
+    t0 = a0 * ia;
+    t1 = a1 * ia;
+    t2 = a2 * ia;
+    return ia;
 }
 
 }  // namespace twisty_hip

@@ -21,13 +21,6 @@
 
 namespace twisty_hip {
 
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Block-wide merge of the per-lane accumulators into partials[blockIdx.x].
 __device__ void block_reduce_store(LseAcc acc, float *scratch, BlockPartial *partials)
 {
@@ -65,12 +58,6 @@ __device__ void block_reduce_store(LseAcc acc, float *scratch, BlockPartial *par
     }
 }
 
-__device__ __forceinline__ DrawRec load_rec(const uint32_t *p)
-{
-    const uint4 v = *reinterpret_cast<const uint4 *>(p);
-    return { v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w) };
-}
-
 // One perturbation applied by lane j of its group: read this lane's segments of
 // the sub-chain [left, right), sum them across the group into the axis
 // p_right - p_left, build the rotation and write the rotated segments back.
@@ -81,14 +68,6 @@ __device__ __forceinline__ DrawRec load_rec(const uint32_t *p)
 // offset is added.  (Skipping the last slot behind a wave-uniform branch when
 // no path of the wave needs it measured slower: the compiler copies the window
 // registers around the branch.)
-// LDS accesses by byte address.  The dynamic-LDS block starts right after the
-// static LDS, at __builtin_amdgcn_groupstaticsize(); addresses built from it are
-// plain integers, so a slot's row offset stays an instruction immediate.
-typedef __attribute__((address_space(3))) float lds_f32;
-
-__device__ __forceinline__ float lds_ld(uint32_t byteAddr) { return *(lds_f32 *)(size_t)byteAddr; }
-__device__ __forceinline__ void lds_st(uint32_t byteAddr, float v) { *(lds_f32 *)(size_t)byteAddr = v; }
-
 template<int G, int BLOCK>
 __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
@@ -137,6 +116,23 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
         lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
         lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
     }
+}
+
+// Exit-direction bin of the unit tangent t: cos(theta) = t.e over nTheta bins
+// of [-1, 1], phi = atan2(t.f2, t.f1) over nPhi bins of [-pi, pi), clamped to
+// the grid (build-defined; the reference bins only weight magnitudes,
+// FullExperimentRunnerOptimalPerturb.cpp:26-37).
+__device__ __forceinline__ uint32_t exit_bin(const FwdParams &p, float t0, float t1, float t2)
+{
+    const float ct = fmaf(t0, p.exitE[0], fmaf(t1, p.exitE[1], t2 * p.exitE[2]));
+    const float c1 = fmaf(t0, p.exitF1[0], fmaf(t1, p.exitF1[1], t2 * p.exitF1[2]));
+    const float c2 = fmaf(t0, p.exitF2[0], fmaf(t1, p.exitF2[1], t2 * p.exitF2[2]));
+    const float phi = atan2f(c2, c1);
+    int it = (int)floorf((ct + 1.0f) * 0.5f * (float)p.nTheta);
+    int ip = (int)floorf((phi + 3.14159265358979f) * (0.5f / 3.14159265358979f) * (float)p.nPhi);
+    it = min(max(it, 0), (int)p.nTheta - 1);
+    ip = min(max(ip, 0), (int)p.nPhi - 1);
+    return (uint32_t)it * p.nPhi + (uint32_t)ip;
 }
 
 template<int G, int BLOCK>
@@ -227,35 +223,38 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
             // (CurvePerturbUtils.cpp:219-243), lookup and log
             // (PathWeighters.cpp:41-96).  Lane j handles joints i = j mod G.
             if constexpr (G > 1) wave_sync_lds();
+            if (p.finalSeg) {
+                for (uint32_t s = 0; s < S; ++s) {
+                    const uint32_t i = s * G + j;
+                    if (i < N) {
+                        float *o = p.finalSeg + (uint64_t)(3 * i) * p.numPaths + local;
+                        o[0] = smem[s * SLOT + lane];
+                        o[p.numPaths] = smem[s * SLOT + BLOCK + lane];
+                        o[2 * p.numPaths] = smem[s * SLOT + 2 * BLOCK + lane];
+                    }
+                }
+            }
             FF lsum { 0.0f, 0.0f };
             uint32_t flags = 0;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
             float ta0 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
             if constexpr (G == 1) {
-                const float a0 = smem[lane], a1 = smem[BLOCK + lane], a2 = smem[2 * BLOCK + lane];
-                const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
-                ta0 = a0 * ia;
-                ta1 = a1 * ia;
-                ta2 = a2 * ia;
+                unit_tangent(smem[lane], smem[BLOCK + lane], smem[2 * BLOCK + lane], ta0, ta1, ta2);
             }
             for (uint32_t i = j; i + 1 < N; i += G) {
                 if constexpr (G > 1) {
                     const uint32_t ai = (i / G) * SLOT + lane;
-                    const float a0 = smem[ai], a1 = smem[ai + BLOCK], a2 = smem[ai + 2 * BLOCK];
-                    const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
-                    ta0 = a0 * ia;
-                    ta1 = a1 * ia;
-                    ta2 = a2 * ia;
+                    unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
                 }
                 const uint32_t i1 = i + 1;
                 const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
-                const float b0 = smem[bi], b1 = smem[bi + BLOCK], b2 = smem[bi + 2 * BLOCK];
-                const float ib = __builtin_amdgcn_rsqf(fmaf(b0, b0, fmaf(b1, b1, b2 * b2)));
-                const float tb0 = b0 * ib, tb1 = b1 * ib, tb2 = b2 * ib;
+                float tb0, tb1, tb2;
+                unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
                 const float e0 = (tb0 - ta0) * p.invDs;
                 const float e1 = (tb1 - ta1) * p.invDs;
                 const float e2 = (tb2 - ta2) * p.invDs;
                 const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
                 if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = kappa;
+                if (p.pathBin && i == N - 2) p.pathBin[local] = exit_bin(p, ta0, ta1, ta2);
                 const float wseg = lookup_weight(tv, kappa, clamped);
                 const float l = log2f(wseg);
                 if (isnan(l)) flags |= 1u;
@@ -292,6 +291,13 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                     }
                 }
                 if (p.perPathLog10) p.perPathLog10[local] = outv;
+                if (p.pathLog2) {
+                    float2 v;
+                    if (invalid) v = make_float2(__int_as_float(0x7fc00000), 0.0f);
+                    else if (flags & 6u) v = make_float2((flags & 2u) ? -INFINITY : INFINITY, 0.0f);
+                    else v = make_float2(lsum.hi, lsum.lo);
+                    p.pathLog2[local] = v;
+                }
             }
         }
     }
@@ -361,6 +367,8 @@ __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts
     if (threadIdx.x == 0) {
         const double l2 = 0.30102999566398119521;  // log10(2)
         const double m = shM[0];
+        out->maxLog2 = m;
+        out->sumScaled = shS[0];
         if (m == -INFINITY) {
             out->log10SumW = -INFINITY;
             out->log10SumW2 = -INFINITY;

@@ -30,6 +30,12 @@ struct FwdParams {
     float minK, maxK, step, invDs;
     float log10of2;
     float absorbLog10PerPath;
+    // Optional per-path outputs of the forms built on the forward pass.
+    float2 *pathLog2;      // nullable: (hi, lo) of log2 w without absorption; hi NaN = invalid
+    uint32_t *pathBin;     // nullable: exit-direction bin of the last free tangent t_{N-2}
+    float *finalSeg;       // nullable: final segment i, component c of path local at (3i + c)*numPaths + local
+    float exitE[3], exitF1[3], exitF2[3];  // end frame of the exit-direction bins
+    uint32_t nTheta, nPhi;
 };
 
 struct FinalOut {
@@ -37,6 +43,8 @@ struct FinalOut {
     double log10SumW2;
     double maxLog10W;
     unsigned long long nValid, nInvalid, nClamped;
+    double maxLog2;     // M: max per-path log2 weight (no absorption), -inf if none
+    double sumScaled;   // S = sum over valid paths of 2^(log2 w - M)
 };
 
 // Block sizes compiled for G lanes per path (ascending); returns the count.
@@ -68,5 +76,17 @@ hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_
       hipStream_t s);
 hipError_t launch_finalize(const BlockPartial *parts, uint32_t n, double absorbLog2PerPath, FinalOut *out,
       hipStream_t s);
+
+// Exit-direction histogram (histogram.hip).  Pass 1 bins the scaled path
+// weights 2^(log2 w - M) of a chunk of paths into an LDS tile of bins and
+// writes the tile to partial[chunk][bin]; pass 2 sums the chunks per bin with
+// compensated FP32 and divides by S in FP64.
+struct HistShape {
+    uint32_t numBins, tileBins, numTiles, numChunks;
+    uint64_t chunkPaths;
+};
+HistShape hist_shape(uint64_t numPaths, uint32_t numBins);
+hipError_t launch_histogram(const float2 *pathLog2, const uint32_t *pathBin, uint64_t numPaths,
+      const FinalOut *fin, const HistShape &h, float *partial, double *hist, hipStream_t s);
 
 }  // namespace twisty_hip

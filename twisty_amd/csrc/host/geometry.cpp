@@ -1,4 +1,6 @@
 // geometry.cpp -- boundary conditions and the initial curve.
+#include <algorithm>
+#include <array>
 #include <cmath>
 
 #include "twisty_host.hpp"
@@ -40,13 +42,87 @@ float CalculateMinimumArclength(const PerturbUtils::BoundaryConditions &bc, uint
 // axis when that component vanishes).  With half-opening angle Phi and radius
 // R: |p_{N-1} - p1| = 2 R sin(Phi), ds = 2 R sin(Phi / M); Phi in [0, pi) is
 // found by bisection on sin(Phi) / sin(Phi / M), which falls from M to 0.
+namespace {
+// bulgeIn (nullable) replaces startDir as the source of the bulge direction;
+// bulgeOut (nullable) receives the unit bulge direction used; *degenerate is set
+// when startDir lies on the chord and the least-aligned axis was taken instead.
+bool arc_f64(const std::array<double, 3> &s0, const std::array<double, 3> &sd, const std::array<double, 3> &e0,
+      const std::array<double, 3> &ed, float arclength, uint32_t N, std::vector<std::array<double, 3>> &P,
+      const double *bulgeIn = nullptr, double *bulgeOut = nullptr, bool *degenerate = nullptr);
+}
+
 bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vector<Vec3> &positions)
 {
+    std::vector<double> P;
+    if (!InitialArcF64(bc, N, P)) return false;
+    positions.assign(N + 1, Vec3());
+    for (uint32_t i = 0; i <= N; ++i)
+        positions[i] = Vec3((float)P[3 * i], (float)P[3 * i + 1], (float)P[3 * i + 2]);
+    return true;
+}
+
+bool InitialArcF64(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vector<double> &positions)
+{
     if (N < 3 || !(bc.arclength > 0.0f)) return false;
-    const double ds = (double)(bc.arclength / (float)N);
     auto d3 = [](const Vec3 &v) { return std::array<double, 3> { v.x, v.y, v.z }; };
-    const auto s0 = d3(bc.m_startPos), sd = d3(bc.m_startDir), e0 = d3(bc.m_endPos), ed = d3(bc.m_endDir);
-    std::vector<std::array<double, 3>> P(N + 1);
+    std::vector<std::array<double, 3>> P;
+    if (!arc_f64(d3(bc.m_startPos), d3(bc.m_startDir), d3(bc.m_endPos), d3(bc.m_endDir), bc.arclength, N, P))
+        return false;
+    positions.resize(3 * (N + 1));
+    for (uint32_t i = 0; i <= N; ++i)
+        for (int a = 0; a < 3; ++a) positions[3 * i + a] = P[i][a];
+    return true;
+}
+
+bool InitialArcExitJacobian(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vector<double> &dTheta,
+      std::vector<double> &dPhi)
+{
+    if (N < 3 || !(bc.arclength > 0.0f)) return false;
+    auto d3 = [](const Vec3 &v) { return std::array<double, 3> { v.x, v.y, v.z }; };
+    const auto s0 = d3(bc.m_startPos), sd = d3(bc.m_startDir), e0 = d3(bc.m_endPos);
+    const double ex = bc.m_endDir.x, ey = bc.m_endDir.y, ez = bc.m_endDir.z;
+    const double en = std::sqrt(ex * ex + ey * ey + ez * ez);
+    const double theta = std::acos(std::max(-1.0, std::min(1.0, ez / en)));
+    const double phi = std::atan2(ey, ex);
+    // When startDir lies on the chord the bulge plane is a free choice (the
+    // least-aligned axis) and flips under any tilt of the end direction, so the
+    // curve is not differentiable there; the derivative is then taken with the
+    // bulge direction of the base curve held fixed.
+    const std::array<double, 3> ed0 { std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta) };
+    std::vector<std::array<double, 3>> Pp, Pm;
+    double bulge[3];
+    bool degenerate = false;
+    if (!arc_f64(s0, sd, e0, ed0, bc.arclength, N, Pp, nullptr, bulge, &degenerate)) return false;
+    auto curve = [&](double th, double ph, std::vector<std::array<double, 3>> &P) {
+        const std::array<double, 3> ed { std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th) };
+        return arc_f64(s0, sd, e0, ed, bc.arclength, N, P, degenerate ? bulge : nullptr);
+    };
+    auto central = [&](double dth, double dph, double h, std::vector<double> &out) {
+        if (!curve(theta + dth * h, phi + dph * h, Pp) || !curve(theta - dth * h, phi - dph * h, Pm)) return false;
+        out.resize(3 * (N + 1));
+        for (uint32_t i = 0; i <= N; ++i)
+            for (int a = 0; a < 3; ++a) out[3 * i + a] = (Pp[i][a] - Pm[i][a]) / (2.0 * h);
+        return true;
+    };
+    const double h = 1e-4;
+    for (int which = 0; which < 2; ++which) {
+        std::vector<double> d1, d2;
+        const double dth = which == 0 ? 1.0 : 0.0, dph = which == 0 ? 0.0 : 1.0;
+        if (!central(dth, dph, h, d1) || !central(dth, dph, 0.5 * h, d2)) return false;
+        std::vector<double> &out = which == 0 ? dTheta : dPhi;
+        out.resize(d1.size());
+        for (size_t k = 0; k < d1.size(); ++k) out[k] = (4.0 * d2[k] - d1[k]) / 3.0;
+    }
+    return true;
+}
+
+namespace {
+bool arc_f64(const std::array<double, 3> &s0, const std::array<double, 3> &sd, const std::array<double, 3> &e0,
+      const std::array<double, 3> &ed, float arclength, uint32_t N, std::vector<std::array<double, 3>> &P,
+      const double *bulgeIn, double *bulgeOut, bool *degenerate)
+{
+    const double ds = (double)(arclength / (float)N);
+    P.assign(N + 1, { 0.0, 0.0, 0.0 });
     for (int a = 0; a < 3; ++a) {
         P[0][a] = s0[a];
         P[1][a] = s0[a] + ds * sd[a];
@@ -62,12 +138,14 @@ bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vec
     const double c = std::sqrt(c2);
     if (!(c > 0.0) || c > (double)M * ds * (1.0 + 1e-12)) return false;
     for (double &x : u) x /= c;
-    // bulge direction v: startDir minus its chord component
-    double v[3], vd = sd[0] * u[0] + sd[1] * u[1] + sd[2] * u[2], vn = 0.0;
+    // bulge direction v: startDir (or the given bulge) minus its chord component
+    const double *bs = bulgeIn ? bulgeIn : sd.data();
+    double v[3], vd = bs[0] * u[0] + bs[1] * u[1] + bs[2] * u[2], vn = 0.0;
     for (int a = 0; a < 3; ++a) {
-        v[a] = sd[a] - vd * u[a];
+        v[a] = bs[a] - vd * u[a];
         vn += v[a] * v[a];
     }
+    if (degenerate) *degenerate = std::sqrt(vn) < 1e-9;
     if (std::sqrt(vn) < 1e-9) {
         int k = 0;
         if (std::fabs(u[1]) < std::fabs(u[0])) k = 1;
@@ -83,8 +161,9 @@ bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vec
     }
     vn = std::sqrt(vn);
     for (double &x : v) x /= vn;
+    if (bulgeOut)
+        for (int a = 0; a < 3; ++a) bulgeOut[a] = v[a];
     const double ratio = c / ds;
-    positions.assign(N + 1, Vec3());
     if (ratio >= (double)M * (1.0 - 1e-15)) {
         for (uint32_t j = 1; j < M; ++j)
             for (int a = 0; a < 3; ++a) P[1 + j][a] = P[1][a] + ds * (double)j * u[a];
@@ -105,9 +184,9 @@ bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t N, std::vec
             for (int a = 0; a < 3; ++a) P[1 + j][a] = C[a] + R * (std::sin(phi) * u[a] + std::cos(phi) * v[a]);
         }
     }
-    for (uint32_t i = 0; i <= N; ++i) positions[i] = Vec3((float)P[i][0], (float)P[i][1], (float)P[i][2]);
     return true;
 }
+}  // namespace
 
 }  // namespace PathGeneration
 }  // namespace twisty

@@ -60,6 +60,43 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
     return 0;
 }
 
+int twisty_host_rt_table_eps_derivative(float mu, float eps, float scatter, float bias, uint32_t numStepsInt,
+      uint32_t numCurvatureSteps, float minBound, float maxBound, int maxBoundFollowsEps, float ds, int threads,
+      double *dTableDEps)
+{
+    if (!dTableDEps || numCurvatureSteps < 2 || numStepsInt < 4 || numStepsInt % 2 != 0 || !(ds > 0.0f) ||
+          !(eps > 0.0f))
+        return 1;
+    WeightingParameters wp;
+    wp.mu = mu;
+    wp.eps = eps;
+    wp.scatter = scatter;
+    wp.bias = bias;
+    wp.numStepsInt = numStepsInt;
+    wp.numCurvatureSteps = numCurvatureSteps;
+    wp.minBound = minBound;
+    wp.maxBound = maxBound;
+    try {
+        const std::vector<double> d =
+              PathWeighting::WeightLookupTableIntegral::EpsDerivative(wp, ds, maxBoundFollowsEps != 0, threads);
+        std::memcpy(dTableDEps, d.data(), d.size() * sizeof(double));
+    } catch (const std::exception &) {
+        return 1;
+    }
+    return 0;
+}
+
+int twisty_host_initial_arc_exit_jacobian(const float boundary[13], uint32_t numSegments, double *dPosDTheta,
+      double *dPosDPhi)
+{
+    if (!boundary || !dPosDTheta || !dPosDPhi) return 1;
+    std::vector<double> a, b;
+    if (!PathGeneration::InitialArcExitJacobian(from13(boundary), numSegments, a, b)) return 1;
+    std::memcpy(dPosDTheta, a.data(), a.size() * sizeof(double));
+    std::memcpy(dPosDPhi, b.data(), b.size() * sizeof(double));
+    return 0;
+}
+
 int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut)
 {
     if (!boundary || !positionsOut) return 1;

@@ -129,6 +129,12 @@ class WeightLookupTableIntegral : public BaseWeightLookupTable {
     void Initialize(int threads);
     std::string ExportFilename() const override { return "RT.csv"; }
     static float Integrate(float curvature, const WeightingParameters &wp, float ds, std::vector<double> &scratch);
+    // d T[i] / d eps for every bin, eps the config's `eps` slot (the Gaussian
+    // phase width after the reference's argument swap), FP64.  When
+    // maxBoundFollowsEps, maxBound = 10/eps moves with eps
+    // (ExperimentRunner.cpp:168-171) and so do the Simpson nodes.
+    static std::vector<double> EpsDerivative(const WeightingParameters &wp, float ds, bool maxBoundFollowsEps,
+          int threads);
 };
 
 // SimpleWeightLookupTable (PathWeightUtils.h:210-218, .cpp:436-485).
@@ -152,6 +158,13 @@ float CalculateMinimumArclength(const PerturbUtils::BoundaryConditions &bc, uint
 // a planar circular arc of N-2 chords of length ds between them.  Returns false
 // when the boundary conditions admit no such arc (L below the minimum).
 bool InitialArc(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments, std::vector<Vec3> &positions);
+// The same curve in FP64, (N+1)*3 doubles.
+bool InitialArcF64(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments, std::vector<double> &positions);
+// d positions / d theta and d phi of the end direction e = (sin theta cos phi,
+// sin theta sin phi, cos theta) at the boundary's own end direction, FP64
+// (Richardson-extrapolated central differences of InitialArcF64).
+bool InitialArcExitJacobian(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments,
+      std::vector<double> &dTheta, std::vector<double> &dPhi);
 }  // namespace PathGeneration
 
 // ---------------------------------------------------------------------------

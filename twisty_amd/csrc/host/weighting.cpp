@@ -1,5 +1,6 @@
 // weighting.cpp -- curvature -> segment-weight tables (PathWeightUtils.cpp).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -187,6 +188,85 @@ float WeightLookupTableIntegral::Integrate(float curvature, const WeightingParam
 {
     const NodeSet ns = make_nodes(wp, ds);
     return integrate_with_nodes(ns, curvature, ds, wp, scratch);
+}
+
+// d T[i] / d eps, with m = eps the phase width and c = mu the regulariser
+// (swapped as the reference passes them).  Per node p_j = start + j h with
+// Simpson coefficient a_j (1/3, 4/3 or 2/3 of h):
+//   T = B sum_j a_j h F(p_j),  F(p) = p e^{g(p)} s(p),
+//   g = bds Phi(p; m) - c^2 p^2 / 2,  s = sin(kds p) / kds (p at kds = 0),
+//   Phi = A(m) e^{-m p^2/2},  A = sqrt(pi m / 2) / (1 - e^{-2/m}),
+// so with h = maxBound / n and maxBound = 10 / m (dh/dm = -h/m, dp_j/dm = -p_j/m):
+//   dT/dm = B sum_j a_j h [ -F/m - (p_j/m) F_p + F_m ],
+//   F_p = e^g (1 + p g_p) s + p e^g cos(kds p),  g_p = -bds m p Phi - c^2 p,
+//   F_m = F bds Phi (1/(2m) - (2/m^2) e^{-2/m} / (1 - e^{-2/m}) - p^2/2).
+// The reference evaluates Phi in float and rounds T to float; the derivative is
+// that of the exact expression.  Entries the reference replaces by the table
+// minimum (<= 0) take the derivative of that minimum's entry.
+std::vector<double> WeightLookupTableIntegral::EpsDerivative(const WeightingParameters &wp, float ds,
+      bool maxBoundFollowsEps, int threads)
+{
+    if (wp.numStepsInt < 4 || wp.numStepsInt % 2 != 0) throw std::runtime_error("numStepsInt must be even and >= 4");
+    const uint32_t n = wp.numStepsInt, nb = wp.numCurvatureSteps;
+    const double m = wp.eps, c = wp.mu;
+    const double bds = (float)(wp.scatter * ds);
+    const double h = (double)(wp.maxBound - wp.minBound) / (double)n;
+    const double start = wp.minBound;
+    const double B = (double)wp.bias / (2.0 * (double)TwistyPi * (double)TwistyPi);
+    const double em = std::exp(-2.0 / m);
+    const double A = std::sqrt(M_PI * m * 0.5) / (1.0 - em);
+    const double dlnA = 0.5 / m - (2.0 / (m * m)) * em / (1.0 - em);
+    const double follow = maxBoundFollowsEps ? 1.0 : 0.0;
+    struct Node {
+        double p, coef, eg, gp, phim;  // abscissa, a_j h, e^g, g_p, bds * dPhi/dm
+    };
+    std::vector<Node> nodes;
+    auto add = [&](double p, double a) {
+        const double phi = A * std::exp(-m * p * p * 0.5);
+        const double g = bds * phi - c * c * p * p * 0.5;
+        const double eg = std::exp(g);
+        if (eg == 0.0) return;
+        nodes.push_back({ p, a * h, eg, -bds * m * p * phi - c * c * p, bds * phi * (dlnA - p * p * 0.5) });
+    };
+    add(start, 1.0 / 3.0);
+    add(start + (double)n * h, 1.0 / 3.0);
+    for (uint32_t i = 1; i < n / 2; ++i) add(start + (double)(2 * i - 1) * h, 4.0 / 3.0);
+    for (uint32_t i = 1; i < n / 2 - 1; ++i) add(start + (double)(2 * i) * h, 2.0 / 3.0);
+    const float maxK = (float)((2.0 / (double)ds) * (double)1.01f);
+    const float step = maxK / (float)(nb - 1);
+    std::vector<double> d(nb), val(nb);
+    if (threads < 1) threads = hardware_threads();
+    std::vector<std::thread> pool;
+    const uint32_t per = (nb + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const uint32_t b = std::min<uint32_t>(nb, t * per), e = std::min<uint32_t>(nb, b + per);
+        pool.emplace_back([&, b, e] {
+            for (uint32_t i = b; i < e; ++i) {
+                const float k = (i == 0) ? 0.0f : ((float)(int64_t)i * step);
+                const double kds = (float)(k * ds);
+                double acc = 0.0, v = 0.0;
+                for (const Node &nd : nodes) {
+                    const double p = nd.p;
+                    const double s = kds > 0.0 ? std::sin(kds * p) / kds : p;
+                    const double sp = kds > 0.0 ? std::cos(kds * p) : 1.0;
+                    const double F = p * nd.eg * s;
+                    const double Fp = nd.eg * (1.0 + p * nd.gp) * s + p * nd.eg * sp;
+                    const double Fm = F * nd.phim;
+                    acc += nd.coef * (Fm - follow * (F + p * Fp) / m);
+                    v += nd.coef * F;
+                }
+                d[i] = B * acc;
+                val[i] = B * v;
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+    uint32_t argmin = 0;
+    for (uint32_t i = 1; i < nb; ++i)
+        if ((float)val[i] < (float)val[argmin] && (float)val[i] >= 0.0f) argmin = i;
+    for (uint32_t i = 0; i < nb; ++i)
+        if ((float)val[i] <= 0.0f) d[i] = d[argmin];
+    return d;
 }
 
 void WeightLookupTableIntegral::Initialize() { Initialize(hardware_threads()); }

@@ -10,7 +10,8 @@ from . import capi
 
 LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
-            "twisty_host_norm_log10", "twisty_host_run_config"]
+            "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
+            "twisty_host_initial_arc_exit_jacobian"]
 
 _lib = None
 
@@ -45,6 +46,9 @@ def lib():
         L.twisty_host_norm_log10.argtypes = [u32, f32, vp]
         L.twisty_host_norm_log10.restype = f64
         L.twisty_host_run_config.argtypes = [C.c_char_p, C.POINTER(HostResult), C.c_char_p, u32]
+        L.twisty_host_rt_table_eps_derivative.argtypes = [f32, f32, f32, f32, u32, u32, f32, f32, C.c_int, f32,
+                                                          C.c_int, vp]
+        L.twisty_host_initial_arc_exit_jacobian.argtypes = [vp, u32, vp, vp]
         _lib = L
     return _lib
 
@@ -74,6 +78,32 @@ def build_rt_table(ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000
     if rc != 0:
         raise ValueError("twisty_host_build_rt_table rejected its arguments")
     return WeightTable(t, meta)
+
+
+def rt_table_eps_derivative(ds, mu, eps, scatter, bias=1.0, numStepsInt=20000, numCurvatureSteps=4096,
+                            minBound=0.0, maxBound=None, threads=0):
+    """d T / d eps (FP64) of build_rt_table's table; maxBound = None means 10/eps
+    and is differentiated along with eps."""
+    follow = maxBound is None
+    if maxBound is None:
+        maxBound = float(np.float32(10.0) / np.float32(eps))
+    d = np.zeros(numCurvatureSteps, np.float64)
+    rc = lib().twisty_host_rt_table_eps_derivative(mu, eps, scatter, bias, numStepsInt, numCurvatureSteps, minBound,
+                                                   maxBound, int(follow), ds, threads, _p(d))
+    if rc != 0:
+        raise ValueError("twisty_host_rt_table_eps_derivative rejected its arguments")
+    return d
+
+
+def initial_arc_exit_jacobian(boundary, N):
+    """(d positions / d theta, d positions / d phi), each (N+1, 3) FP64, for the
+    end direction (sin theta cos phi, sin theta sin phi, cos theta)."""
+    b = np.ascontiguousarray(boundary, np.float32)
+    a = np.zeros((N + 1) * 3, np.float64)
+    c = np.zeros((N + 1) * 3, np.float64)
+    if lib().twisty_host_initial_arc_exit_jacobian(_p(b), N, _p(a), _p(c)) != 0:
+        raise ValueError("no initial arc for these boundary conditions")
+    return a.reshape(N + 1, 3), c.reshape(N + 1, 3)
 
 
 def boundary13(start, start_dir, end, end_dir, arclength):

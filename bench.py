@@ -93,6 +93,53 @@ def cpu_baseline(table, pos, w, target_seconds):
     return out, ref_part
 
 
+def measure_adjoint(ctx, w, reps=3):
+    """Forward + adjoint device time at the same workload (HIP events)."""
+    out = {}
+    for seg, key in ((False, "table_only"), (True, "full")):
+        ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
+        tot, adj = [], []
+        for _ in range(reps):
+            ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
+            tot.append(ctx.last_timing()[1])
+            adj.append(ctx.last_adjoint_ms())
+        out[key] = {"fwd_adjoint_ms": statistics.fmean(tot), "adjoint_ms": statistics.fmean(adj)}
+    return out
+
+
+def measure_batched(ctx, capi, n_exp=4096, autotune=True):
+    """The batched workload in one graph-replayed launch vs one plain run per experiment."""
+    from twisty_amd import workloads
+
+    t0 = time.perf_counter()
+    wl = workloads.batch_experiments(n_exp=n_exp, seed=1, K=100)
+    setup_s = time.perf_counter() - t0
+    exps = wl["experiments"]
+    jw = sum(x["paths"] * (x["N"] - 1) for x in exps)
+    b = capi.Batch(ctx, exps, wl["tables"], wl["meta"])
+    tuned, tried = None, []
+    if autotune:
+        tuned, tried = b.autotune(reps=1)
+    b.run()
+    ms = min(b.run()[2] for _ in range(3))
+    sched = b.get_schedule()
+    b.close()
+    t0 = time.perf_counter()
+    for x in exps:
+        m = wl["meta"][x["table"]]
+        ctx.set_tables(wl["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]), float(m[3]), float(m[4]))
+        ctx.set_curve(x["positions"], x["boundary"])
+        ctx.run(x["paths"], x["K"], seed=x["seed"])
+    plain_s = time.perf_counter() - t0
+    return {"experiments": len(exps), "paths": sum(x["paths"] for x in exps), "joint_weights": jw,
+            "batched_ms": ms, "batched_joint_weights_per_s": jw / (ms / 1e3), "schedule": sched,
+            "autotune_candidates": len(tried),
+            "plain_loop_ms": plain_s * 1e3, "plain_joint_weights_per_s": jw / plain_s,
+            "batched_vs_plain": plain_s * 1e3 / ms, "setup_s": setup_s,
+            "workload": "4096 experiments: N log-uniform [16,256], L/D ~ U[1.05,3], end direction uniform on "
+                        "the sphere, paths log-uniform [2^10,2^16], K=100, 8 shared tables sigma ~ U[0.03,0.2]"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +148,8 @@ def main(argv=None):
     ap.add_argument("--paths", type=int, default=WORKLOAD["paths"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -154,6 +203,16 @@ def main(argv=None):
         "fwd_adjoint_ms": None,
         "batched": None,
     }
+    if not a.no_adjoint:
+        adj = measure_adjoint(ctx, w)
+        out["fwd_adjoint_ms"] = adj["full"]["fwd_adjoint_ms"]
+        out["adjoint"] = {"forward_ms": fwd_ms, **adj,
+                          "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
+                          "ratio_table_only": adj["table_only"]["fwd_adjoint_ms"] / fwd_ms,
+                          "gradients": "full: dE/dT (4096) + dE/d(initial segments) -> sigma, exit angles; "
+                                       "table_only: dE/dT (-> sigma)"}
+    if not a.no_batched:
+        out["batched"] = measure_batched(ctx, capi)
     if not a.no_cpu_baseline:
         cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
         out["cpu_baseline"] = cb

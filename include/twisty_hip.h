@@ -159,6 +159,65 @@ int twisty_hip_set_adjoint_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, 
  * twisty_hip_run_adjoint; 0 after a plain run. */
 int twisty_hip_last_adjoint_timing(twisty_hip_ctx *ctx, float *adjointMs);
 
+/* ------------------------------------------------------------------------
+ * Batched experiments.  One persistent launch (captured once in a HIP graph and
+ * replayed) runs many experiments with different segment counts, path counts,
+ * geometries and tables.  Replaces a driver's serial loop of one
+ * ExperimentRunner per experiment (Experiments/BeamSpreadExperiment.cpp:209-284,
+ * Experiments/FullExperiment.cpp:204-232).
+ * ------------------------------------------------------------------------ */
+typedef struct twisty_hip_batch twisty_hip_batch;
+
+/* One experiment of a batch.  Paths restart from the experiment's initial curve
+ * (chainLength 1); draws as twisty_hip_run with path ids 0..numPaths-1. */
+typedef struct twisty_hip_experiment {
+    uint64_t num_paths;
+    uint64_t seed;
+    uint32_t num_segments;            /* N: 4 <= N <= 64 * segments_per_lane */
+    uint32_t perturbations_per_path;  /* K >= 1 */
+    uint32_t table_index;             /* < numTables of the batch */
+    uint32_t positions_offset;        /* first point (not float) of its curve in `positions` */
+    float arclength;                  /* the boundary's arclength (ds = arclength / N) */
+} twisty_hip_experiment;
+
+/* Launch schedule of a batch: blockSize 256 or 512; blocksPerCU 0 = as many as
+ * LDS allows; segmentsPerLane picks each experiment's lanes per path (the
+ * smallest power of two >= 2 with ceil(N / lanes) <= segmentsPerLane);
+ * tilePaths = paths per work item. */
+typedef struct twisty_hip_batch_schedule {
+    uint32_t block_size;
+    uint32_t blocks_per_cu;
+    uint32_t segments_per_lane;
+    uint32_t tile_paths;
+} twisty_hip_batch_schedule;
+
+/* Uploads a batch: experiments, numTables tables of numBins floats with their
+ * grids (tableMeta: minCurvature, maxCurvature, curvatureStepSize, ds,
+ * absorption per table), and the concatenated initial curves (x,y,z floats;
+ * experiment e uses points positions_offset .. positions_offset + N).  Tables
+ * are shared: an experiment weighs its joints with its table's curvature grid
+ * and its own ds = arclength / N for curvature and absorption.  Copies
+ * everything; the schedule starts at the built-in choice. */
+int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *experiments, uint32_t numExperiments,
+      const float *tables, uint32_t numBins, uint32_t numTables, const float *tableMeta, const float *positions,
+      uint64_t numPoints, twisty_hip_batch **out);
+
+int twisty_hip_batch_destroy(twisty_hip_batch *batch);
+
+int twisty_hip_batch_set_schedule(twisty_hip_batch *batch, const twisty_hip_batch_schedule *schedule);
+int twisty_hip_batch_get_schedule(twisty_hip_batch *batch, twisty_hip_batch_schedule *schedule);
+
+/* Runs the batch (graph replay).  out: numExperiments partials.
+ * perPathLog10 (nullable): every experiment's paths back to back, in
+ * experiment order.  deviceMs (nullable): HIP-event time of the replay. */
+int twisty_hip_batch_run(twisty_hip_batch *batch, twisty_hip_partials *out, float *perPathLog10, float *deviceMs);
+
+/* Times every candidate schedule (reps replays each, best kept), leaves the
+ * fastest set and returns it.  tried / triedMs (nullable, capacity cap)
+ * receive the candidates and their times; *numTried their count. */
+int twisty_hip_batch_autotune(twisty_hip_batch *batch, uint32_t reps, twisty_hip_batch_schedule *best,
+      twisty_hip_batch_schedule *tried, float *triedMs, uint32_t cap, uint32_t *numTried);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,144 @@
+"""Persistent batched kernel (twisty_hip_batch_*) against plain runs and the oracle.
+
+Each experiment of a batch runs the same per-path code as twisty_hip_run with
+the same draws (Philox key = the experiment's seed, path ids 0..n-1), so when
+both use the same lanes per path the per-path log10 weights are bit-identical
+and the FP64 partials agree to rounding of the merge order (1e-9 relative).
+The plain run itself is checked against the oracle in test_gpu_forward; one
+experiment is also checked against the oracle directly here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import cases
+from twisty_amd import capi, host, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def plain_lanes(N):
+    G = 1
+    while G < 64 and (N + G - 1) // G > 16:
+        G <<= 1
+    return G
+
+
+def batch_lanes(N, spl):
+    G = 2
+    while G < 64 and (N + G - 1) // G > spl:
+        G <<= 1
+    return G
+
+
+def plain_run(ctx, x, tables, meta):
+    m = meta[x["table"]]
+    ctx.set_tables(tables[x["table"]], float(m[0]), float(m[1]), float(m[2]), float(m[3]), float(m[4]))
+    ctx.set_curve(x["positions"], x["boundary"])
+    return ctx.run(x["paths"], x["K"], seed=x["seed"], per_path=True)
+
+
+def hetero_batch():
+    return workloads.batch_experiments(n_exp=14, seed=7, K=30, n_range=(5, 300), paths_range=(200, 6000), n_tables=3,
+                                       bins=1024, numStepsInt=2000)
+
+
+def test_batched_matches_plain_runs(hip_device):
+    w = hetero_batch()
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    sched = b.get_schedule()
+    print(f"[batched hetero] schedule {sched}", flush=True)
+    res, pp, ms = b.run(per_path=True)
+    exact = 0
+    for x, r, g in zip(w["experiments"], res, pp):
+        ref = plain_run(ctx, x, w["tables"], w["meta"])
+        same_g = batch_lanes(x["N"], sched["segments_per_lane"]) == plain_lanes(x["N"])
+        a, c = g.astype(np.float64), ref["log10w"].astype(np.float64)
+        assert np.array_equal(np.isnan(a), np.isnan(c))
+        ok = np.isfinite(c)
+        if same_g:
+            assert np.array_equal(a[ok], c[ok]), f"N={x['N']}"
+            exact += 1
+        else:
+            assert np.abs(a[ok] - c[ok]).max() <= 1e-5 * np.abs(c[ok]).max()
+        assert r["num_paths"] == x["paths"] and r["num_valid"] == ref["num_valid"]
+        assert abs(r["log10_sum_w"] - ref["log10_sum_w"]) <= 1e-6 * abs(ref["log10_sum_w"]) + 1e-9
+    print(f"  {exact} of {len(res)} experiments bit-identical to plain runs; replay {ms:.3f} ms", flush=True)
+    b.close()
+    ctx.close()
+
+
+def test_batched_replays_are_deterministic(hip_device):
+    w = hetero_batch()
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    first, _, _ = b.run()
+    for _ in range(3):
+        again, _, _ = b.run()
+        assert again == first
+    b.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("sched", [(256, 0, 8, 1024), (512, 1, 16, 333), (256, 2, 16, 4096), (256, 0, 32, 100000)])
+def test_batched_schedules_agree(hip_device, sched):
+    w = hetero_batch()
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    base, pb, _ = b.run(per_path=True)
+    b.set_schedule(*sched)
+    other, po, ms = b.run(per_path=True)
+    print(f"[batched schedule {sched}] {ms:.3f} ms", flush=True)
+    for x, r0, r1, a, c in zip(w["experiments"], base, other, pb, po):
+        assert r0["num_valid"] == r1["num_valid"]
+        assert abs(r0["log10_sum_w"] - r1["log10_sum_w"]) <= 1e-5 * abs(r0["log10_sum_w"])
+        ok = np.isfinite(a)
+        if batch_lanes(x["N"], sched[2]) == batch_lanes(x["N"], 16):
+            assert np.array_equal(a[ok], c[ok])
+    b.close()
+    ctx.close()
+
+
+def test_batched_experiment_vs_oracle(hip_device, oracle):
+    """One experiment of a batch, per path against the FP64 oracle (same-bin rule of the forward tests)."""
+    case = cases.MID
+    table, pos32 = cases.setup(case, oracle)
+    x = {"N": case.N, "paths": 4000, "K": case.K, "seed": case.seed, "arclength": case.L, "table": 1,
+         "positions": pos32, "boundary": cases.bc13(case)}
+    other = dict(x, N=20, positions=host.initial_arc(host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2),
+                                                                 20), arclength=1.2, table=0, paths=1500, seed=5)
+    tables = np.stack([table.table, table.table])
+    meta = np.array([(table.minK, table.maxK, table.step, table.ds, case.absorption)] * 2, np.float32)
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, [other, x], tables, meta)
+    res, pp, _ = b.run(per_path=True)
+    ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, 4000, table, case.absorption, seed=case.seed)
+    g = pp[1].astype(np.float64)
+    ok = ref["valid"] & np.isfinite(ref["log10w"])
+    rel = np.abs(g[ok] - ref["log10w"][ok]) / np.abs(ref["log10w"][ok])
+    print(f"[batched vs oracle] median rel {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
+    assert np.median(rel) < 1e-6 and np.mean(rel < 1e-5) > 0.97
+    assert res[1]["num_valid"] == ref["n_valid"]
+    b.close()
+    ctx.close()
+
+
+def test_batched_autotune(hip_device):
+    w = workloads.batch_experiments(n_exp=64, seed=3, K=20, paths_range=(1 << 8, 1 << 12), bins=1024,
+                                    numStepsInt=2000)
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    base, _, _ = b.run()
+    best, tried = b.autotune(reps=1)
+    print(f"[batched autotune] best {best}; {len(tried)} candidates, "
+          f"{min(t for _, t in tried):.3f}..{max(t for _, t in tried):.3f} ms", flush=True)
+    assert len(tried) >= 8 and b.get_schedule()["block_size"] == best["block_size"]
+    after, _, ms = b.run()
+    assert abs(ms - min(t for _, t in tried)) < 0.5 * max(ms, 1e-3) + 1.0
+    for r0, r1 in zip(base, after):
+        assert r0["num_valid"] == r1["num_valid"]
+        assert abs(r0["log10_sum_w"] - r1["log10_sum_w"]) <= 1e-5 * abs(r0["log10_sum_w"])
+    b.close()
+    ctx.close()

@@ -9,10 +9,13 @@ Checks:
   * per-path bins equal the oracle's bins of the FP64 exit tangent, except for
     paths whose FP64 coordinate lies within 1e-3 of a bin edge (FP32 tangents
     differ from FP64 by ~1e-6);
-  * bins: the GPU histogram equals the FP64 sum over the oracle's per-path
-    weights binned with the GPU's own bins, |dh| <= 1e-4 h + 1e-9 (per-path
-    weights agree to 1e-5 relative in log10, about 1e-3 in weight at log10 w
-    ~ -40; most bins hold many paths);
+  * bin arithmetic: the GPU histogram equals an FP64 histogram of the GPU's
+    own per-path weights, |dh| <= 3e-5 h + 1e-12 (the float log10 output
+    carries ~1e-5 relative weight error at log10 w ~ -40);
+  * against the oracle's weights in the GPU's bins: L1 distance no larger than
+    the per-path weight differences allow, sum_p |w_gpu - w_ref| / W (the
+    per-path log10 weights themselves are checked to 1e-5 in the forward
+    tests, with the bin-edge allowance of the reference's lerp);
   * invariants: sum of bins = 1, bins >= 0, a constant table gives the path
     fraction of every bin.
 """
@@ -66,7 +69,7 @@ def test_exit_histogram_matches_oracle(hip_device, oracle, case, nt, nph, paths)
     print(f"[hist {case.name} {nt}x{nph} paths={paths}]", flush=True)
     ctx = cases.hip_context(case, oracle)
     ctx.set_exit_histogram(nt, nph)
-    res = ctx.run(paths, case.K, seed=case.seed)
+    res = ctx.run(paths, case.K, seed=case.seed, per_path=True)
     h, gbins = ctx.exit_histogram(paths)
     table, pos32 = cases.setup(case, oracle)
     ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, paths, table, case.absorption,
@@ -79,10 +82,20 @@ def test_exit_histogram_matches_oracle(hip_device, oracle, case, nt, nph, paths)
     assert not np.any(mism & ~near)
     assert h.shape == (nt, nph) and np.all(h >= 0)
     assert abs(h.sum() - 1.0) < 1e-5
+    g = res["log10w"].astype(np.float64)
+    hown = reference_histogram(g, gbins.astype(np.int64), nt * nph).reshape(nt, nph)
+    err = np.abs(h - hown)
+    print(f"  own weights: max |dh| {err.max():.3e} (max h {h.max():.3e})", flush=True)
+    assert np.all(err <= 3e-5 * hown + 1e-12)
     href = reference_histogram(ref["log10w"], gbins.astype(np.int64), nt * nph).reshape(nt, nph)
-    err = np.abs(h - href)
-    print(f"  max |dh| {err.max():.3e}, max h {h.max():.3e}", flush=True)
-    assert np.all(err <= 1e-4 * href + 1e-9)
+    ok = ref["valid"] & np.isfinite(ref["log10w"])
+    m = ref["log10w"][ok].max()
+    wr = 10.0 ** (ref["log10w"][ok] - m)
+    wg = 10.0 ** (g[ok] - m)
+    allow = np.abs(wg - wr).sum() / wr.sum()
+    l1 = np.abs(h - href).sum()
+    print(f"  oracle weights: L1 {l1:.3e} allowance {allow:.3e}", flush=True)
+    assert l1 <= 2 * allow + 1e-6
     ctx.close()
 
 

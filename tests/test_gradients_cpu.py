@@ -33,6 +33,24 @@ def test_oracle_adjoint_forward_matches_simulate(oracle):
     assert np.abs(a["log10w"] - s["log10w"]).max() < 1e-9
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_oracle_adjoint_identities(oracle, mode):
+    """Table homogeneity: sum_b T[b] dE/dT[b] = (N-1)/ln 10.  Invariance of E
+    under a rotation or uniform scaling of the initial segments:
+    sum_i d_i x dE/dd_i = 0, sum_i d_i . dE/dd_i = 0."""
+    t, pos = small_case(oracle)
+    N = 20
+    r = oracle.adjoint(pos, N, 1.2, 20, 4096, t, 0.004, lerp_mode=mode)
+    # (mode 0 lerps in float: each joint's value carries one float rounding)
+    tol = 1e-10 if mode == 1 else 1e-6
+    assert abs(t.table.astype(np.float64) @ r["d_table"] - (N - 1) / math.log(10)) < tol * (N - 1)
+    d = np.diff(pos, axis=0)
+    lam = r["d_seg"]
+    scale = np.linalg.norm(d, axis=1) @ np.linalg.norm(lam, axis=1)
+    assert np.abs(np.cross(d, lam).sum(axis=0)).max() < 1e-10 * scale
+    assert abs(np.sum(d * lam)) < 1e-10 * scale
+
+
 @pytest.mark.parametrize("mode,h,tol", [(1, 1e-4, 1e-7), (0, 1e-3, 2e-4)])
 def test_oracle_table_gradient_fd(oracle, mode, h, tol):
     t, pos = small_case(oracle)

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 
 ARCH = "gfx950"
-HIP_SOURCES = ["forward.hip", "histogram.hip", "adjoint.hip", "capi.hip"]
+HIP_SOURCES = ["forward.hip", "histogram.hip", "adjoint.hip", "batched.hip", "capi.hip", "batch_capi.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -21,7 +21,33 @@ EXPORTED = [
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
+    "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
+    "twisty_hip_batch_get_schedule", "twisty_hip_batch_run", "twisty_hip_batch_autotune",
 ]
+
+
+class Experiment(C.Structure):
+    _fields_ = [
+        ("num_paths", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("num_segments", C.c_uint32),
+        ("perturbations_per_path", C.c_uint32),
+        ("table_index", C.c_uint32),
+        ("positions_offset", C.c_uint32),
+        ("arclength", C.c_float),
+    ]
+
+
+class BatchSchedule(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_uint32),
+        ("blocks_per_cu", C.c_uint32),
+        ("segments_per_lane", C.c_uint32),
+        ("tile_paths", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Partials(C.Structure):
@@ -78,6 +104,12 @@ def _declare(L):
     L.twisty_hip_run_adjoint.argtypes = [vp, u64, u64, u32, u64, C.POINTER(Partials), vp, vp]
     L.twisty_hip_set_adjoint_schedule.argtypes = [vp, u32, u32, u32]
     L.twisty_hip_last_adjoint_timing.argtypes = [vp, C.POINTER(f32)]
+    L.twisty_hip_batch_create.argtypes = [vp, vp, u32, vp, u32, u32, vp, vp, u64, C.POINTER(vp)]
+    L.twisty_hip_batch_destroy.argtypes = [vp]
+    L.twisty_hip_batch_set_schedule.argtypes = [vp, C.POINTER(BatchSchedule)]
+    L.twisty_hip_batch_get_schedule.argtypes = [vp, C.POINTER(BatchSchedule)]
+    L.twisty_hip_batch_run.argtypes = [vp, vp, vp, C.POINTER(f32)]
+    L.twisty_hip_batch_autotune.argtypes = [vp, u32, C.POINTER(BatchSchedule), vp, vp, u32, C.POINTER(u32)]
 
 
 def device_count():
@@ -193,6 +225,79 @@ class Context:
         a = C.c_float()
         self._check(lib().twisty_hip_last_adjoint_timing(self._h, C.byref(a)))
         return a.value
+
+
+class Batch:
+    """A batch of experiments on one context (twisty_hip_batch_*).
+
+    experiments: list of dicts with N, paths, K, seed, table, arclength and
+    positions ((N+1, 3)); tables: (numTables, bins); meta: (numTables, 5) =
+    minK, maxK, step, ds, absorption.
+    """
+
+    def __init__(self, ctx, experiments, tables, meta):
+        self.ctx = ctx
+        n = len(experiments)
+        arr = (Experiment * n)()
+        pos = []
+        off = 0
+        self.paths = []
+        for i, x in enumerate(experiments):
+            p = np.ascontiguousarray(x["positions"], np.float32).reshape(-1, 3)
+            assert p.shape[0] == x["N"] + 1
+            arr[i] = Experiment(x["paths"], x.get("seed", 1), x["N"], x["K"], x["table"], off, x["arclength"])
+            pos.append(p)
+            off += p.shape[0]
+            self.paths.append(x["paths"])
+        self.positions = np.ascontiguousarray(np.concatenate(pos), np.float32)
+        t = np.ascontiguousarray(tables, np.float32)
+        m = np.ascontiguousarray(meta, np.float32)
+        self.n = n
+        self._h = C.c_void_p()
+        ctx._check(lib().twisty_hip_batch_create(ctx._h, C.cast(arr, C.c_void_p), n, _ptr(t), t.shape[1], t.shape[0],
+                                                 _ptr(m), _ptr(self.positions), self.positions.shape[0],
+                                                 C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().twisty_hip_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, block_size=0, blocks_per_cu=0, segments_per_lane=0, tile_paths=0):
+        s = BatchSchedule(block_size, blocks_per_cu, segments_per_lane, tile_paths)
+        self.ctx._check(lib().twisty_hip_batch_set_schedule(self._h, C.byref(s)))
+
+    def get_schedule(self):
+        s = BatchSchedule()
+        self.ctx._check(lib().twisty_hip_batch_get_schedule(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def run(self, per_path=False):
+        """(list of partial dicts, per-path log10 list or None, device ms)."""
+        out = (Partials * self.n)()
+        pp = np.empty(sum(self.paths), np.float32) if per_path else None
+        ms = C.c_float()
+        self.ctx._check(lib().twisty_hip_batch_run(self._h, C.cast(out, C.c_void_p), _ptr(pp), C.byref(ms)))
+        res = [out[i].as_dict() for i in range(self.n)]
+        split = None
+        if per_path:
+            split = np.split(pp, np.cumsum(self.paths)[:-1])
+        return res, split, ms.value
+
+    def autotune(self, reps=2, cap=64):
+        best = BatchSchedule()
+        tried = (BatchSchedule * cap)()
+        ms = np.zeros(cap, np.float32)
+        n = C.c_uint32()
+        self.ctx._check(lib().twisty_hip_batch_autotune(self._h, reps, C.byref(best), C.cast(tried, C.c_void_p),
+                                                        _ptr(ms), cap, C.byref(n)))
+        return best.as_dict(), [(tried[i].as_dict(), float(ms[i])) for i in range(min(n.value, cap))]
 
 
 def estimator(res, norm=1.0):

@@ -13,91 +13,12 @@
 
 #include "../../include/twisty_hip.h"
 #include "adjoint.hpp"
+#include "ctx.hpp"
 #include "forward.hpp"
 
 using namespace twisty_hip;
 
-struct twisty_hip_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    uint32_t numCUs = 0;
-    uint32_t ldsPerBlock = 0;
-    uint32_t ldsPerCU = 0;
-    std::string lastError;
-
-    float *dTable = nullptr;
-    uint32_t tableSize = 0;
-    float minK = 0, maxK = 0, step = 0, ds = 0, absorption = 0;
-    bool haveTables = false;
-
-    float *dInitSeg = nullptr;
-    uint32_t N = 0;
-    float arclength = 0;
-    float endDir[3] = { 0, 0, 1 };
-    bool haveCurve = false;
-
-    BlockPartial *dPartials = nullptr;
-    uint64_t partialsCap = 0;
-    FinalOut *dFinal = nullptr;
-
-    float *dPerPath = nullptr;
-    uint64_t perPathCap = 0;
-    float *dPerJoint = nullptr;
-    uint64_t perJointCap = 0;
-
-    // Exit-direction histogram.
-    uint32_t histTheta = 0, histPhi = 0;
-    float2 *dPathLog2 = nullptr;
-    uint64_t pathLog2Cap = 0;
-    uint32_t *dPathBin = nullptr;
-    uint64_t pathBinCap = 0;
-    float *dHistPartial = nullptr;
-    uint64_t histPartialCap = 0;
-    double *dHist = nullptr;
-    uint64_t histCap = 0;
-    uint64_t lastHistPaths = 0;
-    uint32_t lastHistBins = 0;
-
-    // Adjoint.
-    float *dFinalSeg = nullptr;
-    uint64_t finalSegCap = 0;
-    float *dGradTPartial = nullptr;
-    uint64_t gradTPartialCap = 0;
-    float *dGradSPartial = nullptr;
-    uint64_t gradSPartialCap = 0;
-    double *dGradT = nullptr;
-    uint64_t gradTCap = 0;
-    double *dGradS = nullptr;
-    uint64_t gradSCap = 0;
-
-    uint32_t schedG = 0, schedBlock = 0, schedBpc = 0;
-    uint32_t adjG = 0, adjBlock = 0, adjBpc = 0;
-    float lastFwdMs = 0.0f, lastTotalMs = 0.0f, lastAdjMs = 0.0f;
-};
-
 namespace {
-
-int fail(twisty_hip_ctx *ctx, int code, const std::string &msg)
-{
-    if (ctx) ctx->lastError = msg;
-    return code;
-}
-
-int hipfail(twisty_hip_ctx *ctx, hipError_t e, const char *what)
-{
-    return fail(ctx, TWISTY_HIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(ctx, expr)                                                                                        \
-    do {                                                                                                          \
-        hipError_t _e = (expr);                                                                                   \
-        if (_e != hipSuccess) return hipfail((ctx), _e, #expr);                                                   \
-    } while (0)
-
-bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
-
-uint32_t table_stride(uint32_t n) { return (n + 63u) & ~63u; }
 
 // Resolves a schedule for segment count N.  Zero fields are filled in: G from N
 // (about 16 segments per lane), then the largest compiled block size whose LDS
@@ -146,19 +67,6 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
         if (bpc * block > 2048) bpc = 2048 / block;
         if (bpc < 1) bpc = 1;
     }
-    return TWISTY_HIP_OK;
-}
-
-template<class T>
-int ensure_dev(twisty_hip_ctx *ctx, T *&ptr, uint64_t &cap, uint64_t n, const char *what)
-{
-    if (n <= cap && ptr) return TWISTY_HIP_OK;
-    hipError_t e = hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    if (e != hipSuccess) return hipfail(ctx, e, what);
-    if ((e = hipMalloc(&ptr, n * sizeof(T))) != hipSuccess) return hipfail(ctx, e, what);
-    cap = n;
     return TWISTY_HIP_OK;
 }
 

@@ -18,105 +18,9 @@
 // compile-time offset from one per-perturbation base address.
 #include "device_common.hpp"
 #include "forward.hpp"
+#include "path_kernels.hpp"
 
 namespace twisty_hip {
-
-// Block-wide merge of the per-lane accumulators into partials[blockIdx.x].
-__device__ void block_reduce_store(LseAcc acc, float *scratch, BlockPartial *partials)
-{
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        LseAcc b;
-        b.m = __shfl_xor(acc.m, o, 64);
-        b.s.hi = __shfl_xor(acc.s.hi, o, 64);
-        b.s.lo = __shfl_xor(acc.s.lo, o, 64);
-        b.s2.hi = __shfl_xor(acc.s2.hi, o, 64);
-        b.s2.lo = __shfl_xor(acc.s2.lo, o, 64);
-        b.nValid = (uint32_t)__shfl_xor((int)acc.nValid, o, 64);
-        b.nInvalid = (uint32_t)__shfl_xor((int)acc.nInvalid, o, 64);
-        b.nClamped = (uint32_t)__shfl_xor((int)acc.nClamped, o, 64);
-        lse_merge(acc, b);
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
-    LseAcc *sh = reinterpret_cast<LseAcc *>(scratch);
-    if ((threadIdx.x & 63) == 0) sh[wave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        LseAcc a = sh[0];
-        for (int w = 1; w < nWaves; ++w) lse_merge(a, sh[w]);
-        BlockPartial p;
-        p.m = a.m;
-        p.s_hi = a.s.hi;
-        p.s_lo = a.s.lo;
-        p.s2_hi = a.s2.hi;
-        p.s2_lo = a.s2.lo;
-        p.nValid = a.nValid;
-        p.nInvalid = a.nInvalid;
-        p.nClamped = a.nClamped;
-        partials[blockIdx.x] = p;
-    }
-}
-
-// One perturbation applied by lane j of its group: read this lane's segments of
-// the sub-chain [left, right), sum them across the group into the axis
-// p_right - p_left, build the rotation and write the rotated segments back.
-// Window slots past this lane's count are redirected to the lane's zeroed
-// scratch row: they read zeros, add nothing to the axis and write zeros back,
-// so every load and store is unconditional with a compile-time offset.
-// scrB[w] is the byte address that lands on the scratch row after slot w's
-// offset is added.  (Skipping the last slot behind a wave-uniform branch when
-// no path of the wave needs it measured slower: the compiler copies the window
-// registers around the branch.)
-template<int G, int BLOCK>
-__device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
-      const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
-{
-    constexpr int WMAX = (25 + G - 1) / G;       // sub-chain slots per lane (diff <= 25)
-    constexpr uint32_t SLOTB = 12u * BLOCK;      // bytes per slot row
-    constexpr uint32_t COMPB = 4u * BLOCK;       // bytes between components
-    const uint32_t diff = rec.packed & 0xFFFFu;
-    const uint32_t left = rec.packed >> 16;
-    const uint32_t right = left + diff;
-    // First segment index >= left owned by this lane, and how many of this
-    // lane's segments lie in [left, right).
-    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
-    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
-    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
-    uint32_t addr[WMAX];
-    float dx[WMAX], dy[WMAX], dz[WMAX];
-#pragma unroll
-    for (int w = 0; w < WMAX; ++w) {
-        addr[w] = ((uint32_t)w < cnt) ? baseB : scrB[w];
-        const uint32_t a = addr[w] + w * SLOTB;
-        dx[w] = lds_ld(a);
-        dy[w] = lds_ld(a + COMPB);
-        dz[w] = lds_ld(a + 2 * COMPB);
-    }
-    float ax = dx[0], ay = dy[0], az = dz[0];
-#pragma unroll
-    for (int w = 1; w < WMAX; ++w) {
-        ax += dx[w];
-        ay += dy[w];
-        az += dz[w];
-    }
-    if constexpr (G > 1) {
-        ax = group_sum<G>(ax);
-        ay = group_sum<G>(ay);
-        az = group_sum<G>(az);
-    }
-    // |axis|^2 >= ds^2 > 0 for any sub-chain, far from denormals.
-    const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
-    const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec.c, rec.s, rec.C);
-#pragma unroll
-    for (int w = 0; w < WMAX; ++w) {
-        const float x = dx[w], y = dy[w], z = dz[w];
-        const uint32_t a = addr[w] + w * SLOTB;
-        lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
-        lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
-        lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
-    }
-}
 
 // Exit-direction bin of the unit tangent t: cos(theta) = t.e over nTheta bins
 // of [-1, 1], phi = atan2(t.f2, t.f1) over nPhi bins of [-pi, pi), clamped to
@@ -302,7 +206,7 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
         }
     }
     acc.nClamped = clamped;
-    block_reduce_store(acc, smem, p.partials);
+    block_reduce_lse(acc, smem, p.partials + blockIdx.x);
 }
 
 // FP64 cross-block normaliser: combines the per-block partials in a fixed
@@ -310,78 +214,7 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
 __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts, uint32_t n,
       double absorbLog2PerPath, FinalOut *out)
 {
-    __shared__ double shM[256], shS[256], shS2[256];
-    __shared__ unsigned long long shV[256], shI[256], shC[256];
-    double M = -INFINITY, S = 0.0, S2 = 0.0;
-    unsigned long long nv = 0, ni = 0, nc = 0;
-    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) {
-        const BlockPartial q = parts[b];
-        nv += q.nValid;
-        ni += q.nInvalid;
-        nc += q.nClamped;
-        if (q.m == -INFINITY) continue;
-        const double m = q.m;
-        const double s = (double)q.s_hi + (double)q.s_lo;
-        const double s2 = (double)q.s2_hi + (double)q.s2_lo;
-        if (m > M) {
-            const double f = (M == -INFINITY) ? 0.0 : exp2(M - m);
-            S = S * f + s;
-            S2 = S2 * f * f + s2;
-            M = m;
-        } else {
-            const double f = exp2(m - M);
-            S += s * f;
-            S2 += s2 * f * f;
-        }
-    }
-    shM[threadIdx.x] = M;
-    shS[threadIdx.x] = S;
-    shS2[threadIdx.x] = S2;
-    shV[threadIdx.x] = nv;
-    shI[threadIdx.x] = ni;
-    shC[threadIdx.x] = nc;
-    __syncthreads();
-    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) {
-            const uint32_t o = threadIdx.x + h;
-            const double ma = shM[threadIdx.x], mb = shM[o];
-            if (mb != -INFINITY) {
-                if (ma == -INFINITY) {
-                    shM[threadIdx.x] = mb;
-                    shS[threadIdx.x] = shS[o];
-                    shS2[threadIdx.x] = shS2[o];
-                } else {
-                    const double Mx = fmax(ma, mb);
-                    const double fa = exp2(ma - Mx), fb = exp2(mb - Mx);
-                    shS[threadIdx.x] = shS[threadIdx.x] * fa + shS[o] * fb;
-                    shS2[threadIdx.x] = shS2[threadIdx.x] * fa * fa + shS2[o] * fb * fb;
-                    shM[threadIdx.x] = Mx;
-                }
-            }
-            shV[threadIdx.x] += shV[o];
-            shI[threadIdx.x] += shI[o];
-            shC[threadIdx.x] += shC[o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double l2 = 0.30102999566398119521;  // log10(2)
-        const double m = shM[0];
-        out->maxLog2 = m;
-        out->sumScaled = shS[0];
-        if (m == -INFINITY) {
-            out->log10SumW = -INFINITY;
-            out->log10SumW2 = -INFINITY;
-            out->maxLog10W = -INFINITY;
-        } else {
-            out->log10SumW = (m + log2(shS[0]) + absorbLog2PerPath) * l2;
-            out->log10SumW2 = (2.0 * m + log2(shS2[0]) + 2.0 * absorbLog2PerPath) * l2;
-            out->maxLog10W = (m + absorbLog2PerPath) * l2;
-        }
-        out->nValid = shV[0];
-        out->nInvalid = shI[0];
-        out->nClamped = shC[0];
-    }
+    finalize_partials(parts, nullptr, n, absorbLog2PerPath, out);
 }
 
 // ---------------------------------------------------------------------------

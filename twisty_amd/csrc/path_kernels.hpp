@@ -1,0 +1,264 @@
+// path_kernels.hpp -- the per-perturbation step shared by the path kernels
+// (plain forward and batched).  Included by .hip files only.
+#pragma once
+
+#include "device_common.hpp"
+#include "forward.hpp"
+
+namespace twisty_hip {
+
+// One perturbation applied by lane j of its group: read this lane's segments of
+// the sub-chain [left, right), sum them across the group into the axis
+// p_right - p_left, build the rotation and write the rotated segments back.
+// Window slots past this lane's count are redirected to the lane's zeroed
+// scratch row: they read zeros, add nothing to the axis and write zeros back,
+// so every load and store is unconditional with a compile-time offset.
+// scrB[w] is the byte address that lands on the scratch row after slot w's
+// offset is added.  (Skipping the last slot behind a wave-uniform branch when
+// no path of the wave needs it measured slower: the compiler copies the window
+// registers around the branch.)
+template<int G, int BLOCK>
+__device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
+      const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
+{
+    constexpr int WMAX = (25 + G - 1) / G;       // sub-chain slots per lane (diff <= 25)
+    constexpr uint32_t SLOTB = 12u * BLOCK;      // bytes per slot row
+    constexpr uint32_t COMPB = 4u * BLOCK;       // bytes between components
+    const uint32_t diff = rec.packed & 0xFFFFu;
+    const uint32_t left = rec.packed >> 16;
+    const uint32_t right = left + diff;
+    // First segment index >= left owned by this lane, and how many of this
+    // lane's segments lie in [left, right).
+    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
+    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
+    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
+    uint32_t addr[WMAX];
+    float dx[WMAX], dy[WMAX], dz[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        addr[w] = ((uint32_t)w < cnt) ? baseB : scrB[w];
+        const uint32_t a = addr[w] + w * SLOTB;
+        dx[w] = lds_ld(a);
+        dy[w] = lds_ld(a + COMPB);
+        dz[w] = lds_ld(a + 2 * COMPB);
+    }
+    float ax = dx[0], ay = dy[0], az = dz[0];
+#pragma unroll
+    for (int w = 1; w < WMAX; ++w) {
+        ax += dx[w];
+        ay += dy[w];
+        az += dz[w];
+    }
+    if constexpr (G > 1) {
+        ax = group_sum<G>(ax);
+        ay = group_sum<G>(ay);
+        az = group_sum<G>(az);
+    }
+    // The axis is the chord of >= 2 segments; it vanishes only for a sub-chain
+    // that closes exactly, and then the path turns NaN and is counted invalid.
+    const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+    const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec.c, rec.s, rec.C);
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        const float x = dx[w], y = dy[w], z = dz[w];
+        const uint32_t a = addr[w] + w * SLOTB;
+        lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
+        lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
+        lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
+    }
+}
+
+
+// K perturbations of one path, applied in order by its group of G lanes.
+// For G > 1 lane j draws perturbation kb + j of each batch of G and the
+// records are exchanged through the group's slice of drawbuf (4 words per lane).
+template<int G, int BLOCK>
+__device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint32_t j, uint32_t groupBase,
+      const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t *drawbuf, uint64_t seed, uint32_t stream, uint64_t path,
+      uint32_t K, uint32_t N, uint32_t maxDiff)
+{
+    for (uint32_t kb = 0; kb < K; kb += G) {
+        const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
+        if constexpr (G == 1) {
+            perturb_one<G, BLOCK>(laneB, j, scrB, pack_draw(make_draw(seed, stream, path, kb, N, maxDiff)));
+        } else {
+            if (kb + j < K) {
+                const DrawRec r = pack_draw(make_draw(seed, stream, path, kb + j, N, maxDiff));
+                *reinterpret_cast<uint4 *>(drawbuf + lane * 4) =
+                      make_uint4(r.packed, __float_as_uint(r.c), __float_as_uint(r.s), __float_as_uint(r.C));
+            }
+            wave_sync_lds();
+            if constexpr (G <= 8) {
+                DrawRec recs[G];
+#pragma unroll
+                for (int kk = 0; kk < G; ++kk) recs[kk] = load_rec(drawbuf + (groupBase + kk) * 4);
+                wave_sync_lds();
+#pragma unroll
+                for (int kk = 0; kk < G; ++kk) {
+                    if ((uint32_t)kk >= kcount) break;
+                    perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
+                }
+            } else {
+                for (uint32_t kk = 0; kk < kcount; ++kk)
+                    perturb_one<G, BLOCK>(laneB, j, scrB, load_rec(drawbuf + (groupBase + kk) * 4));
+                wave_sync_lds();
+            }
+        }
+    }
+}
+
+struct PathLog {
+    FF lsum;         // sum of log2 segment weights (valid joints)
+    uint32_t flags;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
+};
+
+// Weighting of the state in LDS (G > 1): tangents (CurvePerturbUtils.cpp:147-154),
+// curvatures (:219-243), lookup and log (PathWeighters.cpp:41-96); lane j takes
+// joints i = j mod G; every lane of the group returns the group's totals.
+template<int G, int BLOCK>
+__device__ __forceinline__ PathLog weigh_state(const float *smem, uint32_t lane, uint32_t j, uint32_t groupBase,
+      uint32_t N, float invDs, const TableView &tv, uint32_t &clamped)
+{
+    static_assert(G > 1, "weigh_state handles groups of two or more lanes");
+    constexpr uint32_t SLOT = 3u * BLOCK;
+    FF lsum { 0.0f, 0.0f };
+    uint32_t flags = 0;
+    for (uint32_t i = j; i + 1 < N; i += G) {
+        const uint32_t ai = (i / G) * SLOT + lane;
+        float ta0, ta1, ta2, tb0, tb1, tb2;
+        unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
+        const uint32_t i1 = i + 1;
+        const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
+        unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
+        const float e0 = (tb0 - ta0) * invDs;
+        const float e1 = (tb1 - ta1) * invDs;
+        const float e2 = (tb2 - ta2) * invDs;
+        const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+        const float l = log2f(lookup_weight(tv, kappa, clamped));
+        if (isnan(l)) flags |= 1u;
+        else if (l == -INFINITY) flags |= 2u;
+        else if (l == INFINITY) flags |= 4u;
+        else lsum = ff_add_f(lsum, l);
+    }
+    return { group_sum_ff<G>(lsum), group_or<G>(flags) };
+}
+
+// FP64 merge of per-block partials parts[idx[k]] (idx NULL: parts[k]),
+// k < n, by one block of 256 threads; fixed order, so deterministic.
+__device__ __forceinline__ void finalize_partials(const BlockPartial *parts, const uint32_t *idx, uint32_t n,
+      double absorbLog2PerPath, FinalOut *out)
+{
+    __shared__ double shM[256], shS[256], shS2[256];
+    __shared__ unsigned long long shV[256], shI[256], shC[256];
+    double M = -INFINITY, S = 0.0, S2 = 0.0;
+    unsigned long long nv = 0, ni = 0, nc = 0;
+    for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) {
+        const BlockPartial q = parts[idx ? idx[b] : b];
+        nv += q.nValid;
+        ni += q.nInvalid;
+        nc += q.nClamped;
+        if (q.m == -INFINITY) continue;
+        const double m = q.m;
+        const double s = (double)q.s_hi + (double)q.s_lo;
+        const double s2 = (double)q.s2_hi + (double)q.s2_lo;
+        if (m > M) {
+            const double f = (M == -INFINITY) ? 0.0 : exp2(M - m);
+            S = S * f + s;
+            S2 = S2 * f * f + s2;
+            M = m;
+        } else {
+            const double f = exp2(m - M);
+            S += s * f;
+            S2 += s2 * f * f;
+        }
+    }
+    shM[threadIdx.x] = M;
+    shS[threadIdx.x] = S;
+    shS2[threadIdx.x] = S2;
+    shV[threadIdx.x] = nv;
+    shI[threadIdx.x] = ni;
+    shC[threadIdx.x] = nc;
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const uint32_t o = threadIdx.x + h;
+            const double ma = shM[threadIdx.x], mb = shM[o];
+            if (mb != -INFINITY) {
+                if (ma == -INFINITY) {
+                    shM[threadIdx.x] = mb;
+                    shS[threadIdx.x] = shS[o];
+                    shS2[threadIdx.x] = shS2[o];
+                } else {
+                    const double Mx = fmax(ma, mb);
+                    const double fa = exp2(ma - Mx), fb = exp2(mb - Mx);
+                    shS[threadIdx.x] = shS[threadIdx.x] * fa + shS[o] * fb;
+                    shS2[threadIdx.x] = shS2[threadIdx.x] * fa * fa + shS2[o] * fb * fb;
+                    shM[threadIdx.x] = Mx;
+                }
+            }
+            shV[threadIdx.x] += shV[o];
+            shI[threadIdx.x] += shI[o];
+            shC[threadIdx.x] += shC[o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double l2 = 0.30102999566398119521;  // log10(2)
+        const double m = shM[0];
+        out->maxLog2 = m;
+        out->sumScaled = shS[0];
+        if (m == -INFINITY) {
+            out->log10SumW = -INFINITY;
+            out->log10SumW2 = -INFINITY;
+            out->maxLog10W = -INFINITY;
+        } else {
+            out->log10SumW = (m + log2(shS[0]) + absorbLog2PerPath) * l2;
+            out->log10SumW2 = (2.0 * m + log2(shS2[0]) + 2.0 * absorbLog2PerPath) * l2;
+            out->maxLog10W = (m + absorbLog2PerPath) * l2;
+        }
+        out->nValid = shV[0];
+        out->nInvalid = shI[0];
+        out->nClamped = shC[0];
+    }
+}
+
+// Block-wide merge of the per-lane accumulators into *out; scratch holds one
+// LseAcc per wave.  Ends with every thread past a barrier only after thread 0
+// has read the scratch (the caller's next barrier).
+__device__ __forceinline__ void block_reduce_lse(LseAcc acc, float *scratch, BlockPartial *out)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        LseAcc b;
+        b.m = __shfl_xor(acc.m, o, 64);
+        b.s.hi = __shfl_xor(acc.s.hi, o, 64);
+        b.s.lo = __shfl_xor(acc.s.lo, o, 64);
+        b.s2.hi = __shfl_xor(acc.s2.hi, o, 64);
+        b.s2.lo = __shfl_xor(acc.s2.lo, o, 64);
+        b.nValid = (uint32_t)__shfl_xor((int)acc.nValid, o, 64);
+        b.nInvalid = (uint32_t)__shfl_xor((int)acc.nInvalid, o, 64);
+        b.nClamped = (uint32_t)__shfl_xor((int)acc.nClamped, o, 64);
+        lse_merge(acc, b);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+    LseAcc *sh = reinterpret_cast<LseAcc *>(scratch);
+    if ((threadIdx.x & 63) == 0) sh[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        LseAcc a = sh[0];
+        for (int w = 1; w < nWaves; ++w) lse_merge(a, sh[w]);
+        BlockPartial p;
+        p.m = a.m;
+        p.s_hi = a.s.hi;
+        p.s_lo = a.s.lo;
+        p.s2_hi = a.s2.hi;
+        p.s2_lo = a.s2.lo;
+        p.nValid = a.nValid;
+        p.nInvalid = a.nInvalid;
+        p.nClamped = a.nClamped;
+        *out = p;
+    }
+}
+
+}  // namespace twisty_hip

@@ -41,6 +41,18 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
  * them; (numSegments+1)*3 floats.  Returns 1 when no arc exists. */
 int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut);
 
+/* CachedMultiArclengthWeightLookupTable (PathWeightUtils.cpp:123-278):
+ * numDsSteps tables for ds = minDs + i (maxDs - minDs)/(numDsSteps - 1), read
+ * from <cacheRoot>/CachedWeightTables/<hash>.cwt when present, otherwise
+ * computed and written there (cacheRoot NULL or "" = current directory).
+ * tablesOut: numDsSteps*numCurvatureSteps floats; metaOut: 6 floats per table
+ * as twisty_host_build_rt_table; *loadedFromCache = 1 when read from the file.
+ * errorOut receives the cache file path on success, the message on failure. */
+int twisty_host_cached_rt_tables(const char *cacheRoot, float mu, float eps, float scatter, float absorption,
+      float bias, uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound, float maxBound, float minDs,
+      float maxDs, uint32_t numDsSteps, int threads, float *tablesOut, float *metaOut, int *loadedFromCache,
+      char *errorOut, uint32_t errorLen);
+
 /* d T[i] / d eps of the table twisty_host_build_rt_table builds, in FP64, for
  * the config's eps slot (the Gaussian phase width the reference actually uses,
  * PathWeightUtils.cpp:401-420 with its (mu, eps) swap).  maxBoundFollowsEps != 0

@@ -147,3 +147,37 @@ def test_reference_config_schema_parses():
         host.run_config(text)
     except RuntimeError as e:
         assert "twisty_hip_create" in str(e) or capi.device_count() > 0
+
+
+def test_cached_multi_arclength_tables_round_trip(tmp_path, oracle):
+    """CachedMultiArclengthWeightLookupTable: compute + write, then reload from
+    the .cwt file (the reference's Tests/WeightTableTest.cpp checks the reload
+    to 1e-4; here it is bit-exact), each table equal to a directly built one."""
+    nseg, lo, hi, n = 66, 10.0, 20.0, 4
+    min_ds, max_ds = float(np.float32(lo) / np.float32(nseg)), float(np.float32(hi) / np.float32(nseg))
+    kw = dict(mu=0.1, eps=0.075, scatter=0.1, absorption=0.04, numStepsInt=2000, numCurvatureSteps=1000,
+              cache_root=tmp_path)
+    computed, loaded1, path = host.cached_rt_tables(min_ds, max_ds, n, **kw)
+    assert not loaded1 and os.path.exists(path) and path.endswith(".cwt")
+    # file = n records of (table floats, min weight, max weight)
+    assert os.path.getsize(path) == n * (1000 + 2) * 4
+    reloaded, loaded2, path2 = host.cached_rt_tables(min_ds, max_ds, n, **kw)
+    assert loaded2 and path2 == path
+    step = np.float32(np.float32(max_ds) - np.float32(min_ds)) / np.float32(n - 1)
+    for i, (a, b) in enumerate(zip(computed, reloaded)):
+        assert np.array_equal(a.table, b.table) and a.minW == b.minW and a.maxW == b.maxW
+        ds = float(np.float32(min_ds) + np.float32(i) * step)
+        assert a.ds == ds == b.ds
+        d = host.build_rt_table(ds, 0.1, 0.075, 0.1, 0.04, numStepsInt=2000, numCurvatureSteps=1000)
+        assert np.array_equal(a.table, d.table)
+    csvs = list((tmp_path / "CachedWeightTables").rglob("*_RT.csv"))
+    assert len(csvs) == n
+
+
+def test_cached_tables_reject_bad_grid(tmp_path):
+    with pytest.raises(ValueError):
+        host.cached_rt_tables(0.2, 0.1, 4, 1.0, 0.01, 0.1, 0.004, numStepsInt=200, numCurvatureSteps=64,
+                              cache_root=tmp_path)
+    with pytest.raises(ValueError):
+        host.cached_rt_tables(0.1, 0.2, 1, 1.0, 0.01, 0.1, 0.004, numStepsInt=200, numCurvatureSteps=64,
+                              cache_root=tmp_path)

@@ -60,6 +60,42 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
     return 0;
 }
 
+int twisty_host_cached_rt_tables(const char *cacheRoot, float mu, float eps, float scatter, float absorption,
+      float bias, uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound, float maxBound, float minDs,
+      float maxDs, uint32_t numDsSteps, int threads, float *tablesOut, float *metaOut, int *loadedFromCache,
+      char *errorOut, uint32_t errorLen)
+{
+    if (!tablesOut || !metaOut || numCurvatureSteps < 2 || numStepsInt < 4 || numStepsInt % 2 != 0) return 1;
+    WeightingParameters wp;
+    wp.mu = mu;
+    wp.eps = eps;
+    wp.scatter = scatter;
+    wp.absorption = absorption;
+    wp.bias = bias;
+    wp.numStepsInt = numStepsInt;
+    wp.numCurvatureSteps = numCurvatureSteps;
+    wp.minBound = minBound;
+    wp.maxBound = maxBound;
+    try {
+        PathWeighting::CachedMultiArclengthWeightLookupTable c(wp, minDs, maxDs, numDsSteps,
+              cacheRoot ? std::string(cacheRoot) : std::string(), threads);
+        for (size_t i = 0; i < c.Count(); ++i) {
+            const auto &t = c.Table(i);
+            const auto &v = t.AccessLookupTable();
+            std::memcpy(tablesOut + i * numCurvatureSteps, v.data(), v.size() * sizeof(float));
+            const float meta[6] = { t.GetDs(), t.GetMinCurvature(), t.GetMaxCurvature(), t.GetCurvatureStepSize(),
+                t.GetMinSegmentWeight(), t.GetMaxSegmentWeight() };
+            std::memcpy(metaOut + 6 * i, meta, sizeof(meta));
+        }
+        if (loadedFromCache) *loadedFromCache = c.LoadedFromCache() ? 1 : 0;
+        put_error(errorOut, errorLen, c.CachePath());
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int twisty_host_rt_table_eps_derivative(float mu, float eps, float scatter, float bias, uint32_t numStepsInt,
       uint32_t numCurvatureSteps, float minBound, float maxBound, int maxBoundFollowsEps, float ds, int threads,
       double *dTableDEps)

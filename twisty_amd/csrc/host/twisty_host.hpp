@@ -137,6 +137,38 @@ class WeightLookupTableIntegral : public BaseWeightLookupTable {
           int threads);
 };
 
+// CachedMultiArclengthWeightLookupTable (PathWeightUtils.h:168-193,
+// .cpp:123-278): numDsSteps radiative-transfer tables for ds on the float grid
+// minDs + i (maxDs - minDs)/(numDsSteps - 1), loaded from
+// <cacheRoot>/CachedWeightTables/<hash>.cwt when that file exists, otherwise
+// computed, written there (the tables' StreamOut records back to back) and
+// exported as CSV under CachedWeightTables/<hash>/.  The hash is FNV-1a of the
+// reference's UUID string (the reference uses std::hash).  cacheRoot "" is the
+// current directory, as the reference.
+class CachedMultiArclengthWeightLookupTable {
+   public:
+    CachedMultiArclengthWeightLookupTable(const WeightingParameters &wp, float minDs, float maxDs, uint32_t numDsSteps,
+          const std::string &cacheRoot = "", int threads = 0);
+    // The table whose ds is closest to ds (first on ties).
+    const BaseWeightLookupTable *GetWeightLookupTable(float ds) const;
+    size_t ClosestIndex(float ds) const;
+    std::pair<std::string, uint64_t> GetUUID() const { return m_uuid; }
+    bool LoadedFromCache() const { return m_loaded; }
+    const std::string &CachePath() const { return m_cachePath; }
+    size_t Count() const { return m_tables.size(); }
+    const BaseWeightLookupTable &Table(size_t i) const { return *m_tables[i]; }
+
+   private:
+    float DsAt(uint32_t i) const;
+    WeightingParameters m_wp;
+    float m_minDs, m_maxDs;
+    uint32_t m_numDsSteps;
+    std::pair<std::string, uint64_t> m_uuid;
+    std::string m_cachePath;
+    bool m_loaded = false;
+    std::vector<std::unique_ptr<BaseWeightLookupTable>> m_tables;
+};
+
 // SimpleWeightLookupTable (PathWeightUtils.h:210-218, .cpp:436-485).
 class SimpleWeightLookupTable : public BaseWeightLookupTable {
    public:

@@ -3,7 +3,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <limits>
+#include <memory>
 #include <iomanip>
 #include <sstream>
 #include <stdexcept>
@@ -305,6 +308,75 @@ void WeightLookupTableIntegral::Initialize(int threads)
     m_lookupTable = std::move(raw);
     m_minSegmentWeight = mn;
     m_maxSegmentWeight = mx;
+}
+
+CachedMultiArclengthWeightLookupTable::CachedMultiArclengthWeightLookupTable(const WeightingParameters &wp,
+      float minDs, float maxDs, uint32_t numDsSteps, const std::string &cacheRoot, int threads)
+    : m_wp(wp)
+    , m_minDs(minDs)
+    , m_maxDs(maxDs)
+    , m_numDsSteps(numDsSteps)
+{
+    if (numDsSteps < 2 || !(maxDs > minDs) || !(minDs > 0.0f))
+        throw std::runtime_error("cached weight tables need 0 < minDs < maxDs and numDsSteps >= 2");
+    std::ostringstream u;
+    u << "minDs_" << hex_of(minDs) << "maxDs_" << hex_of(maxDs) << "numDsSteps_" << numDsSteps << "wp_"
+      << wp.GenerateStringUUID().first;
+    m_uuid = { u.str(), fnv1a(u.str()) };
+    const std::string root = cacheRoot.empty() ? std::string(".") : cacheRoot;
+    const std::string dir = root + "/CachedWeightTables";
+    m_cachePath = dir + "/" + std::to_string(m_uuid.second) + ".cwt";
+    auto make = [&](uint32_t i) -> std::unique_ptr<BaseWeightLookupTable> {
+        if (wp.weightingMethod == WeightingMethod::RadiativeTransfer)
+            return std::make_unique<WeightLookupTableIntegral>(wp, DsAt(i));
+        return std::make_unique<SimpleWeightLookupTable>(wp, DsAt(i));
+    };
+    std::ifstream in(m_cachePath, std::ios::binary);
+    if (in.is_open()) {
+        for (uint32_t i = 0; i < numDsSteps; ++i) {
+            m_tables.push_back(make(i));
+            m_tables.back()->InitializeFromStream(in);
+        }
+        m_loaded = true;
+        return;
+    }
+    for (uint32_t i = 0; i < numDsSteps; ++i) {
+        m_tables.push_back(make(i));
+        if (auto *rt = dynamic_cast<WeightLookupTableIntegral *>(m_tables.back().get())) rt->Initialize(threads);
+        else m_tables.back()->Initialize();
+    }
+    std::error_code ec;
+    std::filesystem::create_directories(dir + "/" + std::to_string(m_uuid.second), ec);
+    std::ofstream out(m_cachePath, std::ios::binary);
+    if (!out.is_open()) throw std::runtime_error("Failed to export weight table: " + m_cachePath);
+    for (const auto &t : m_tables) t->StreamOut(out);
+    out.close();
+    for (const auto &t : m_tables) t->ExportValues(dir + "/" + std::to_string(m_uuid.second));
+}
+
+float CachedMultiArclengthWeightLookupTable::DsAt(uint32_t i) const
+{
+    const float step = (m_maxDs - m_minDs) / (float)(m_numDsSteps - 1);
+    return m_minDs + (float)i * step;
+}
+
+size_t CachedMultiArclengthWeightLookupTable::ClosestIndex(float ds) const
+{
+    size_t best = 0;
+    float bestDist = std::numeric_limits<float>::max();
+    for (size_t i = 0; i < m_tables.size(); ++i) {
+        const float d = std::abs(m_tables[i]->GetDs() - ds);
+        if (d < bestDist) {
+            bestDist = d;
+            best = i;
+        }
+    }
+    return best;
+}
+
+const BaseWeightLookupTable *CachedMultiArclengthWeightLookupTable::GetWeightLookupTable(float ds) const
+{
+    return m_tables.empty() ? nullptr : m_tables[ClosestIndex(ds)].get();
 }
 
 // PathWeightUtils.cpp:436-485.

@@ -11,7 +11,7 @@ from . import capi
 LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
-            "twisty_host_initial_arc_exit_jacobian"]
+            "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables"]
 
 _lib = None
 
@@ -49,6 +49,8 @@ def lib():
         L.twisty_host_rt_table_eps_derivative.argtypes = [f32, f32, f32, f32, u32, u32, f32, f32, C.c_int, f32,
                                                           C.c_int, vp]
         L.twisty_host_initial_arc_exit_jacobian.argtypes = [vp, u32, vp, vp]
+        L.twisty_host_cached_rt_tables.argtypes = [C.c_char_p, f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, f32,
+                                                   u32, C.c_int, vp, vp, C.POINTER(C.c_int), C.c_char_p, u32]
         _lib = L
     return _lib
 
@@ -78,6 +80,24 @@ def build_rt_table(ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000
     if rc != 0:
         raise ValueError("twisty_host_build_rt_table rejected its arguments")
     return WeightTable(t, meta)
+
+
+def cached_rt_tables(min_ds, max_ds, num_ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000,
+                     numCurvatureSteps=4096, minBound=0.0, maxBound=None, cache_root=None, threads=0):
+    """CachedMultiArclengthWeightLookupTable: (list of WeightTable, loaded_from_cache, cache path)."""
+    if maxBound is None:
+        maxBound = float(np.float32(10.0) / np.float32(eps))
+    t = np.zeros((num_ds, numCurvatureSteps), np.float32)
+    meta = np.zeros((num_ds, 6), np.float32)
+    loaded = C.c_int()
+    msg = C.create_string_buffer(4096)
+    root = None if cache_root is None else str(cache_root).encode()
+    rc = lib().twisty_host_cached_rt_tables(root, mu, eps, scatter, absorption, bias, numStepsInt, numCurvatureSteps,
+                                            minBound, maxBound, min_ds, max_ds, num_ds, threads, _p(t), _p(meta),
+                                            C.byref(loaded), msg, 4096)
+    if rc != 0:
+        raise ValueError(f"twisty_host_cached_rt_tables: {msg.value.decode()}")
+    return [WeightTable(t[i], meta[i]) for i in range(num_ds)], bool(loaded.value), msg.value.decode()
 
 
 def rt_table_eps_derivative(ds, mu, eps, scatter, bias=1.0, numStepsInt=20000, numCurvatureSteps=4096,

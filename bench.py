@@ -109,6 +109,8 @@ def measure_adjoint(ctx, w, reps=3):
 
 def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     """The batched workload in one graph-replayed launch vs one plain run per experiment."""
+    import numpy as np
+
     from twisty_amd import workloads
 
     t0 = time.perf_counter()
@@ -127,7 +129,8 @@ def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     t0 = time.perf_counter()
     for x in exps:
         m = wl["meta"][x["table"]]
-        ctx.set_tables(wl["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]), float(m[3]), float(m[4]))
+        ctx.set_tables(wl["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]),
+                       float(np.float32(x["arclength"]) / np.float32(x["N"])), float(m[4]))
         ctx.set_curve(x["positions"], x["boundary"])
         ctx.run(x["paths"], x["K"], seed=x["seed"])
     plain_s = time.perf_counter() - t0

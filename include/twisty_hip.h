@@ -202,6 +202,9 @@ int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *ex
       const float *tables, uint32_t numBins, uint32_t numTables, const float *tableMeta, const float *positions,
       uint64_t numPoints, twisty_hip_batch **out);
 
+/* Frees the batch.  Destroying its context first releases the batch's device
+ * memory; the batch then only accepts twisty_hip_batch_destroy (other calls
+ * return TWISTY_HIP_ERR_STATE). */
 int twisty_hip_batch_destroy(twisty_hip_batch *batch);
 
 int twisty_hip_batch_set_schedule(twisty_hip_batch *batch, const twisty_hip_batch_schedule *schedule);

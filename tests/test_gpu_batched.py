@@ -33,8 +33,10 @@ def batch_lanes(N, spl):
 
 
 def plain_run(ctx, x, tables, meta):
+    """A plain run of one experiment: its table's grid, its own ds (absorption)."""
     m = meta[x["table"]]
-    ctx.set_tables(tables[x["table"]], float(m[0]), float(m[1]), float(m[2]), float(m[3]), float(m[4]))
+    ds = float(np.float32(x["arclength"]) / np.float32(x["N"]))
+    ctx.set_tables(tables[x["table"]], float(m[0]), float(m[1]), float(m[2]), ds, float(m[4]))
     ctx.set_curve(x["positions"], x["boundary"])
     return ctx.run(x["paths"], x["K"], seed=x["seed"], per_path=True)
 

@@ -6,6 +6,7 @@ the library is missing the import of `lib()` raises: there is no CPU fallback.
 import ctypes as C
 import math
 import os
+import weakref
 
 import numpy as np
 
@@ -132,6 +133,8 @@ class Context:
             raise TwistyHipError(rc, f"twisty_hip_create(device={device}) failed")
 
     def close(self):
+        for b in list(getattr(self, "_batches", [])):
+            b.close()
         if self._h:
             lib().twisty_hip_destroy(self._h)
             self._h = C.c_void_p()
@@ -254,6 +257,9 @@ class Batch:
         m = np.ascontiguousarray(meta, np.float32)
         self.n = n
         self._h = C.c_void_p()
+        if not hasattr(ctx, "_batches"):
+            ctx._batches = weakref.WeakSet()
+        ctx._batches.add(self)
         ctx._check(lib().twisty_hip_batch_create(ctx._h, C.cast(arr, C.c_void_p), n, _ptr(t), t.shape[1], t.shape[0],
                                                  _ptr(m), _ptr(self.positions), self.positions.shape[0],
                                                  C.byref(self._h)))

@@ -211,6 +211,34 @@ int replay(twisty_hip_batch *b, float *ms)
 
 }  // namespace
 
+namespace twisty_hip {
+
+void batch_release(twisty_hip_batch *b)
+{
+    if (!b || !b->ctx) return;
+    twisty_hip_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drop_graph(b);
+    for (void *p : { (void *)b->dTableData, (void *)b->dSegs, (void *)b->dTables, (void *)b->dExps,
+               (void *)b->dItems, (void *)b->dItemPartials, (void *)b->dQueue, (void *)b->dExpItemStart,
+               (void *)b->dExpItems, (void *)b->dAbsorb, (void *)b->dOuts, (void *)b->dPerPath })
+        (void)hipFree(p);
+    b->dTableData = b->dSegs = b->dPerPath = nullptr;
+    b->dTables = nullptr;
+    b->dExps = nullptr;
+    b->dItems = nullptr;
+    b->dItemPartials = nullptr;
+    b->dQueue = b->dExpItemStart = b->dExpItems = nullptr;
+    b->dAbsorb = nullptr;
+    b->dOuts = nullptr;
+    auto &v = ctx->batches;
+    v.erase(std::remove(v.begin(), v.end(), b), v.end());
+    b->ctx = nullptr;
+}
+
+}  // namespace twisty_hip
+
 extern "C" {
 
 int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *experiments, uint32_t numExperiments,
@@ -225,6 +253,7 @@ int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *ex
         return fail(ctx, TWISTY_HIP_ERR_ARG, "batch_create: need experiments, tables (2..65536 bins), meta, positions");
     auto *b = new twisty_hip_batch();
     b->ctx = ctx;
+    ctx->batches.push_back(b);
     b->numBins = numBins;
     b->exps.assign(experiments, experiments + numExperiments);
     auto bail = [&](int code, const std::string &m) {
@@ -315,23 +344,7 @@ int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *ex
 int twisty_hip_batch_destroy(twisty_hip_batch *b)
 {
     if (!b) return TWISTY_HIP_ERR_ARG;
-    if (b->ctx) {
-        (void)hipSetDevice(b->ctx->device);
-        (void)hipStreamSynchronize(b->ctx->stream);
-    }
-    drop_graph(b);
-    (void)hipFree(b->dTableData);
-    (void)hipFree(b->dSegs);
-    (void)hipFree(b->dTables);
-    (void)hipFree(b->dExps);
-    (void)hipFree(b->dItems);
-    (void)hipFree(b->dItemPartials);
-    (void)hipFree(b->dQueue);
-    (void)hipFree(b->dExpItemStart);
-    (void)hipFree(b->dExpItems);
-    (void)hipFree(b->dAbsorb);
-    (void)hipFree(b->dOuts);
-    (void)hipFree(b->dPerPath);
+    batch_release(b);
     delete b;
     return TWISTY_HIP_OK;
 }
@@ -339,6 +352,7 @@ int twisty_hip_batch_destroy(twisty_hip_batch *b)
 int twisty_hip_batch_set_schedule(twisty_hip_batch *b, const twisty_hip_batch_schedule *s)
 {
     if (!b || !s) return TWISTY_HIP_ERR_ARG;
+    if (!b->ctx) return TWISTY_HIP_ERR_STATE;  // its context was destroyed
     const twisty_hip_batch_schedule old = b->sched;
     b->sched = *s;
     if (b->sched.block_size == 0) b->sched.block_size = kDefaultSchedule.block_size;
@@ -357,6 +371,7 @@ int twisty_hip_batch_set_schedule(twisty_hip_batch *b, const twisty_hip_batch_sc
 int twisty_hip_batch_get_schedule(twisty_hip_batch *b, twisty_hip_batch_schedule *s)
 {
     if (!b || !s) return TWISTY_HIP_ERR_ARG;
+    if (!b->ctx) return TWISTY_HIP_ERR_STATE;  // its context was destroyed
     *s = b->sched;
     if (s->blocks_per_cu == 0 && b->ctx->numCUs) s->blocks_per_cu = b->grid / b->ctx->numCUs;
     return TWISTY_HIP_OK;
@@ -365,6 +380,7 @@ int twisty_hip_batch_get_schedule(twisty_hip_batch *b, twisty_hip_batch_schedule
 int twisty_hip_batch_run(twisty_hip_batch *b, twisty_hip_partials *out, float *perPathLog10, float *deviceMs)
 {
     if (!b) return TWISTY_HIP_ERR_ARG;
+    if (!b->ctx) return TWISTY_HIP_ERR_STATE;  // its context was destroyed
     twisty_hip_ctx *ctx = b->ctx;
     if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "batch_run: out is NULL");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -400,6 +416,7 @@ int twisty_hip_batch_autotune(twisty_hip_batch *b, uint32_t reps, twisty_hip_bat
       twisty_hip_batch_schedule *tried, float *triedMs, uint32_t cap, uint32_t *numTried)
 {
     if (!b) return TWISTY_HIP_ERR_ARG;
+    if (!b->ctx) return TWISTY_HIP_ERR_STATE;  // its context was destroyed
     twisty_hip_ctx *ctx = b->ctx;
     if (reps == 0) reps = 1;
     // Candidates: block size x lanes policy x blocks per CU x tile size.  A

@@ -296,6 +296,7 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     // Teardown is best effort: errors here have no caller to act on them.
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    while (!ctx->batches.empty()) batch_release(ctx->batches.back());
     (void)hipFree(ctx->dTable);
     (void)hipFree(ctx->dInitSeg);
     (void)hipFree(ctx->dPartials);

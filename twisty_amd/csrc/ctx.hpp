@@ -11,6 +11,16 @@
 
 using namespace twisty_hip;
 
+#include <vector>
+
+struct twisty_hip_batch;
+
+namespace twisty_hip {
+// Frees a batch's device resources and detaches it from its context; the
+// batch object itself stays until twisty_hip_batch_destroy (batch_capi.hip).
+void batch_release(twisty_hip_batch *b);
+}  // namespace twisty_hip
+
 struct twisty_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -68,6 +78,8 @@ struct twisty_hip_ctx {
     uint32_t schedG = 0, schedBlock = 0, schedBpc = 0;
     uint32_t adjG = 0, adjBlock = 0, adjBpc = 0;
     float lastFwdMs = 0.0f, lastTotalMs = 0.0f, lastAdjMs = 0.0f;
+
+    std::vector<twisty_hip_batch *> batches;  // live batches, released with the context
 };
 
 namespace twisty_hip {

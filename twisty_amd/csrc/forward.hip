@@ -91,37 +91,8 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                 }
             }
 
-            for (uint32_t kb = 0; kb < K; kb += G) {
-                const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
-                if constexpr (G == 1) {
-                    perturb_one<G, BLOCK>(laneB, j, scrB, pack_draw(make_draw(p.seed, p.stream, path, kb, N, p.maxDiff)));
-                } else {
-                    // Each lane draws one perturbation of the batch into the
-                    // group's slice of drawbuf.
-                    if (kb + j < K) {
-                        const DrawRec r = pack_draw(make_draw(p.seed, p.stream, path, kb + j, N, p.maxDiff));
-                        *reinterpret_cast<uint4 *>(drawbuf + lane * 4) =
-                              make_uint4(r.packed, __float_as_uint(r.c), __float_as_uint(r.s), __float_as_uint(r.C));
-                    }
-                    wave_sync_lds();
-                    if constexpr (G <= 8) {
-                        // Whole batch into registers, then apply it.
-                        DrawRec recs[G];
-#pragma unroll
-                        for (int kk = 0; kk < G; ++kk) recs[kk] = load_rec(drawbuf + (groupBase + kk) * 4);
-                        wave_sync_lds();  // read before drawbuf is rewritten
-#pragma unroll
-                        for (int kk = 0; kk < G; ++kk) {
-                            if ((uint32_t)kk >= kcount) break;
-                            perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
-                        }
-                    } else {
-                        for (uint32_t kk = 0; kk < kcount; ++kk)
-                            perturb_one<G, BLOCK>(laneB, j, scrB, load_rec(drawbuf + (groupBase + kk) * 4));
-                        wave_sync_lds();
-                    }
-                }
-            }
+            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream, path, K, N,
+                  p.maxDiff);
 
             // Weighting: tangents (CurvePerturbUtils.cpp:147-154), curvature
             // (CurvePerturbUtils.cpp:219-243), lookup and log

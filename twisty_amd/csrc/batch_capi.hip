@@ -47,7 +47,7 @@ struct twisty_hip_batch {
 
 namespace {
 
-const twisty_hip_batch_schedule kDefaultSchedule { 512, 0, 16, 4096 };
+const twisty_hip_batch_schedule kDefaultSchedule { 256, 0, 16, 4096 };
 
 uint32_t lanes_for(uint32_t N, uint32_t spl)
 {

@@ -5,6 +5,10 @@
         --calib-fetch DB --calib-write DB --calib-bytes BYTES --match SUBSTR [--json OUT]
 
 `kernels` prints per-kernel dispatch count and mean/min/max duration.
+`counters` prints the mean per dispatch of every counter collected for the
+matching kernel, plus VALU and LDS utilisation derived from them
+(SQ_INSTS_VALU wave instructions x 4 cycles / (GRBM_GUI_ACTIVE per XCD x
+SIMDs), SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS).
 `traffic` prints FETCH_SIZE and WRITE_SIZE per dispatch of the matching kernel,
 scaled into bytes by the counters a plain copy of known size produced
 (copy kernel from twisty_copy_calib), and optionally writes the per-launch HBM
@@ -74,6 +78,27 @@ def traffic(fetch_db, write_db, calib_fetch_db, calib_write_db, calib_bytes, mat
     return text, res
 
 
+def counters(db, match, simds=1024, xcds=8):
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    by = {}
+    for k, c, v in rows:
+        if match in k:
+            by.setdefault(c, []).append(v)
+    mean = {c: statistics.fmean(v) for c, v in by.items()}
+    lines = ["| counter | mean per dispatch |", "|---|---|"]
+    lines += [f"| {c} | {v:.4g} |" for c, v in sorted(mean.items())]
+    derived = {}
+    if "SQ_INSTS_VALU" in mean and "GRBM_GUI_ACTIVE" in mean:
+        cycles = mean["GRBM_GUI_ACTIVE"] / xcds  # GRBM_GUI_ACTIVE sums over the XCDs
+        derived["kernel_cycles"] = cycles
+        derived["valu_issue_utilisation"] = mean["SQ_INSTS_VALU"] * 4.0 / (cycles * simds)
+    if "SQ_LDS_BANK_CONFLICT" in mean and "SQ_INSTS_LDS" in mean:
+        derived["lds_bank_conflicts_per_lds_instruction"] = mean["SQ_LDS_BANK_CONFLICT"] / mean["SQ_INSTS_LDS"]
+    lines += [f"| derived: {k} | {v:.4g} |" for k, v in derived.items()]
+    return "\n".join(lines), {**mean, **derived}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -87,9 +112,15 @@ def main(argv=None):
     t.add_argument("--calib-bytes", type=int, required=True)
     t.add_argument("--match", required=True)
     t.add_argument("--json")
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("--match", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         text, _ = kernels(a.db, a.skip_first, a.match)
+        print(text)
+    elif a.cmd == "counters":
+        text, _ = counters(a.db, a.match)
         print(text)
     else:
         text, res = traffic(a.fetch, a.write, a.calib_fetch, a.calib_write, a.calib_bytes, a.match)

@@ -73,10 +73,12 @@ def cpu_baseline(table, pos, w, target_seconds):
                               precision=32, threads=th, want_w=False)
         return r["seconds"]
 
+    # Sample: paths 0..n-1 of the same workload, n a power of two sized for
+    # about target_seconds of wall time (at most 4x the workload's paths).
     n0 = 1 << 12
     s0 = run(n0, threads)
     n = n0
-    while n < w["paths"] and s0 * (n / n0) * 2 <= target_seconds * 1.5:
+    while n < 4 * w["paths"] and s0 * (n / n0) * 2 <= target_seconds * 1.5:
         n *= 2
     secs = run(n, threads) if n != n0 else s0
     jw = n * (w["N"] - 1) / secs
@@ -143,6 +145,22 @@ def measure_batched(ctx, capi, n_exp=4096, autotune=True):
                         "the sphere, paths log-uniform [2^10,2^16], K=100, 8 shared tables sigma ~ U[0.03,0.2]"}
 
 
+def measure_batched_c2(ctx, capi, w, table, pos, bc):
+    """The C2 workload (one experiment, 2^22 paths) through the batched kernel."""
+    ds = float(table.ds)
+    exp = [{"N": w["N"], "paths": w["paths"], "K": w["K"], "seed": w["seed"], "table": 0, "arclength": w["L"],
+            "positions": pos}]
+    meta = [[table.minK, table.maxK, table.step, ds, w["absorption"]]]
+    b = capi.Batch(ctx, exp, [table.table], meta)
+    best, _ = b.autotune(reps=1)
+    b.run()
+    ms = min(b.run()[2] for _ in range(3))
+    res = b.run()[0][0]
+    b.close()
+    return {"ms": ms, "joint_weights_per_s": w["paths"] * (w["N"] - 1) / (ms / 1e3), "schedule": best,
+            "log10_sum_w": res["log10_sum_w"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,7 +168,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--paths", type=int, default=WORKLOAD["paths"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     a = ap.parse_args(argv)
@@ -216,6 +234,7 @@ def main(argv=None):
                                        "table_only: dE/dT (-> sigma)"}
     if not a.no_batched:
         out["batched"] = measure_batched(ctx, capi)
+        out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
     if not a.no_cpu_baseline:
         cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -40,7 +40,8 @@ namespace twisty_hip {
 // One perturbation undone by lane j of its group: rewinds this lane's
 // segments of the sub-chain and their adjoints (same window and scratch-row
 // redirection as the forward's perturb_one; the adjoint rows sit lamB bytes
-// after the segment rows).
+// after the segment rows).  Every slot of a lane past its count lands on the
+// same scratch address; they load the same values and store the same values.
 template<int G, int BLOCK>
 __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
@@ -125,14 +126,12 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
 #pragma unroll
     for (int w = 0; w < WMAX; ++w) {
         const uint32_t a = addr[w] + w * SLOTB;
-        float qx = fmaf(R.m00, lx[w], fmaf(R.m10, ly[w], R.m20 * lz[w]));
-        float qy = fmaf(R.m01, lx[w], fmaf(R.m11, ly[w], R.m21 * lz[w]));
-        float qz = fmaf(R.m02, lx[w], fmaf(R.m12, ly[w], R.m22 * lz[w]));
-        // Redirected slots read zeros and must write zeros back.
-        const bool in = (uint32_t)w < cnt;
-        qx = in ? qx + dax : qx;
-        qy = in ? qy + day : qy;
-        qz = in ? qz + daz : qz;
+        // Redirected slots add dE/da to the lane's adjoint scratch row too.  That
+        // row only ever meets segment scratch values of zero (R^T 0 = 0), so it
+        // adds nothing to dE/dR; it is cleared at every path start.
+        const float qx = fmaf(R.m00, lx[w], fmaf(R.m10, ly[w], fmaf(R.m20, lz[w], dax)));
+        const float qy = fmaf(R.m01, lx[w], fmaf(R.m11, ly[w], fmaf(R.m21, lz[w], day)));
+        const float qz = fmaf(R.m02, lx[w], fmaf(R.m12, ly[w], fmaf(R.m22, lz[w], daz)));
         lds_st(a, px[w]);
         lds_st(a + COMPB, py[w]);
         lds_st(a + 2 * COMPB, pz[w]);
@@ -227,6 +226,10 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
                 smem[s * SLOT + BLOCK + lane] = src[p.numPaths];
                 smem[s * SLOT + 2 * BLOCK + lane] = src[2 * p.numPaths];
             }
+        }
+        if constexpr (SWEEP) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) smem[lamOff + scr + c * BLOCK] = 0.0f;
         }
         wave_sync_lds();
 

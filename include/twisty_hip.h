@@ -119,6 +119,17 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
+/* Weight-magnitude histogram of the following runs: every valid path counted
+ * in one of numBins linear bins of its weight 10^(log10 w + pathNormalizerLog10)
+ * on [minWeight, maxWeight], below -> bin 0, above -> last bin (the histogram
+ * FullExperimentRunnerOptimalPerturb.cpp:26-37, 235-246, 394-422, 562-575
+ * keeps with 500 bins on [4.69653e-16, 1.17517e-13]).  0 bins switches it off. */
+int twisty_hip_set_weight_histogram(twisty_hip_ctx *ctx, uint32_t numBins, double minWeight, double maxWeight,
+      double pathNormalizerLog10);
+
+/* Path counts of the last run's weight histogram (numBins uint64). */
+int twisty_hip_get_weight_histogram(twisty_hip_ctx *ctx, uint64_t *counts);
+
 /* Exit-direction histogram of the following runs: nTheta x nPhi bins of the
  * last free tangent t_{N-2} in (cos theta, phi) about the end direction of the
  * boundary conditions (end frame: f1 = the coordinate axis least aligned with

@@ -152,3 +152,40 @@ def test_exit_histogram_c5_three_tables(hip_device, oracle):
     assert err < 5e-2
     assert abs(h.sum() - 1.0) < 1e-5
     ctx.close()
+
+
+def _weight_bins(log10w, lo, hi, nb):
+    """The reference's rule (FullExperimentRunnerOptimalPerturb.cpp:562-575) in FP64."""
+    w = 10.0 ** log10w
+    b = np.floor((w - lo) / (hi - lo) * nb)
+    b = np.where(w < lo, 0, np.where(w > hi, nb - 1, b))
+    return np.minimum(b, nb - 1).astype(np.int64)
+
+
+def test_weight_histogram(hip_device, oracle):
+    """Path counts per linear weight bin: against the same rule applied to the
+    kernel's own per-path weights (paths within 3e-5 of a bin edge in relative
+    weight may move one bin: the float log10 output carries ~1e-5), and the
+    reference's default range, which every path of this workload falls below."""
+    case = cases.MID
+    paths = 1 << 14
+    ctx = cases.hip_context(case, oracle)
+    ref = ctx.run(paths, case.K, seed=case.seed, per_path=True)
+    g = ref["log10w"].astype(np.float64)
+    ok = np.isfinite(g)
+    w = 10.0 ** g[ok]
+    lo, hi, nb = np.quantile(w, 0.05), np.quantile(w, 0.95), 64
+    ctx.set_weight_histogram(nb, lo, hi, 0.0)
+    r = ctx.run(paths, case.K, seed=case.seed)
+    counts = ctx.weight_histogram().astype(np.int64)
+    assert counts.sum() == r["num_valid"] == ok.sum()
+    mine = np.bincount(_weight_bins(g[ok], lo, hi, nb), minlength=nb)
+    u = (w - lo) / (hi - lo) * nb
+    near = np.abs(u - np.round(u)) < 3e-5 * np.maximum(u, 1.0)
+    print(f"[weight histogram] |diff| {np.abs(counts - mine).sum()} near-edge paths {near.sum()}", flush=True)
+    assert np.abs(counts - mine).sum() <= 2 * near.sum()
+    ctx.set_weight_histogram(500)  # the reference's range: weights ~1e-40 all land in bin 0
+    r = ctx.run(paths, case.K, seed=case.seed)
+    counts = ctx.weight_histogram()
+    assert counts[0] == r["num_valid"] and counts[1:].sum() == 0
+    ctx.close()

@@ -71,3 +71,9 @@ def test_full_experiment_driver(hip_device, tmp_path):
     assert len(results) == 1 and len(tables) == 1
     assert "Avg path weight" in results[0].read_text()
     assert len(tables[0].read_text().splitlines()) == 4096
+    hist = list(tmp_path.rglob("histogram.txt"))
+    assert len(hist) == 1
+    lines = hist[0].read_text().splitlines()
+    assert lines[0] == "Histogram" and len(lines) == 501
+    counts = [float(l.split()[2]) for l in lines[1:]]
+    assert sum(counts) == cfg["experiment"]["experimentParams"]["pathsToGenerate"]

@@ -20,6 +20,7 @@ EXPORTED = [
     "twisty_hip_version", "twisty_hip_device_count", "twisty_hip_create", "twisty_hip_destroy",
     "twisty_hip_last_error", "twisty_hip_set_tables", "twisty_hip_set_curve", "twisty_hip_run",
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
+    "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -100,6 +101,8 @@ def _declare(L):
     L.twisty_hip_set_schedule.argtypes = [vp, u32, u32, u32]
     L.twisty_hip_get_schedule.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
+    L.twisty_hip_set_weight_histogram.argtypes = [vp, u32, C.c_double, C.c_double, C.c_double]
+    L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
     L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
     L.twisty_hip_get_exit_histogram.argtypes = [vp, vp, vp, u64]
     L.twisty_hip_run_adjoint.argtypes = [vp, u64, u64, u32, u64, C.POINTER(Partials), vp, vp]
@@ -195,6 +198,16 @@ class Context:
         a, b = C.c_float(), C.c_float()
         self._check(lib().twisty_hip_last_timing(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_weight_histogram(self, num_bins=500, min_weight=4.69653e-16, max_weight=1.17517e-13, norm_log10=0.0):
+        """Weight-magnitude histogram (defaults: the reference's 500 bins and range)."""
+        self._check(lib().twisty_hip_set_weight_histogram(self._h, num_bins, min_weight, max_weight, norm_log10))
+        self.weight_bins = num_bins
+
+    def weight_histogram(self):
+        h = np.empty(self.weight_bins, np.uint64)
+        self._check(lib().twisty_hip_get_weight_histogram(self._h, _ptr(h)))
+        return h
 
     def set_exit_histogram(self, n_theta, n_phi):
         self._check(lib().twisty_hip_set_exit_histogram(self._h, n_theta, n_phi))

@@ -20,6 +20,9 @@ using namespace twisty_hip;
 
 namespace {
 
+// LDS words the forward kernel reserves for the weight-magnitude histogram.
+uint32_t hist_stride(const twisty_hip_ctx *ctx) { return ctx->whBins ? ((ctx->whBins + 63u) & ~63u) : 0u; }
+
 // Resolves a schedule for segment count N.  Zero fields are filled in: G from N
 // (about 16 segments per lane), then the largest compiled block size whose LDS
 // fits one block, then as many blocks per CU as the per-CU LDS holds (at most
@@ -40,7 +43,7 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
     const int ns = forward_block_sizes(G, sizes, 32);
     if (block == 0) {
         for (int k = ns - 1; k >= 0; --k) {
-            if (forward_lds_bytes(G, sizes[k], N, ts) <= ctx->ldsPerBlock) {
+            if (forward_lds_bytes(G, sizes[k], N, ts, hist_stride(ctx)) <= ctx->ldsPerBlock) {
                 block = sizes[k];
                 break;
             }
@@ -56,7 +59,7 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
         why = "no kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
         return TWISTY_HIP_ERR_UNSUPPORTED;
     }
-    lds = forward_lds_bytes(G, block, N, ts);
+    lds = forward_lds_bytes(G, block, N, ts, hist_stride(ctx));
     if (lds > ctx->ldsPerBlock) {
         why = "schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
               std::to_string(ctx->ldsPerBlock);
@@ -168,6 +171,17 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     end_frame(ctx->endDir, p.exitE, p.exitF1, p.exitF2);
     p.nTheta = ctx->histTheta;
     p.nPhi = ctx->histPhi;
+    if (ctx->whBins) {
+        if ((rc = ensure_dev(ctx, ctx->dWeightHist, ctx->weightHistCap, ctx->whBins, "weight histogram"))) return rc;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->dWeightHist, 0, ctx->whBins * sizeof(unsigned long long), ctx->stream));
+        p.weightHist = ctx->dWeightHist;
+        p.weightBins = ctx->whBins;
+        p.histStride = hist_stride(ctx);
+        p.weightMin = ctx->whMin;
+        p.weightMax = ctx->whMax;
+        p.weightLog10Offset = absorbLog2PerPath * 0.30102999566398119521 + ctx->whNormLog10;
+    }
+    ctx->lastWeightBins = ctx->whBins;
 
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
@@ -312,6 +326,7 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dGradSPartial);
     (void)hipFree(ctx->dGradT);
     (void)hipFree(ctx->dGradS);
+    (void)hipFree(ctx->dWeightHist);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -481,6 +496,31 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastHistBins = hist ? ctx->histTheta * ctx->histPhi : 0;
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_weight_histogram(twisty_hip_ctx *ctx, uint32_t numBins, double minWeight, double maxWeight,
+      double pathNormalizerLog10)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (numBins > 4096 || (numBins && !(maxWeight > minWeight && std::isfinite(minWeight) && std::isfinite(maxWeight) &&
+                                 std::isfinite(pathNormalizerLog10))))
+        return fail(ctx, TWISTY_HIP_ERR_ARG,
+              "set_weight_histogram: need numBins <= 4096 and finite minWeight < maxWeight (0 bins = off)");
+    ctx->whBins = numBins;
+    ctx->whMin = minWeight;
+    ctx->whMax = maxWeight;
+    ctx->whNormLog10 = pathNormalizerLog10;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_get_weight_histogram(twisty_hip_ctx *ctx, uint64_t *counts)
+{
+    if (!ctx || !counts) return TWISTY_HIP_ERR_ARG;
+    if (ctx->lastWeightBins == 0)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "get_weight_histogram: the last run had no weight histogram");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(counts, ctx->dWeightHist, ctx->lastWeightBins * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return TWISTY_HIP_OK;
 }
 

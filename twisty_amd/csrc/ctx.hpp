@@ -63,6 +63,13 @@ struct twisty_hip_ctx {
     uint64_t lastHistPaths = 0;
     uint32_t lastHistBins = 0;
 
+    // Weight-magnitude histogram.
+    uint32_t whBins = 0;
+    double whMin = 0.0, whMax = 0.0, whNormLog10 = 0.0;
+    unsigned long long *dWeightHist = nullptr;
+    uint64_t weightHistCap = 0;
+    uint32_t lastWeightBins = 0;
+
     // Adjoint.
     float *dFinalSeg = nullptr;
     uint64_t finalSegCap = 0;

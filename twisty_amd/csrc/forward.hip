@@ -39,6 +39,24 @@ __device__ __forceinline__ uint32_t exit_bin(const FwdParams &p, float t0, float
     return (uint32_t)it * p.nPhi + (uint32_t)ip;
 }
 
+// Bin of a valid path in the weight-magnitude histogram of
+// FullExperimentRunnerOptimalPerturb.cpp:562-575: the weight
+// 10^(log10 w + normaliser) in FP64, below the range -> bin 0, above -> the
+// last bin, else floor((w - min) / (max - min) * bins) (clamped: the reference
+// indexes one past the end for w == max).  Zero weight -> bin 0.
+__device__ __forceinline__ uint32_t weight_bin(const FwdParams &p, uint32_t flags, FF lsum)
+{
+    if (flags & 2u) return 0u;
+    if (flags & 4u) return p.weightBins - 1u;
+    const double l10 = ((double)lsum.hi + (double)lsum.lo) * 0.30102999566398119521 + p.weightLog10Offset;
+    const double w = exp10(l10);
+    if (w < p.weightMin) return 0u;
+    if (w > p.weightMax) return p.weightBins - 1u;
+    const double b = (w - p.weightMin) / (p.weightMax - p.weightMin) * (double)p.weightBins;
+    const uint32_t bi = (uint32_t)b;
+    return bi < p.weightBins ? bi : p.weightBins - 1u;
+}
+
 template<int G, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
 {
@@ -48,10 +66,14 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
     // Rows 0..S-1 hold segments; row p.scratchRow (>= S and >= WMAX-1, so that
     // scr - w*SLOT never underflows) is the lane's zero row.
     const uint32_t tabOff = (p.scratchRow + 1) * SLOT;
-    const uint32_t drawOff = tabOff + p.tableStride;
+    const uint32_t histOff = tabOff + p.tableStride;
+    const uint32_t drawOff = histOff + p.histStride;
     uint32_t *drawbuf = reinterpret_cast<uint32_t *>(smem + drawOff);
+    uint32_t *whist = reinterpret_cast<uint32_t *>(smem + histOff);
 
     for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
+    if (p.weightHist)
+        for (uint32_t i = threadIdx.x; i < p.weightBins; i += BLOCK) whist[i] = 0u;
     __syncthreads();
     const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
 
@@ -111,42 +133,50 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
             }
             FF lsum { 0.0f, 0.0f };
             uint32_t flags = 0;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
-            float ta0 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
-            if constexpr (G == 1) {
-                unit_tangent(smem[lane], smem[BLOCK + lane], smem[2 * BLOCK + lane], ta0, ta1, ta2);
-            }
-            for (uint32_t i = j; i + 1 < N; i += G) {
+            if (G > 1 && !p.perJointCurv && !p.pathBin) {
+                // No per-joint outputs: the shared weighting loop.
                 if constexpr (G > 1) {
-                    const uint32_t ai = (i / G) * SLOT + lane;
-                    unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
+                    const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, p.invDs, tv, clamped);
+                    lsum = pl.lsum;
+                    flags = pl.flags;
                 }
-                const uint32_t i1 = i + 1;
-                const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
-                float tb0, tb1, tb2;
-                unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
-                const float e0 = (tb0 - ta0) * p.invDs;
-                const float e1 = (tb1 - ta1) * p.invDs;
-                const float e2 = (tb2 - ta2) * p.invDs;
-                const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
-                if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = kappa;
-                if (p.pathBin && i == N - 2) p.pathBin[local] = exit_bin(p, ta0, ta1, ta2);
-                const float wseg = lookup_weight(tv, kappa, clamped);
-                const float l = log2f(wseg);
-                if (isnan(l)) flags |= 1u;
-                else if (l == -INFINITY) flags |= 2u;
-                else if (l == INFINITY) flags |= 4u;
-                else lsum = ff_add_f(lsum, l);
+            } else {
+                float ta0 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
                 if constexpr (G == 1) {
-                    ta0 = tb0;
-                    ta1 = tb1;
-                    ta2 = tb2;
+                    unit_tangent(smem[lane], smem[BLOCK + lane], smem[2 * BLOCK + lane], ta0, ta1, ta2);
+                }
+                for (uint32_t i = j; i + 1 < N; i += G) {
+                    if constexpr (G > 1) {
+                        const uint32_t ai = (i / G) * SLOT + lane;
+                        unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
+                    }
+                    const uint32_t i1 = i + 1;
+                    const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
+                    float tb0, tb1, tb2;
+                    unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
+                    const float e0 = (tb0 - ta0) * p.invDs;
+                    const float e1 = (tb1 - ta1) * p.invDs;
+                    const float e2 = (tb2 - ta2) * p.invDs;
+                    const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+                    if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = kappa;
+                    if (p.pathBin && i == N - 2) p.pathBin[local] = exit_bin(p, ta0, ta1, ta2);
+                    const float l = log2f(lookup_weight(tv, kappa, clamped));
+                    if (isnan(l)) flags |= 1u;
+                    else if (l == -INFINITY) flags |= 2u;
+                    else if (l == INFINITY) flags |= 4u;
+                    else lsum = ff_add_f(lsum, l);
+                    if constexpr (G == 1) {
+                        ta0 = tb0;
+                        ta1 = tb1;
+                        ta2 = tb2;
+                    }
+                }
+                if constexpr (G > 1) {
+                    lsum = group_sum_ff<G>(lsum);
+                    flags = group_or<G>(flags);
                 }
             }
-            if constexpr (G > 1) {
-                lsum = group_sum_ff<G>(lsum);
-                flags = group_or<G>(flags);
-                wave_sync_lds();  // joints read other lanes' columns before the next reset
-            }
+            if constexpr (G > 1) wave_sync_lds();  // joints read other lanes' columns before the next reset
             if (j == 0) {
                 const bool invalid = (flags & 1u) || ((flags & 6u) == 6u);
                 float outv;
@@ -166,6 +196,8 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                     }
                 }
                 if (p.perPathLog10) p.perPathLog10[local] = outv;
+                if (p.weightHist && !invalid)
+                    atomicAdd(&whist[weight_bin(p, flags, lsum)], 1u);
                 if (p.pathLog2) {
                     float2 v;
                     if (invalid) v = make_float2(__int_as_float(0x7fc00000), 0.0f);
@@ -177,6 +209,11 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
         }
     }
     acc.nClamped = clamped;
+    if (p.weightHist) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < p.weightBins; i += BLOCK)
+            if (whist[i]) atomicAdd(&p.weightHist[i], (unsigned long long)whist[i]);
+    }
     block_reduce_lse(acc, smem, p.partials + blockIdx.x);
 }
 

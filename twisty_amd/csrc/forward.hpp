@@ -36,6 +36,13 @@ struct FwdParams {
     float *finalSeg;       // nullable: final segment i, component c of path local at (3i + c)*numPaths + local
     float exitE[3], exitF1[3], exitF2[3];  // end frame of the exit-direction bins
     uint32_t nTheta, nPhi;
+    // Weight-magnitude histogram (nullable): path counts over weightBins linear
+    // bins of 10^(log10 w + normaliser) on [weightMin, weightMax].
+    unsigned long long *weightHist;
+    uint32_t weightBins;
+    uint32_t histStride;   // LDS words reserved for the block's bins (multiple of 64, 0 when off)
+    double weightMin, weightMax;
+    double weightLog10Offset;  // absorption per path + path normaliser, log10
 };
 
 struct FinalOut {
@@ -62,11 +69,14 @@ inline uint32_t forward_scratch_row(uint32_t G, uint32_t N)
 }
 
 // Dynamic LDS bytes of the forward kernel: segment rows + zero row, table,
-// draw records (G > 1); at least the block reduction's scratch.
-inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride)
+// weight-histogram bins, draw records (G > 1); at least the block reduction's
+// scratch.
+inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride,
+      uint32_t histStride = 0)
 {
     const uint64_t rows = (uint64_t)forward_scratch_row(G, N) + 1;
-    const uint64_t floats = rows * 3ull * block + (uint64_t)tableStride + (G > 1 ? 4ull * block : 0ull);
+    const uint64_t floats =
+          rows * 3ull * block + (uint64_t)tableStride + histStride + (G > 1 ? 4ull * block : 0ull);
     const uint64_t reduceBytes = (uint64_t)(block / 64) * sizeof(LseAcc);
     const uint64_t bytes = floats * 4ull;
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);

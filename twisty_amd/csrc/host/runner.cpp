@@ -1,6 +1,7 @@
 // runner.cpp -- ExperimentRunner and the HIP runner behind it.
 #include <cstdio>
 #include <filesystem>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -195,10 +196,32 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
     // Chain length 0: as many chains as 65536 groups can run side by side.
     uint32_t L = ep.chainLength;
     if (L == 0) L = (uint32_t)std::max<uint64_t>(1, (nPaths + 65535) / 65536);
+    // The reference always keeps its weight-magnitude histogram
+    // (FullExperimentRunnerOptimalPerturb.cpp:26-37: 500 linear bins on
+    // [4.69653e-16, 1.17517e-13]) and writes histogram.txt (:394-422).
+    constexpr uint32_t kHistBins = 500;
+    constexpr double kHistMin = 4.69653e-16, kHistMax = 1.17517e-13;
+    if (ep.writeFiles) {
+        rc = twisty_hip_set_weight_histogram(ctx, kHistBins, kHistMin, kHistMax, normLog10);
+        if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    }
     twisty_hip_partials part {};
     if (nPaths > 0) {
         rc = twisty_hip_run(ctx, nPaths, 0, ep.perturbationsPerPath, L, seed, &part, nullptr, nullptr);
         if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+        if (ep.writeFiles) {
+            std::vector<uint64_t> counts(kHistBins);
+            rc = twisty_hip_get_weight_histogram(ctx, counts.data());
+            if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+            std::ofstream hf(m_experimentDirPath + "/histogram.txt");
+            if (!hf.is_open()) return fail("cannot write histogram.txt in " + m_experimentDirPath);
+            hf << "Histogram" << '\n';
+            for (uint32_t i = 0; i < kHistBins; ++i) {
+                const double binMin = kHistMin + (kHistMax - kHistMin) * i / kHistBins;
+                const double binMax = kHistMin + (kHistMax - kHistMin) * (i + 1) / kHistBins;
+                hf << binMin << " " << binMax << " " << (double)counts[i] << '\n';
+            }
+        }
     }
     const auto runEnd = std::chrono::high_resolution_clock::now();
     float fwdMs = 0.0f, totMs = 0.0f;

@@ -32,8 +32,11 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
 {
     const uint32_t ts = table_stride(ctx->tableSize ? ctx->tableSize : 4096);
     if (G == 0) {
+        // About 16 segments per lane; from N = 512 on, 32 per lane (two paths
+        // per wave measured faster than one at N = 1024).
+        const uint32_t spl = N >= 512 ? 32u : 16u;
         G = 1;
-        while (G < 64 && (N + G - 1) / G > 16) G <<= 1;
+        while (G < 64 && (N + G - 1) / G > spl) G <<= 1;
     }
     if (!is_pow2(G) || G > 64) {
         why = "lanesPerPath must be a power of two <= 64";

@@ -283,7 +283,7 @@ struct BlockPartial {
 // Cross-lane sums inside aligned groups of G lanes.  G <= 16 uses DPP row
 // patterns (xor 1, xor 2 inside a quad, then half-row and row mirrors, after
 // which every lane of the group holds the group total); larger groups finish
-// with ds_swizzle-free bpermute shuffles.
+// with a permlane swap (32) and a shuffle (64).
 // ---------------------------------------------------------------------------
 template<int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
@@ -302,6 +302,22 @@ constexpr int DPP_QUAD_XOR2 = 0x4E;       // quad_perm [2,3,0,1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;
 constexpr int DPP_ROW_MIRROR = 0x140;
 
+// The 32-lane step combines row pairs with gfx950's v_permlane16_swap: called
+// with the same register twice, the first result holds the even row of each
+// pair and the second the odd one, so their sum is the pair total in every
+// lane, in the same order everywhere (every lane of a row already holds the row
+// total).  The 64-lane step uses a bpermute shuffle: v_permlane32_swap measured
+// slower there.
+struct Halves {
+    uint32_t even, odd;
+};
+
+__device__ __forceinline__ Halves row_pair(uint32_t v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return { r[0], r[1] };
+}
+
 template<int G>
 __device__ __forceinline__ float group_sum(float v)
 {
@@ -309,7 +325,10 @@ __device__ __forceinline__ float group_sum(float v)
     if constexpr (G >= 4) v += dpp_f<DPP_QUAD_XOR2>(v);
     if constexpr (G >= 8) v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
     if constexpr (G >= 16) v += dpp_f<DPP_ROW_MIRROR>(v);
-    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 32) {
+        const Halves h = row_pair(__float_as_uint(v));
+        v = __uint_as_float(h.even) + __uint_as_float(h.odd);
+    }
     if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
     return v;
 }
@@ -321,7 +340,10 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v)
     if constexpr (G >= 4) v |= dpp_u<DPP_QUAD_XOR2>(v);
     if constexpr (G >= 8) v |= dpp_u<DPP_ROW_HALF_MIRROR>(v);
     if constexpr (G >= 16) v |= dpp_u<DPP_ROW_MIRROR>(v);
-    if constexpr (G >= 32) v |= (uint32_t)__shfl_xor((int)v, 16, 64);
+    if constexpr (G >= 32) {
+        const Halves h = row_pair(v);
+        v = h.even | h.odd;
+    }
     if constexpr (G >= 64) v |= (uint32_t)__shfl_xor((int)v, 32, 64);
     return v;
 }
@@ -335,7 +357,11 @@ __device__ __forceinline__ FF group_sum_ff(FF v)
     if constexpr (G >= 8)
         v = ff_add(v, FF { dpp_f<DPP_ROW_HALF_MIRROR>(v.hi), dpp_f<DPP_ROW_HALF_MIRROR>(v.lo) });
     if constexpr (G >= 16) v = ff_add(v, FF { dpp_f<DPP_ROW_MIRROR>(v.hi), dpp_f<DPP_ROW_MIRROR>(v.lo) });
-    if constexpr (G >= 32) v = ff_add(v, FF { __shfl_xor(v.hi, 16, 64), __shfl_xor(v.lo, 16, 64) });
+    if constexpr (G >= 32) {
+        const Halves h = row_pair(__float_as_uint(v.hi)), l = row_pair(__float_as_uint(v.lo));
+        v = ff_add(FF { __uint_as_float(h.even), __uint_as_float(l.even) },
+              FF { __uint_as_float(h.odd), __uint_as_float(l.odd) });
+    }
     if constexpr (G >= 64) v = ff_add(v, FF { __shfl_xor(v.hi, 32, 64), __shfl_xor(v.lo, 32, 64) });
     return v;
 }

@@ -233,7 +233,8 @@ __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts
 // ---------------------------------------------------------------------------
 #define TWISTY_FWD_INSTANCES(X)                                                                                   \
     X(1, 128) X(1, 192) X(2, 192) X(2, 256) X(2, 320) X(4, 256) X(4, 384) X(4, 512) X(8, 512)                   \
-    X(8, 768) X(8, 1024) X(16, 512) X(16, 1024) X(32, 512) X(32, 1024) X(64, 256) X(64, 512) X(64, 640)          \
+    X(8, 768) X(8, 1024) X(16, 512) X(16, 1024) X(32, 256) X(32, 384) X(32, 512) X(32, 1024) X(64, 256)          \
+    X(64, 512) X(64, 640)                                                                                         \
     X(64, 704) X(64, 1024)
 
 template<int G, int B>

@@ -119,6 +119,17 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
+/* Keep every path's final state in device memory on the following runs
+ * (12*N bytes per path), for twisty_hip_get_final_positions. */
+int twisty_hip_set_keep_final_state(twisty_hip_ctx *ctx, int enable);
+
+/* Final points of the last run's paths, (numSegments+1) x,y,z floats per path,
+ * path after path (the first point of the initial curve plus the running sum of
+ * the final segments, accumulated in FP64).  Feeds the path-batch export of
+ * FullExperimentRunnerOptimalPerturb.cpp:599-858 (.pbd).  numPaths must equal
+ * the last run's path count; twisty_hip_run_adjoint keeps the final state too. */
+int twisty_hip_get_final_positions(twisty_hip_ctx *ctx, float *positions, uint64_t numPaths);
+
 /* Weight-magnitude histogram of the following runs: every valid path counted
  * in one of numBins linear bins of its weight 10^(log10 w + pathNormalizerLog10)
  * on [minWeight, maxWeight], below -> bin 0, above -> last bin (the histogram

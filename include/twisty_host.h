@@ -75,6 +75,19 @@ float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegmen
 /* log10 of NormalizerStuff::Norm (PathWeightUtils.cpp:721-752). */
 double twisty_host_norm_log10(uint32_t numSegments, float ds, const float boundary[13]);
 
+/* Path-batch export of numPaths paths ((numSegments+1)*3 floats each, and their
+ * log10 weights, NaN = invalid and skipped) under <experimentDir>/GeneratedCurves:
+ * BoundaryConditions.bcf, <prepend>Paths_Metadata.pmd, <prepend>Paths_Binary.pbd,
+ * <prepend>Log10PathWeights_Binary.bdt (the weight 10^(log10 w + normLog10), a
+ * double per path), <prepend>Log10PathWeights_Text.pwt,
+ * <prepend>FiveSegment_Binary.bdt and index.json, in batches of up to 1e6 paths.
+ * The writer of ExperimentRunner.cpp:210-262,303-396 and
+ * FullExperimentRunnerOptimalPerturb.cpp:599-858; the runner uses it when
+ * outputPathBatches is set. */
+int twisty_host_write_path_batches(const char *experimentDir, const char *prepend, const char *experimentName,
+      const float boundary[13], uint32_t numSegments, const float *positions, const float *log10w, uint64_t numPaths,
+      double normLog10, char *errorOut, uint32_t errorLen);
+
 /* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
  * config given as text: parse experiment.experimentParams and
  * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.

@@ -14,6 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from tests.lerp import joint_log10w
 from twisty_amd import capi, host
 
 pytestmark = pytest.mark.gpu
@@ -77,3 +78,54 @@ def test_full_experiment_driver(hip_device, tmp_path):
     assert lines[0] == "Histogram" and len(lines) == 501
     counts = [float(l.split()[2]) for l in lines[1:]]
     assert sum(counts) == cfg["experiment"]["experimentParams"]["pathsToGenerate"]
+
+
+def _histogram_counts(root):
+    hist = list(root.rglob("histogram.txt"))
+    assert len(hist) == 1
+    return [float(l.split()[2]) for l in hist[0].read_text().splitlines()[1:]]
+
+
+def test_runner_path_batch_export(hip_device, tmp_path):
+    """outputPathBatches: the chunked run (two chunks of whole chains) gives the
+    estimate of the plain run; the exported paths keep their endpoints, segment
+    lengths and boundary directions, and their curvatures re-weighted on the
+    host give the exported weights; sum of exported weights = experiment weight."""
+    N, L, K, sigma, paths = 5, 1.2, 20, 0.1, 300000
+    plain_cfg = config(N=N, L=L, paths=paths, K=K, sigma=sigma, write=True, outdir=str(tmp_path / "plain"))
+    exp_cfg = config(N=N, L=L, paths=paths, K=K, sigma=sigma, write=True, outdir=str(tmp_path / "export"))
+    exp_cfg["experiment"]["experimentParams"]["outputPathBatches"] = True
+    a = host.run_config(plain_cfg)
+    b = host.run_config(exp_cfg)
+    print(f"[export] plain {a['log10_experiment_weight']:.12f} chunked {b['log10_experiment_weight']:.12f}", flush=True)
+    assert b["total_paths"] == paths and b["failed_paths"] == a["failed_paths"]
+    assert abs(a["log10_experiment_weight"] - b["log10_experiment_weight"]) <= 1e-9
+    assert _histogram_counts(tmp_path / "plain") == _histogram_counts(tmp_path / "export")
+
+    gen = list((tmp_path / "export").rglob("GeneratedCurves"))
+    assert len(gen) == 1
+    r = host.read_path_batches(gen[0].parent)
+    pos = r["positions"].astype(np.float64)
+    n = paths - b["failed_paths"]
+    assert pos.shape == (n, N + 1, 3) and r["batches"] == [(0, 0, n)] and r["N"] == N
+    assert r["index"]["experiment_name"] == "hostgpu"
+    ds = np.float32(L) / np.float32(N)
+    seg = np.diff(pos, axis=1)
+    assert np.array_equal(r["positions"][:, 0], np.zeros((n, 3), np.float32))
+    assert np.abs(pos[:, -1] - [0, 0, 1]).max() < 1e-4
+    assert np.abs(np.linalg.norm(seg, axis=2) / ds - 1).max() < 1e-4
+    tan = seg / np.linalg.norm(seg, axis=2, keepdims=True)
+    assert np.abs(tan[:, 0] - [0, 0, 1]).max() < 1e-4 and np.abs(tan[:, -1] - [0, 0, 1]).max() < 1e-4
+    # weights from the exported curves (FullExperimentRunnerOptimalPerturb.cpp:711-722)
+    table = host.build_rt_table(float(ds), 1.0, float(np.float32(sigma) ** 2), 0.1, 0.004, numStepsInt=20000,
+                                numCurvatureSteps=4096)
+    curv = np.linalg.norm(np.diff(tan, axis=1), axis=2) * (N / np.float32(L))
+    lw, _ = joint_log10w(curv, table, 0.004)
+    norm = host.norm_log10(N, float(ds), host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L))
+    got = np.log10(r["weights"]) - norm
+    err = np.abs(lw.sum(axis=1) - got) / np.abs(got)
+    print(f"[export] re-weighted curves: median rel {np.median(err):.2e}, 99% {np.quantile(err, 0.99):.2e}", flush=True)
+    assert np.median(err) < 1e-5 and np.quantile(err, 0.99) < 1e-3
+    total = math.log10(np.sum(r["weights"]))
+    assert abs(total - b["log10_experiment_weight"]) < 1e-5
+    assert np.abs(r["five_segment"][:, 0] - np.arccos(tan[:, 1, 2])).max() < 1e-3

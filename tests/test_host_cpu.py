@@ -181,3 +181,67 @@ def test_cached_tables_reject_bad_grid(tmp_path):
     with pytest.raises(ValueError):
         host.cached_rt_tables(0.1, 0.2, 1, 1.0, 0.01, 0.1, 0.004, numStepsInt=200, numCurvatureSteps=64,
                               cache_root=tmp_path)
+
+
+def _five_segment_angles(p):
+    """phi1, theta1, theta2 of FullExperimentRunnerOptimalPerturb.cpp:756-793, in FP64."""
+    unit = lambda v: v / np.linalg.norm(v, axis=-1, keepdims=True)
+    s1 = unit(p[:, 2] - p[:, 1])
+    z = unit(p[:, 4] - p[:, 2])
+    other = np.where((np.abs(z[:, 0]) >= 0.99)[:, None], [0.0, 1.0, 0.0], [1.0, 0.0, 0.0])
+    x = unit(np.cross(z, other))
+    y = np.cross(z, x)
+    v = unit(p[:, 3] - p[:, 2])
+    return np.stack([np.arccos(s1[:, 2]), np.arctan2(s1[:, 1], s1[:, 0]),
+                     np.arctan2((v * y).sum(1), (v * x).sum(1))], axis=1)
+
+
+@pytest.mark.parametrize("N,n", [(5, 25000), (12, 3000)])
+def test_path_batch_export_round_trip(tmp_path, N, n):
+    """Files of the path-batch export read back: layout, skipped invalid paths,
+    linear weights 10^(log10 w + normaliser), every 10000th weight as text, the
+    five-segment angles (N = 5; zeros otherwise)."""
+    rng = np.random.default_rng(N)
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0.3, 0, 1), (0.6, 0, 0.8), 1.7)
+    arc = host.initial_arc(bc, N)
+    pos = (arc[None] + 0.05 * rng.standard_normal((n, N + 1, 3))).astype(np.float32)
+    w = (-40 + rng.standard_normal(n)).astype(np.float32)
+    w[rng.random(n) < 0.1] = np.nan
+    norm = 37.25
+    host.write_path_batches(tmp_path, bc, pos, w, norm, prepend="pre_", experiment_name='exp "q"')
+    r = host.read_path_batches(tmp_path, prepend="pre_")
+    ok = ~np.isnan(w)
+    m = int(ok.sum())
+    assert r["N"] == N and r["arclength"] == np.float32(1.7)
+    assert np.array_equal(r["start"], bc[0:3]) and np.array_equal(r["end_dir"], bc[9:12])
+    assert r["batches"] == [(0, 0, m)] and r["index"] == {"experiment_name": 'exp "q"', "path_batch_links": []}
+    assert np.array_equal(r["positions"], pos[ok])
+    want = 10.0 ** (w[ok].astype(np.float64) + norm)
+    assert np.allclose(r["weights"], want, rtol=1e-14, atol=0)
+    assert np.allclose(r["text_weights"], want[::10000], rtol=1e-5)
+    if N == 5:
+        assert np.abs(r["five_segment"] - _five_segment_angles(pos[ok].astype(np.float64))).max() < 1e-4
+    else:
+        assert r["five_segment"].shape == (m, 3) and not r["five_segment"].any()
+
+
+def test_path_batch_export_splits_batches(tmp_path):
+    """More than 1e6 valid paths: a second batch line and contiguous data."""
+    n = 1_000_000 + 4321
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2)
+    pos = np.broadcast_to(host.initial_arc(bc, 4)[None], (n, 5, 3)).copy()
+    pos[:, 2, 0] = np.arange(n, dtype=np.float32) * 1e-7
+    w = np.full(n, -30.0, np.float32)
+    host.write_path_batches(tmp_path, bc, pos, w, 0.0)
+    r = host.read_path_batches(tmp_path)
+    assert r["batches"] == [(0, 0, 1_000_000), (0, 1, 4321)]
+    assert np.array_equal(r["positions"][:, 2, 0], pos[:, 2, 0]) and len(r["weights"]) == n
+    assert len(r["text_weights"]) == 100 + 1
+
+
+def test_path_batch_export_reports_unwritable_dir(tmp_path):
+    blocker = tmp_path / "file"
+    blocker.write_text("x")
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2)
+    with pytest.raises(RuntimeError, match="write_path_batches failed"):
+        host.write_path_batches(blocker, bc, np.zeros((1, 5, 3), np.float32), np.zeros(1, np.float32), 0.0)

@@ -70,7 +70,7 @@ def build(asm=False, verbose=True):
     return out
 
 
-HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "host_capi.cpp"]
+HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "export.cpp", "host_capi.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               f"-I{os.path.join(ROOT, 'include')}"]
 

@@ -21,6 +21,7 @@ EXPORTED = [
     "twisty_hip_last_error", "twisty_hip_set_tables", "twisty_hip_set_curve", "twisty_hip_run",
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
+    "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -102,6 +103,8 @@ def _declare(L):
     L.twisty_hip_get_schedule.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
     L.twisty_hip_set_weight_histogram.argtypes = [vp, u32, C.c_double, C.c_double, C.c_double]
+    L.twisty_hip_set_keep_final_state.argtypes = [vp, C.c_int]
+    L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
     L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
     L.twisty_hip_get_exit_histogram.argtypes = [vp, vp, vp, u64]
@@ -208,6 +211,15 @@ class Context:
         h = np.empty(self.weight_bins, np.uint64)
         self._check(lib().twisty_hip_get_weight_histogram(self._h, _ptr(h)))
         return h
+
+    def set_keep_final_state(self, enable=True):
+        self._check(lib().twisty_hip_set_keep_final_state(self._h, int(enable)))
+
+    def final_positions(self, num_paths):
+        """(num_paths, N+1, 3) float32 final points of the last run."""
+        out = np.empty((num_paths, self.N + 1, 3), np.float32)
+        self._check(lib().twisty_hip_get_final_positions(self._h, _ptr(out), num_paths))
+        return out
 
     def set_exit_histogram(self, n_theta, n_phi):
         self._check(lib().twisty_hip_set_exit_histogram(self._h, n_theta, n_phi))

@@ -419,6 +419,7 @@ int twisty_hip_set_curve(twisty_hip_ctx *ctx, const float *positions, uint32_t n
     ctx->N = N;
     ctx->arclength = arclength;
     for (int a = 0; a < 3; ++a) ctx->endDir[a] = boundary[9 + a];
+    for (int a = 0; a < 3; ++a) ctx->startPoint[a] = positions[a];
     ctx->haveCurve = true;
     return TWISTY_HIP_OK;
 }
@@ -470,6 +471,7 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     r.perJoint = perJointCurvature != nullptr;
     r.pathLog2 = hist;
     r.pathBin = hist;
+    r.finalSeg = ctx->keepFinalState;
     int rc = enqueue_forward(ctx, r, "run");
     if (rc) return rc;
     if (hist) {
@@ -497,8 +499,42 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastAdjMs = 0.0f;
     ctx->lastHistPaths = hist ? numPaths : 0;
     ctx->lastHistBins = hist ? ctx->histTheta * ctx->histPhi : 0;
+    ctx->lastFinalPaths = ctx->keepFinalState ? numPaths : 0;
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_keep_final_state(twisty_hip_ctx *ctx, int enable)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    ctx->keepFinalState = enable != 0;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_get_final_positions(twisty_hip_ctx *ctx, float *positions, uint64_t numPaths)
+{
+    if (!ctx || !positions) return TWISTY_HIP_ERR_ARG;
+    if (ctx->lastFinalPaths == 0)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "get_final_positions: the last run kept no final state");
+    if (numPaths != ctx->lastFinalPaths)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "get_final_positions: numPaths must equal the last run's path count");
+    const uint32_t N = ctx->N;
+    std::vector<float> seg((size_t)3 * N * numPaths);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(seg.data(), ctx->dFinalSeg, seg.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // Points p_0 + sum of the first i segments, accumulated in FP64.
+    for (uint64_t q = 0; q < numPaths; ++q) {
+        double acc[3] = { ctx->startPoint[0], ctx->startPoint[1], ctx->startPoint[2] };
+        float *out = positions + q * 3ull * (N + 1);
+        for (int a = 0; a < 3; ++a) out[a] = ctx->startPoint[a];
+        for (uint32_t i = 0; i < N; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                acc[a] += (double)seg[(uint64_t)(3 * i + a) * numPaths + q];
+                out[3 * (i + 1) + a] = (float)acc[a];
+            }
+        }
+    }
     return TWISTY_HIP_OK;
 }
 
@@ -652,6 +688,7 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastAdjMs, ctx->ev2, ctx->ev3));
     ctx->lastHistPaths = 0;
     ctx->lastHistBins = 0;
+    ctx->lastFinalPaths = numPaths;  // the forward pass above kept the final state
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
     return TWISTY_HIP_OK;

@@ -39,6 +39,9 @@ struct twisty_hip_ctx {
     uint32_t N = 0;
     float arclength = 0;
     float endDir[3] = { 0, 0, 1 };
+    float startPoint[3] = { 0, 0, 0 };  // first point of the initial curve
+    bool keepFinalState = false;         // run() keeps every path's final segments
+    uint64_t lastFinalPaths = 0;         // paths whose final segments dFinalSeg holds
     bool haveCurve = false;
 
     BlockPartial *dPartials = nullptr;

@@ -152,6 +152,23 @@ double twisty_host_norm_log10(uint32_t numSegments, float ds, const float bounda
     return PathWeighting::NormLog10((int)numSegments, ds, from13(boundary));
 }
 
+int twisty_host_write_path_batches(const char *experimentDir, const char *prepend, const char *experimentName,
+      const float boundary[13], uint32_t numSegments, const float *positions, const float *log10w, uint64_t numPaths,
+      double normLog10, char *errorOut, uint32_t errorLen)
+{
+    if (!experimentDir || !boundary || (numPaths > 0 && (!positions || !log10w))) return 1;
+    try {
+        PathBatchWriter w(experimentDir, prepend ? prepend : "", experimentName ? experimentName : "Default_Experiment",
+              from13(boundary), numSegments);
+        w.Add(positions, log10w, numPaths, normLog10);
+        w.Finish();
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int twisty_host_run_config(const char *jsonText, twisty_host_result *out, char *errorOut, uint32_t errorLen)
 {
     if (!jsonText || !out) return 1;

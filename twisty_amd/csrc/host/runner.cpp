@@ -133,6 +133,30 @@ FullExperimentRunnerOptimalPerturb_HIP::FullExperimentRunnerOptimalPerturb_HIP(E
 
 FullExperimentRunnerOptimalPerturb_HIP::~FullExperimentRunnerOptimalPerturb_HIP() = default;
 
+namespace {
+double log10_add(double a, double b)
+{
+    if (a == -INFINITY) return b;
+    if (b == -INFINITY) return a;
+    const double m = std::max(a, b);
+    return m + std::log10(std::pow(10.0, a - m) + std::pow(10.0, b - m));
+}
+
+// Partials of two disjoint path ranges of one experiment.
+twisty_hip_partials merge_partials(const twisty_hip_partials &a, const twisty_hip_partials &b)
+{
+    twisty_hip_partials r;
+    r.log10_sum_w = log10_add(a.log10_sum_w, b.log10_sum_w);
+    r.log10_sum_w2 = log10_add(a.log10_sum_w2, b.log10_sum_w2);
+    r.max_log10_w = std::max(a.max_log10_w, b.max_log10_w);
+    r.num_paths = a.num_paths + b.num_paths;
+    r.num_valid = a.num_valid + b.num_valid;
+    r.num_invalid = a.num_invalid + b.num_invalid;
+    r.num_clamped = a.num_clamped + b.num_clamped;
+    return r;
+}
+}  // namespace
+
 // The body of FullExperimentRunnerOptimalPerturb::RunnerSpecificRunExperiment
 // (FullExperimentRunnerOptimalPerturb.cpp:48-435) with the per-thread chains
 // replaced by one twisty_hip_run: build the weight table (initialised, unlike
@@ -206,13 +230,48 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
         if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
     }
     twisty_hip_partials part {};
-    if (nPaths > 0) {
+    std::vector<uint64_t> counts(kHistBins, 0);
+    if (nPaths > 0 && !ep.outputPathBatches) {
         rc = twisty_hip_run(ctx, nPaths, 0, ep.perturbationsPerPath, L, seed, &part, nullptr, nullptr);
         if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
         if (ep.writeFiles) {
-            std::vector<uint64_t> counts(kHistBins);
             rc = twisty_hip_get_weight_histogram(ctx, counts.data());
             if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+        }
+    } else if (nPaths > 0) {
+        // Path-batch export: chunks of whole chains (pathOffset a multiple of
+        // L, so the draws are those of one run), final points and per-path
+        // weights copied back per chunk and appended to the batch files.
+        try {
+            PathBatchWriter writer(m_experimentDirPath, ep.pathBatchPrepend, ep.experimentName, bc, N);
+            rc = twisty_hip_set_keep_final_state(ctx, 1);
+            if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+            const uint64_t chunk = std::max<uint64_t>(1, (uint64_t(1) << 18) / L) * L;
+            std::vector<float> log10w, points;
+            std::vector<uint64_t> chunkCounts(kHistBins);
+            bool first = true;
+            for (uint64_t off = 0; off < nPaths; off += chunk) {
+                const uint64_t n = std::min(chunk, nPaths - off);
+                log10w.resize(n);
+                points.resize(n * 3ull * (N + 1));
+                twisty_hip_partials c {};
+                rc = twisty_hip_run(ctx, n, off, ep.perturbationsPerPath, L, seed, &c, log10w.data(), nullptr);
+                if (rc == TWISTY_HIP_OK) rc = twisty_hip_get_final_positions(ctx, points.data(), n);
+                if (rc == TWISTY_HIP_OK && ep.writeFiles) rc = twisty_hip_get_weight_histogram(ctx, chunkCounts.data());
+                if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+                if (ep.writeFiles)
+                    for (uint32_t i = 0; i < kHistBins; ++i) counts[i] += chunkCounts[i];
+                writer.Add(points.data(), log10w.data(), n, normLog10);
+                part = first ? c : merge_partials(part, c);
+                first = false;
+            }
+            writer.Finish();
+        } catch (const std::exception &e) {
+            return fail(e.what());
+        }
+    }
+    if (nPaths > 0) {
+        if (ep.writeFiles) {
             std::ofstream hf(m_experimentDirPath + "/histogram.txt");
             if (!hf.is_open()) return fail("cannot write histogram.txt in " + m_experimentDirPath);
             hf << "Histogram" << '\n';

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <array>
+#include <fstream>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -198,6 +199,29 @@ bool InitialArcF64(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegme
 bool InitialArcExitJacobian(const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments,
       std::vector<double> &dTheta, std::vector<double> &dPhi);
 }  // namespace PathGeneration
+
+// ---------------------------------------------------------------------------
+// Path-batch export (outputPathBatches; ExperimentRunner.cpp:210-262,303-396,
+// FullExperimentRunnerOptimalPerturb.cpp:599-858), file formats in export.cpp.
+// ---------------------------------------------------------------------------
+class PathBatchWriter {
+   public:
+    PathBatchWriter(const std::string &experimentDir, const std::string &prepend, const std::string &experimentName,
+          const PerturbUtils::BoundaryConditions &bc, uint32_t numSegments);
+    // numPaths paths: (N+1) float3 points each and their log10 weights (NaN =
+    // invalid, skipped); the exported weight is 10^(log10 w + normLog10).
+    void Add(const float *positions, const float *log10w, uint64_t numPaths, double normLog10);
+    void Finish();
+
+   private:
+    void Flush();
+    uint32_t m_numSegments;
+    std::string m_dir, m_experimentName;
+    std::ofstream m_pmd, m_pbd, m_bdt, m_pwt, m_fiveFile;
+    std::vector<Vec3> m_points, m_fiveCache;
+    std::vector<double> m_weights;
+    uint64_t m_batchIdx = 0;
+};
 
 // ---------------------------------------------------------------------------
 // Results (ExperimentRunner.h:73-82).  experimentWeight is carried as log10.

@@ -11,7 +11,8 @@ from . import capi
 LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
-            "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables"]
+            "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
+            "twisty_host_write_path_batches"]
 
 _lib = None
 
@@ -51,6 +52,8 @@ def lib():
         L.twisty_host_initial_arc_exit_jacobian.argtypes = [vp, u32, vp, vp]
         L.twisty_host_cached_rt_tables.argtypes = [C.c_char_p, f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, f32,
                                                    u32, C.c_int, vp, vp, C.POINTER(C.c_int), C.c_char_p, u32]
+        L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
+                                                     C.c_uint64, f64, C.c_char_p, u32]
         _lib = L
     return _lib
 
@@ -146,6 +149,44 @@ def minimum_arclength(boundary, N):
 def norm_log10(N, ds, boundary):
     b = np.ascontiguousarray(boundary, np.float32)
     return lib().twisty_host_norm_log10(N, ds, _p(b))
+
+
+def write_path_batches(experiment_dir, boundary, positions, log10w, norm_log10, prepend="",
+                       experiment_name="Default_Experiment"):
+    """Path-batch export files of (P, N+1, 3) float32 paths and their log10 weights (NaN = invalid)."""
+    b = np.ascontiguousarray(boundary, np.float32)
+    pos = np.ascontiguousarray(positions, np.float32)
+    w = np.ascontiguousarray(log10w, np.float32)
+    n, P, _ = pos.shape
+    if w.shape != (n,):
+        raise ValueError("one log10 weight per path")
+    err = C.create_string_buffer(1024)
+    if lib().twisty_host_write_path_batches(str(experiment_dir).encode(), prepend.encode(), experiment_name.encode(),
+                                            _p(b), P - 1, _p(pos), _p(w), n, float(norm_log10), err, 1024) != 0:
+        raise RuntimeError(f"write_path_batches failed: {err.value.decode()}")
+
+
+def read_path_batches(experiment_dir, prepend=""):
+    """Reads the files of write_path_batches (or of a runner with outputPathBatches) back."""
+    d = os.path.join(str(experiment_dir), "GeneratedCurves")
+    raw = np.fromfile(os.path.join(d, "BoundaryConditions.bcf"), np.uint8)
+    if raw.size != 56:
+        raise ValueError(f"BoundaryConditions.bcf has {raw.size} bytes, expected 56")
+    vecs = raw[:48].view(np.float32).reshape(4, 3)
+    arclength = float(raw[48:52].view(np.float32)[0])
+    N = int(raw[52:56].view(np.uint32)[0])
+    with open(os.path.join(d, prepend + "Paths_Metadata.pmd")) as f:
+        batches = [tuple(int(t) for t in line.split()) for line in f if line.strip()]
+    with open(os.path.join(d, "index.json")) as f:
+        index = json.load(f)
+    return {
+        "start": vecs[0], "start_dir": vecs[1], "end": vecs[2], "end_dir": vecs[3], "arclength": arclength, "N": N,
+        "batches": batches, "index": index,
+        "positions": np.fromfile(os.path.join(d, prepend + "Paths_Binary.pbd"), np.float32).reshape(-1, N + 1, 3),
+        "weights": np.fromfile(os.path.join(d, prepend + "Log10PathWeights_Binary.bdt"), np.float64),
+        "text_weights": np.loadtxt(os.path.join(d, prepend + "Log10PathWeights_Text.pwt"), ndmin=1),
+        "five_segment": np.fromfile(os.path.join(d, prepend + "FiveSegment_Binary.bdt"), np.float32).reshape(-1, 3),
+    }
 
 
 def run_config(config):

@@ -119,6 +119,15 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
+/* Burn-in of the following twisty_hip_run calls: every chain first applies
+ * pathsPerChain unweighted paths of K perturbations (draws keyed by path id
+ * 2^63 | (chain * pathsPerChain + b), chain = absolute path id / chainLength),
+ * then its chainLength weighted paths.  Replaces numPathsToSkip, which the
+ * reference applies at the start of every thread's chain
+ * (FullExperimentRunnerOptimalPerturb.cpp:286, 469-478, 559-561).
+ * 0 (the default) switches it off; twisty_hip_run_adjoint never burns in. */
+int twisty_hip_set_burn_in(twisty_hip_ctx *ctx, uint32_t pathsPerChain);
+
 /* Keep every path's final state in device memory on the following runs
  * (12*N bytes per path), for twisty_hip_get_final_positions. */
 int twisty_hip_set_keep_final_state(twisty_hip_ctx *ctx, int enable);

@@ -89,10 +89,18 @@ void SimulateRange(const std::vector<V3<double>> &init, const PathSpec &ps, uint
     for (uint32_t i = 0; i <= N; ++i) initT[i] = { (T)init[i].x, (T)init[i].y, (T)init[i].z };
     for (uint64_t li = first; li < last; ++li) {
         const uint64_t path = pathOffset + li;
-        if (li == first || (path % chainLength) == 0) pos = initT;
-        if (li == first && (path % chainLength) != 0) {
-            // Replay the chain from its start so any range split gives the same state.
+        if (li == first || (path % chainLength) == 0) {
             const uint64_t chainStart = path - (path % chainLength);
+            pos = initT;
+            // Burn-in (FullExperimentRunnerOptimalPerturb.cpp:469-478, 559-561):
+            // ps.burnIn unweighted paths open every chain, drawn from path ids
+            // 2^63 | (chain * burnIn + b).
+            const uint64_t burnBase = (chainStart / chainLength) * (uint64_t)ps.burnIn;
+            for (uint32_t b = 0; b < ps.burnIn; ++b)
+                for (uint32_t k = 0; k < ps.K; ++k)
+                    ApplyPerturbation(pos.data(),
+                          draw_perturbation(ps.seed, ps.stream, (1ull << 63) | (burnBase + b), k, N));
+            // Replay the chain from its start so any range split gives the same state.
             for (uint64_t p = chainStart; p < path; ++p)
                 for (uint32_t k = 0; k < ps.K; ++k)
                     ApplyPerturbation(pos.data(), draw_perturbation(ps.seed, ps.stream, p, k, N));
@@ -284,6 +292,7 @@ double oracle_accumulate_combine_log10(const double *w, uint64_t m, uint64_t n)
 //   chainLength: 1 = every path restarts from initialPos; L > 1 = state is
 //     carried through L consecutive path ids (the reference's per-thread chain,
 //     FullExperimentRunnerOptimalPerturb.cpp:469-595).
+//   burnIn: unweighted paths that open every chain (numPathsToSkip).
 // Outputs (all nullable): log10w[nPaths], valid[nPaths], curv[nPaths*(N-1)],
 //   exitDir[nPaths*3] (last free tangent).
 // result: [log10 sum_valid 10^w through the reference accumulator,
@@ -293,7 +302,7 @@ double oracle_accumulate_combine_log10(const double *w, uint64_t m, uint64_t n)
 //   reference's AddValue then sets its offset to +inf and its total to NaN
 //   (CombinedWeightUtils.h:62-69).
 int oracle_simulate(int precision, const double *initialPos, uint32_t N, float arclength, uint32_t K,
-      uint32_t chainLength, uint64_t seed, uint32_t stream, uint64_t pathOffset, uint64_t nPaths,
+      uint32_t chainLength, uint32_t burnIn, uint64_t seed, uint32_t stream, uint64_t pathOffset, uint64_t nPaths,
       const float *table, int32_t tableSize, float ds, float minK, float maxK, float step, float absorption,
       int threads, double *log10wOut, uint8_t *validOut, float *curvOut, float *exitOut, double *result)
 {
@@ -306,6 +315,7 @@ int oracle_simulate(int precision, const double *initialPos, uint32_t N, float a
     ps.K = K;
     ps.seed = seed;
     ps.stream = stream;
+    ps.burnIn = burnIn;
     const SimTable st { table, tableSize, ds, minK, maxK, step, absorption };
     if (threads < 1) threads = 1;
     if ((uint64_t)threads > nPaths) threads = (int)std::max<uint64_t>(1, nPaths);

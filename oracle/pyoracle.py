@@ -68,7 +68,7 @@ def _declare(L):
     L.oracle_accumulate_log10.restype = f64
     L.oracle_accumulate_combine_log10.argtypes = [_p(np.float64), u64, u64]
     L.oracle_accumulate_combine_log10.restype = f64
-    L.oracle_simulate.argtypes = [C.c_int, _p(np.float64), u32, f32, u32, u32, u64, u32, u64, u64,
+    L.oracle_simulate.argtypes = [C.c_int, _p(np.float64), u32, f32, u32, u32, u32, u64, u32, u64, u64,
                                   _p(np.float32), i32, f32, f32, f32, f32, f32, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _p(np.float64)]
     L.oracle_simulate.restype = C.c_int
@@ -151,7 +151,7 @@ def weight_curvatures(curv, table, absorption):
 
 
 def simulate(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, stream=0, path_offset=0,
-             chain_length=1, precision=64, threads=None, want_curv=False, want_exit=False, want_w=True):
+             chain_length=1, precision=64, threads=None, want_curv=False, want_exit=False, want_w=True, burn_in=0):
     threads = threads or os.cpu_count() or 1
     init = np.ascontiguousarray(init_pos, np.float64).reshape(-1)
     res = np.zeros(8, np.float64)
@@ -160,7 +160,7 @@ def simulate(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, stre
     cv = np.zeros(n_paths * (N - 1), np.float32) if want_curv else None
     ex = np.zeros(n_paths * 3, np.float32) if want_exit else None
     ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
-    rc = lib().oracle_simulate(precision, init, N, arclength, K, chain_length, seed, stream, path_offset,
+    rc = lib().oracle_simulate(precision, init, N, arclength, K, chain_length, burn_in, seed, stream, path_offset,
                                n_paths, table.table, len(table.table), table.ds, table.minK, table.maxK,
                                table.step, absorption, threads, ptr(w), ptr(v), ptr(cv), ptr(ex), res)
     if rc != 0:

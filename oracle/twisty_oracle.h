@@ -738,6 +738,7 @@ struct PathSpec {
     uint32_t K = 1;
     uint64_t seed = 1;
     uint32_t stream = 0;
+    uint32_t burnIn = 0;  // unweighted paths at the start of every chain
 };
 
 template<class T>

@@ -69,8 +69,9 @@ def hip_context(case, oracle_mod, device=0, schedule=None):
     return ctx
 
 
-def oracle_run(case, oracle_mod, n_paths, path_offset=0, chain_length=1, want_curv=False, precision=64, K=None):
+def oracle_run(case, oracle_mod, n_paths, path_offset=0, chain_length=1, want_curv=False, precision=64, K=None,
+               burn_in=0):
     table, pos32 = setup(case, oracle_mod)
     return oracle_mod.simulate(pos32.astype(np.float64), case.N, case.L, case.K if K is None else K, n_paths, table,
                                case.absorption, seed=case.seed, path_offset=path_offset, chain_length=chain_length,
-                               precision=precision, want_curv=want_curv)
+                               precision=precision, want_curv=want_curv, burn_in=burn_in)

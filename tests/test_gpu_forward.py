@@ -161,6 +161,23 @@ def test_chain_mode(hip_device, oracle):
     check_against_oracle(case, oracle, gpu, ref, ref32, table, "C2 chain K=1 L=64")
 
 
+@pytest.mark.parametrize("K,L,burn", [(1, 32, 100), (3, 1, 7)])
+def test_burn_in(hip_device, oracle, K, L, burn):
+    """pathsToSkip: burn-in paths open every chain (path offset a multiple of L,
+    so the chain index of the draws is global)."""
+    case = cases.C2
+    table, _ = cases.setup(case, oracle)
+    n, off = 2048, 1000 * L
+    with cases.hip_context(case, oracle) as ctx:
+        ctx.set_burn_in(burn)
+        gpu = ctx.run(n, K, seed=case.seed, chain_length=L, path_offset=off, per_path=True, per_joint=True)
+        ctx.set_burn_in(0)
+        plain = ctx.run(n, K, seed=case.seed, chain_length=L, path_offset=off, per_path=True)
+    ref, ref32 = both_refs(case, oracle, n, chain_length=L, K=K, path_offset=off, burn_in=burn)
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, f"C2 burn-in {burn} K={K} L={L}")
+    assert not np.array_equal(gpu["log10w"], plain["log10w"])
+
+
 def test_invalid_and_zero_weights(hip_device, oracle):
     """NaN table entries make paths invalid; zero entries give zero weight."""
     case = cases.MID

@@ -22,10 +22,11 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def config(N=20, L=1.2, paths=1 << 16, K=50, sigma=0.1, chain=1, seed=1, bins=4096, write=False, outdir="/tmp"):
+def config(N=20, L=1.2, paths=1 << 16, K=50, sigma=0.1, chain=1, seed=1, bins=4096, write=False, outdir="/tmp",
+           skip=0):
     return {"experiment": {
         "experimentParams": {
-            "pathsToGenerate": paths, "pathsToSkip": 0, "numSegments": N, "name": "hostgpu",
+            "pathsToGenerate": paths, "pathsToSkip": skip, "numSegments": N, "name": "hostgpu",
             "experimentDir": outdir, "maxPerturbThreads": 0, "maxWeightThreads": 0,
             "outputBigFloatWeights": False, "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
             "random": {"bootstrapSeed": 1, "perturbSeed": seed},
@@ -36,21 +37,22 @@ def config(N=20, L=1.2, paths=1 << 16, K=50, sigma=0.1, chain=1, seed=1, bins=40
                             "endDir": [0, 0, 1], "arclength": L}}}
 
 
-def oracle_same_inputs(oracle, N, L, K, paths, sigma, chain=1, seed=1, bins=4096):
+def oracle_same_inputs(oracle, N, L, K, paths, sigma, chain=1, seed=1, bins=4096, skip=0):
     ds = float(np.float32(L) / np.float32(N))
     eps = float(np.float32(sigma) ** 2)
     t = oracle.build_table(ds, mu=1.0, eps=eps, scatter=0.1, absorption=0.004, numStepsInt=20000,
                            numCurvatureSteps=bins)
     bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
     pos = host.initial_arc(bc, N).astype(np.float64)
-    return oracle.simulate(pos, N, L, K, paths, t, 0.004, seed=seed, chain_length=chain)
+    return oracle.simulate(pos, N, L, K, paths, t, 0.004, seed=seed, chain_length=chain, burn_in=skip)
 
 
-@pytest.mark.parametrize("N,L,K,sigma,chain", [(20, 1.2, 50, 0.1, 1), (64, 1.5, 100, 0.05, 1), (64, 1.5, 1, 0.05, 256)])
-def test_runner_matches_oracle(hip_device, oracle, N, L, K, sigma, chain):
+@pytest.mark.parametrize("N,L,K,sigma,chain,skip", [(20, 1.2, 50, 0.1, 1, 0), (64, 1.5, 100, 0.05, 1, 0),
+                                                    (64, 1.5, 1, 0.05, 256, 0), (64, 1.5, 1, 0.05, 256, 100)])
+def test_runner_matches_oracle(hip_device, oracle, N, L, K, sigma, chain, skip):
     paths = 1 << 15
-    r = host.run_config(config(N=N, L=L, paths=paths, K=K, sigma=sigma, chain=chain))
-    ref = oracle_same_inputs(oracle, N, L, K, paths, sigma, chain=chain)
+    r = host.run_config(config(N=N, L=L, paths=paths, K=K, sigma=sigma, chain=chain, skip=skip))
+    ref = oracle_same_inputs(oracle, N, L, K, paths, sigma, chain=chain, skip=skip)
     ref_avg = ref["log10_sum_w_exact"] - math.log10(paths)
     print(f"[runner N={N} K={K} L={chain}] gpu log10 avg={r['log10_avg_path_weight']:.9f} "
           f"oracle={ref_avg:.9f} kernel_ms={r['kernel_ms']:.3f} setup_ms={r['setup_ms']}", flush=True)

@@ -149,6 +149,22 @@ def test_reference_config_schema_parses():
         assert "twisty_hip_create" in str(e) or capi.device_count() > 0
 
 
+def test_sample_config_fields_parse():
+    """The reference's sample configs (configs/Samples/RingBenchmark.json) give
+    "scatterValues": [0.1] instead of "scatter" and pathsToSkip 100: both parse."""
+    cfg = _config(pathsToSkip=100)
+    w = cfg["experiment"]["experimentParams"]["weighting"]
+    del w["scatter"]
+    w["scatterValues"] = [0.1]
+    try:
+        host.run_config(cfg)
+    except RuntimeError as e:
+        assert "twisty_hip_create" in str(e) or capi.device_count() > 0
+    del w["scatterValues"]
+    with pytest.raises(RuntimeError, match="scatter"):
+        host.run_config(cfg)
+
+
 def test_cached_multi_arclength_tables_round_trip(tmp_path, oracle):
     """CachedMultiArclengthWeightLookupTable: compute + write, then reload from
     the .cwt file (the reference's Tests/WeightTableTest.cpp checks the reload

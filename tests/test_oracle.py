@@ -341,17 +341,27 @@ def test_norm_k4_and_default(oracle):
 # End to end: an independent NumPy path (Rodrigues on positions, float32 lerp)
 # against the oracle's FP64 path for small N.
 # ---------------------------------------------------------------------------
-def numpy_paths(oracle, pos0, N, L, K, n, table, absorption, seed=1):
+def numpy_paths(oracle, pos0, N, L, K, n, table, absorption, seed=1, chain_length=1, burn_in=0):
+    """Chains of chain_length paths; each chain first applies burn_in unweighted
+    paths (the skipped paths of FullExperimentRunnerOptimalPerturb.cpp:559-561)
+    drawn from path ids 2^63 | (chain * burn_in + b)."""
     out = []
     inv_ds = np.float32(N) / np.float32(L)
-    for p in range(n):
-        pos = pos0.copy()
+
+    def perturb(pos, path):
         for k in range(K):
-            d, l, x = oracle.draw(seed, 0, p, k, N)
+            d, l, x = oracle.draw(seed, 0, path, k, N)
             r = l + d
             axis = (pos[r] - pos[l]) / np.linalg.norm(pos[r] - pos[l])
             for i in range(l + 1, r):
                 pos[i] = pos[l] + rodrigues(pos[i] - pos[l], axis, 2 * math.pi * float(x))
+
+    for p in range(n):
+        if p % chain_length == 0:
+            pos = pos0.copy()
+            for b in range(burn_in):
+                perturb(pos, (1 << 63) | ((p // chain_length) * burn_in + b))
+        perturb(pos, p)
         t = np.diff(pos, axis=0)
         t /= np.linalg.norm(t, axis=1, keepdims=True)
         curv = np.linalg.norm(np.diff(t, axis=0) * float(inv_ds), axis=1)
@@ -378,6 +388,24 @@ def test_chain_mode_replays_from_chain_start(oracle):
     part = oracle.simulate(pos32.astype(np.float64), case.N, case.L, 2, 40, table, case.absorption, chain_length=32,
                            path_offset=40)
     assert np.array_equal(full["log10w"][40:80], part["log10w"])
+
+
+def test_burn_in_against_numpy(oracle):
+    """Burn-in (pathsToSkip) against the NumPy chain restatement; burn-in 0 is
+    the plain chain; a range split replays the burn-in of its chain."""
+    case = cases.MID
+    table, pos32 = cases.setup(case, oracle)
+    pos = pos32.astype(np.float64)
+    ref = oracle.simulate(pos, case.N, case.L, 2, 48, table, case.absorption, seed=3, chain_length=16, burn_in=5)
+    npw = numpy_paths(oracle, pos, case.N, case.L, 2, 48, table, case.absorption, seed=3, chain_length=16, burn_in=5)
+    assert np.allclose(ref["log10w"], npw, rtol=1e-9, atol=1e-9)
+    plain = oracle.simulate(pos, case.N, case.L, 2, 48, table, case.absorption, seed=3, chain_length=16)
+    assert np.array_equal(plain["log10w"], oracle.simulate(pos, case.N, case.L, 2, 48, table, case.absorption,
+                                                           seed=3, chain_length=16, burn_in=0)["log10w"])
+    assert not np.allclose(plain["log10w"][::16], ref["log10w"][::16])
+    part = oracle.simulate(pos, case.N, case.L, 2, 20, table, case.absorption, seed=3, chain_length=16, burn_in=5,
+                           path_offset=20)
+    assert np.array_equal(ref["log10w"][20:40], part["log10w"])
 
 
 def test_reference_fp32_close_to_fp64(oracle):

@@ -21,7 +21,7 @@ EXPORTED = [
     "twisty_hip_last_error", "twisty_hip_set_tables", "twisty_hip_set_curve", "twisty_hip_run",
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
-    "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions",
+    "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions", "twisty_hip_set_burn_in",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -104,6 +104,7 @@ def _declare(L):
     L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
     L.twisty_hip_set_weight_histogram.argtypes = [vp, u32, C.c_double, C.c_double, C.c_double]
     L.twisty_hip_set_keep_final_state.argtypes = [vp, C.c_int]
+    L.twisty_hip_set_burn_in.argtypes = [vp, u32]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
     L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
@@ -211,6 +212,10 @@ class Context:
         h = np.empty(self.weight_bins, np.uint64)
         self._check(lib().twisty_hip_get_weight_histogram(self._h, _ptr(h)))
         return h
+
+    def set_burn_in(self, paths_per_chain):
+        """Unweighted paths at the start of every chain of the following runs (pathsToSkip)."""
+        self._check(lib().twisty_hip_set_burn_in(self._h, paths_per_chain))
 
     def set_keep_final_state(self, enable=True):
         self._check(lib().twisty_hip_set_keep_final_state(self._h, int(enable)))

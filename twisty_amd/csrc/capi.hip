@@ -104,7 +104,7 @@ void end_frame(const float endDir[3], float e[3], float f1[3], float f2[3])
 
 struct FwdRequest {
     uint64_t numPaths = 0, pathOffset = 0, seed = 0;
-    uint32_t K = 0, chainLength = 1;
+    uint32_t K = 0, chainLength = 1, burnIn = 0;
     bool perPath = false, perJoint = false, pathLog2 = false, pathBin = false, finalSeg = false;
 };
 
@@ -154,6 +154,7 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     p.numChains = (r.numPaths + r.chainLength - 1) / r.chainLength;
     p.numPaths = r.numPaths;
     p.chainLength = r.chainLength;
+    p.burnIn = r.burnIn;
     p.N = N;
     p.K = r.K;
     p.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
@@ -472,6 +473,7 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     r.pathLog2 = hist;
     r.pathBin = hist;
     r.finalSeg = ctx->keepFinalState;
+    r.burnIn = ctx->burnIn;
     int rc = enqueue_forward(ctx, r, "run");
     if (rc) return rc;
     if (hist) {
@@ -502,6 +504,15 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastFinalPaths = ctx->keepFinalState ? numPaths : 0;
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_burn_in(twisty_hip_ctx *ctx, uint32_t pathsPerChain)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (pathsPerChain > (1u << 24))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_burn_in: at most 2^24 burn-in paths per chain");
+    ctx->burnIn = pathsPerChain;
     return TWISTY_HIP_OK;
 }
 

@@ -41,6 +41,7 @@ struct twisty_hip_ctx {
     float endDir[3] = { 0, 0, 1 };
     float startPoint[3] = { 0, 0, 0 };  // first point of the initial curve
     bool keepFinalState = false;         // run() keeps every path's final segments
+    uint32_t burnIn = 0;                 // run(): unweighted paths at the start of every chain
     uint64_t lastFinalPaths = 0;         // paths whose final segments dFinalSeg holds
     bool haveCurve = false;
 

@@ -111,6 +111,13 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                         smem[s * SLOT + 2 * BLOCK + lane] = p.initSeg[3 * i + 2];
                     }
                 }
+                // Burn-in (FullExperimentRunnerOptimalPerturb.cpp:559-561):
+                // states perturbed, never weighted; draws from path ids with
+                // the top bit set, which weighted paths never reach.
+                const uint64_t burnBase = (p.pathOffset / p.chainLength + chain) * (uint64_t)p.burnIn;
+                for (uint32_t b = 0; b < p.burnIn; ++b)
+                    perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream,
+                          (1ull << 63) | (burnBase + b), K, N, p.maxDiff);
             }
 
             perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream, path, K, N,

@@ -19,6 +19,7 @@ struct FwdParams {
     uint64_t numChains;    // ceil(numPaths / chainLength)
     uint64_t numPaths;
     uint32_t chainLength;
+    uint32_t burnIn;       // paths perturbed but not weighted at the start of every chain
     uint32_t N;
     uint32_t K;
     uint32_t maxDiff;

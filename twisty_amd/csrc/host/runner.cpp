@@ -70,7 +70,12 @@ ExperimentRunner::ExperimentParameters ExperimentRunner::ParseExperimentParamsFr
     ep.weightingParameters.absorption = (float)w.at("absorption").number();
     if (ep.weightingParameters.weightingMethod == WeightingMethod::SimplifiedModel)
         ep.weightingParameters.absorption = 0.0f;
-    ep.weightingParameters.scatter = (float)w.at("scatter").number();
+    // The sample configs of the reference list "scatterValues": [s, ...] where
+    // its parser reads "scatter"; the first value is taken when "scatter" is absent.
+    if (!w.contains("scatter") && w.contains("scatterValues"))
+        ep.weightingParameters.scatter = (float)w.at("scatterValues").at(0).number();
+    else
+        ep.weightingParameters.scatter = (float)w.at("scatter").number();
     if (w.contains("bias")) ep.weightingParameters.bias = (float)w.at("bias").number();
     ep.weightingParameters.minBound = 0.0f;
     ep.weightingParameters.maxBound = 10.0f / ep.weightingParameters.eps;
@@ -179,7 +184,7 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
         return fail("only the radiative-transfer weighting (weightFunction 0) runs on the HIP kernels");
     if (ep.perturbMethod != PerturbMethod::GeometricRandom)
         return fail("Invalid Perturb Method");  // FullExperimentRunnerOptimalPerturb.cpp:320-325
-    if (ep.numPathsToSkip != 0) return fail("pathsToSkip (burn-in) is not supported by the HIP runner");
+    if (ep.numPathsToSkip > (1u << 24)) return fail("pathsToSkip above 2^24");
     const uint32_t N = ep.numSegmentsPerCurve;
     if (N < 4) return fail("numSegments must be >= 4");
     const float ds = bc.arclength / (float)N;
@@ -207,6 +212,11 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
     const auto &tv = table.AccessLookupTable();
     rc = twisty_hip_set_tables(ctx, tv.data(), (uint32_t)tv.size(), 1, table.GetMinCurvature(),
           table.GetMaxCurvature(), table.GetCurvatureStepSize(), table.GetDs(), ep.weightingParameters.absorption);
+    if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    // pathsToSkip: burn-in at the start of every chain, as the reference skips
+    // numPathsToSkip paths at the start of every thread's chain
+    // (FullExperimentRunnerOptimalPerturb.cpp:286, 469-478, 559-561).
+    rc = twisty_hip_set_burn_in(ctx, (uint32_t)ep.numPathsToSkip);
     if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
     const auto bc13 = PerturbUtils::ToCudaSafe(bc);
     rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(positions.data()), N, bc13.data());

@@ -199,8 +199,8 @@ int twisty_hip_last_adjoint_timing(twisty_hip_ctx *ctx, float *adjointMs);
  * ------------------------------------------------------------------------ */
 typedef struct twisty_hip_batch twisty_hip_batch;
 
-/* One experiment of a batch.  Paths restart from the experiment's initial curve
- * (chainLength 1); draws as twisty_hip_run with path ids 0..numPaths-1. */
+/* One experiment of a batch: draws, chains and burn-in as twisty_hip_run with
+ * path ids 0..numPaths-1, pathOffset 0, the same chainLength and burn-in. */
 typedef struct twisty_hip_experiment {
     uint64_t num_paths;
     uint64_t seed;
@@ -209,6 +209,8 @@ typedef struct twisty_hip_experiment {
     uint32_t table_index;             /* < numTables of the batch */
     uint32_t positions_offset;        /* first point (not float) of its curve in `positions` */
     float arclength;                  /* the boundary's arclength (ds = arclength / N) */
+    uint32_t chain_length;            /* paths per chain (0 or 1: every path restarts) */
+    uint32_t burn_in;                 /* unweighted paths opening every chain (<= 2^24) */
 } twisty_hip_experiment;
 
 /* Launch schedule of a batch: blockSize 256 or 512; blocksPerCU 0 = as many as

@@ -88,6 +88,18 @@ int twisty_host_write_path_batches(const char *experimentDir, const char *prepen
       const float boundary[13], uint32_t numSegments, const float *positions, const float *log10w, uint64_t numPaths,
       double normLog10, char *errorOut, uint32_t errorLen);
 
+/* BatchedExperimentRunner: numExperiments experiments sharing the
+ * experimentParams of a JSON config (text), each with its own boundary
+ * (13 floats, BoundaryConditions_CudaSafe layout with the arclength) and seed
+ * (0 = from the clock), run in one batched launch; out[e] as
+ * twisty_host_run_config would report experiment e run alone with the same
+ * chain length.  Replaces the one-runner-per-geometry loops of the
+ * multi-experiment drivers (Experiments/BeamSpreadExperiment.cpp:209-284,
+ * Experiments/RingBenchmark.cpp:166-215).  deviceMs (nullable) receives the
+ * batched launch's device time. */
+int twisty_host_run_experiments(const char *jsonText, const float *boundaries, const uint64_t *seeds,
+      uint32_t numExperiments, twisty_host_result *out, double *deviceMs, char *errorOut, uint32_t errorLen);
+
 /* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
  * config given as text: parse experiment.experimentParams and
  * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.

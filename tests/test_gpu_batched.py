@@ -72,6 +72,38 @@ def test_batched_matches_plain_runs(hip_device):
     ctx.close()
 
 
+def test_batched_chains_and_burn_in_match_plain_runs(hip_device):
+    """Chains and burn-in: every experiment bit-identical to twisty_hip_run with
+    the same chain length and burn-in when both use the same lanes per path."""
+    w = workloads.batch_experiments(n_exp=6, seed=11, K=3, n_range=(5, 120), paths_range=(500, 3000), n_tables=2,
+                                    bins=1024, numStepsInt=2000)
+    shapes = [(1, 0), (16, 0), (16, 20), (1, 5), (250, 3), (7, 1)]
+    for x, (L, B) in zip(w["experiments"], shapes):
+        x["chain"], x["burn_in"] = L, B
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    sched = b.get_schedule()
+    res, pp, _ = b.run(per_path=True)
+    for x, r, g in zip(w["experiments"], res, pp):
+        m = w["meta"][x["table"]]
+        ds = float(np.float32(x["arclength"]) / np.float32(x["N"]))
+        ctx.set_tables(w["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]), ds, float(m[4]))
+        ctx.set_curve(x["positions"], x["boundary"])
+        ctx.set_burn_in(x["burn_in"])
+        ref = ctx.run(x["paths"], x["K"], seed=x["seed"], chain_length=x["chain"], per_path=True)
+        a, c = g.astype(np.float64), ref["log10w"].astype(np.float64)
+        ok = np.isfinite(c)
+        assert np.array_equal(np.isnan(a), np.isnan(c)) and r["num_valid"] == ref["num_valid"]
+        if batch_lanes(x["N"], sched["segments_per_lane"]) == plain_lanes(x["N"]):
+            assert np.array_equal(a[ok], c[ok]), f"N={x['N']} chain={x['chain']} burn={x['burn_in']}"
+        else:
+            assert np.abs(a[ok] - c[ok]).max() <= 1e-5 * np.abs(c[ok]).max()
+        assert abs(r["log10_sum_w"] - ref["log10_sum_w"]) <= 1e-6 * abs(ref["log10_sum_w"]) + 1e-9
+    ctx.set_burn_in(0)
+    b.close()
+    ctx.close()
+
+
 def test_batched_replays_are_deterministic(hip_device):
     w = hetero_batch()
     ctx = capi.Context(0)

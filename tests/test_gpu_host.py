@@ -131,3 +131,95 @@ def test_runner_path_batch_export(hip_device, tmp_path):
     total = math.log10(np.sum(r["weights"]))
     assert abs(total - b["log10_experiment_weight"]) < 1e-5
     assert np.abs(r["five_segment"][:, 0] - np.arccos(tan[:, 1, 2])).max() < 1e-3
+
+
+def test_run_experiments_match_single_runs(hip_device):
+    """BatchedExperimentRunner: each experiment of one batched launch gives the
+    estimate of run_config on that experiment alone (same seed, chain length,
+    burn-in; lanes per path agree at N=20, so the paths are bit-identical and
+    only the merge order of the sums differs)."""
+    N, K, chain, skip = 20, 2, 64, 10
+    geoms = [((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2), ((0, 0, 0), (0, 0, 1), (0.3, 0, 1), (0.6, 0, 0.8), 1.5),
+             ((0, 0, 0), (1, 0, 0), (1, 0.2, 0), (1, 0, 0), 1.2)]
+    cfg = config(N=N, paths=1 << 14, K=K, chain=chain, skip=skip)
+    bcs = np.stack([host.boundary13(*g) for g in geoms])
+    seeds = [3, 4, 5]
+    res, ms = host.run_experiments(cfg, bcs, seeds)
+    print(f"[run_experiments] {len(res)} experiments, device {ms:.3f} ms", flush=True)
+    for g, s, r in zip(geoms, seeds, res):
+        one = config(N=N, paths=1 << 14, K=K, chain=chain, skip=skip, seed=s)
+        one["experiment"]["basicExperiment"].update(startPos=list(g[0]), startDir=list(g[1]), endPos=list(g[2]),
+                                                   endDir=list(g[3]), arclength=g[4])
+        ref = host.run_config(one)
+        assert r["total_paths"] == ref["total_paths"] and r["failed_paths"] == ref["failed_paths"]
+        assert abs(r["log10_experiment_weight"] - ref["log10_experiment_weight"]) <= 1e-9 * max(
+            1.0, abs(ref["log10_experiment_weight"]))
+
+
+def _beam_cfg(outdir):
+    cfg = config(N=40, paths=1 << 13, K=20, chain=1)
+    cfg["experiment"]["experimentParams"]["random"] = {"bootstrapSeed": 5, "perturbSeed": 7}
+    cfg["experiment"]["beamSpreadExperiment"] = {
+        "targetSeed": 1, "sphereRadius": 3.0, "maxArclength": 9.0, "startPos": [0.0, 0.0, 0.0],
+        "startDir": [1.0, 0.0, 0.0], "numInitialCurves": 2, "numPerInitialCurve": 3, "numArclengths": 2}
+    return cfg
+
+
+def test_beam_spread_driver(hip_device, tmp_path):
+    """The batched BeamSpread driver: beamSpreadValues.dat holds one line per
+    experiment and the value = mean over arclengths and initial curves of the
+    max over per-curve seeds; each experiment equals its run alone."""
+    cfg = _beam_cfg(tmp_path)
+    p = tmp_path / "beam.json"
+    p.write_text(json.dumps(cfg))
+    exe = os.path.join(ROOT, "twisty_amd", "lib", "twisty_beam_spread")
+    out = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(out.stdout[-600:], flush=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    text = (tmp_path / "hostgpu" / "beamSpreadValues.dat").read_text().splitlines()
+    value = float(text[1].split()[1])
+    rows = [l.split() for l in text[3:]]
+    assert len(rows) == 12
+    best = {}
+    for r in rows:
+        k = (int(r[0]), int(r[1]))
+        best[k] = max(best.get(k, -np.inf), float(r[5]))
+    want = np.log10(np.sum(10.0 ** (np.array(list(best.values())) - max(best.values())))) + max(best.values()) \
+        - math.log10(len(best))
+    assert abs(value - want) < 1e-9
+    # one experiment alone (its 64-bit seed does not fit the config's 32-bit
+    # perturbSeed, so through run_experiments, which matches run_config above)
+    r = rows[4]
+    plan = subprocess.run([exe, str(p), "--plan"], capture_output=True, text=True, timeout=60, cwd=tmp_path).stdout
+    end_dir = [float(t) for t in next(l for l in plan.splitlines() if l.startswith("endDir:")).split()[1:]]
+    end_pos = [float(t) for t in next(l for l in plan.splitlines() if l.startswith("endPos:")).split()[1:]]
+    bc = host.boundary13((0, 0, 0), (1, 0, 0), end_pos, end_dir, float(r[3]))
+    ref, _ = host.run_experiments(config(N=40, paths=1 << 13, K=20, chain=1), bc[None], [int(r[4])])
+    assert abs(ref[0]["log10_experiment_weight"] - float(r[5])) < 1e-9 * abs(float(r[5]))
+
+
+def test_ring_benchmark_driver(hip_device, tmp_path):
+    """The batched RingBenchmark driver: one row per height in ringBenchmark.dat;
+    the reference's single receiver (the last height) equals its run alone."""
+    cfg = config(N=24, paths=1 << 13, K=30, chain=1, write=True, outdir=str(tmp_path))
+    cfg["experiment"]["ringBenchmark"] = {"startPos": [0.0, 0.0, 0.0], "startDir": [1.0, 0.0, 0.0], "zMin": 3.0,
+                                          "zMax": 10.0, "numZValues": 4, "ringRadius": 5}
+    p = tmp_path / "ring.json"
+    p.write_text(json.dumps(cfg))
+    exe = os.path.join(ROOT, "twisty_amd", "lib", "twisty_ring_benchmark")
+    out = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-600:], flush=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    dat = list(tmp_path.rglob("ringBenchmark.dat"))
+    assert len(dat) == 1
+    rows = [l.split() for l in dat[0].read_text().splitlines()[1:]]
+    assert len(rows) == 4
+    assert np.allclose([float(r[0]) for r in rows], [3.0, 3.0 + 7 / 3, 3.0 + 14 / 3, 10.0], rtol=1e-6)
+    z, L = float(rows[-1][0]), float(rows[-1][1])
+    d = np.array([5.0, 0.0, z], np.float32)
+    one = config(N=24, paths=1 << 13, K=30, chain=1)
+    one["experiment"]["basicExperiment"].update(startPos=[0, 0, 0], startDir=[1, 0, 0], endPos=d.tolist(),
+                                               endDir=(d / np.linalg.norm(d)).tolist(), arclength=L)
+    ref = host.run_config(one)
+    assert abs(ref["log10_experiment_weight"] - float(rows[-1][2])) < 1e-6
+    assert "Avg path weight" in out.stdout

@@ -261,3 +261,97 @@ def test_path_batch_export_reports_unwritable_dir(tmp_path):
     bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2)
     with pytest.raises(RuntimeError, match="write_path_batches failed"):
         host.write_path_batches(blocker, bc, np.zeros((1, 5, 3), np.float32), np.zeros(1, np.float32), 0.0)
+
+
+def _mt19937_raw(seed, n):
+    """std::mt19937(seed) outputs (init_genrand seeding, as NumPy's legacy seeding)."""
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed)
+    return [int(x) for x in bg.random_raw(n)]
+
+
+def _nonzero_stream(seed, n):
+    out, raw = [], iter(_mt19937_raw(seed, 4 * n + 8))
+    while len(out) < n:
+        v = next(raw)
+        if v:
+            out.append(v)
+    return out
+
+
+def test_mt19937_known_answer():
+    # C++ [rand.predef]: the 10000th output of a default-constructed mt19937 is 4123659995
+    assert _mt19937_raw(5489, 10000)[-1] == 4123659995
+
+
+def _beam_config(tmp_path, **over):
+    cfg = _config(numSegments=40, pathsToGenerate=2000)
+    cfg["experiment"]["experimentParams"]["random"] = {"bootstrapSeed": 5, "perturbSeed": 7}
+    cfg["experiment"]["beamSpreadExperiment"] = {
+        "targetSeed": 1, "sphereRadius": 3.0, "maxArclength": 9.0, "startPos": [0.0, 0.0, 0.0],
+        "startDir": [1.0, 0.0, 0.0], "numInitialCurves": 2, "numPerInitialCurve": 3, **over}
+    p = tmp_path / "beam.json"
+    p.write_text(json.dumps(cfg))
+    return cfg, p
+
+
+def _run_driver(name, cfg_path, *args, cwd=None):
+    import subprocess
+    exe = os.path.join(os.path.dirname(host.LIB_PATH), name)
+    return subprocess.run([exe, str(cfg_path), *args], capture_output=True, text=True, timeout=120, cwd=cwd)
+
+
+def test_beam_spread_plan(tmp_path):
+    """BeamSpread geometry and seeds (BeamSpreadExperiment.cpp:183-230): receiver
+    direction from mt19937(targetSeed) uniform floats, nonzero seed streams,
+    midpoint arclengths above the minimum arclength."""
+    cfg, p = _beam_config(tmp_path, numArclengths=2)
+    out = _run_driver("twisty_beam_spread", p, "--plan", cwd=tmp_path)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    u = [np.float32(x) / np.float32(2.0 ** 32) for x in _mt19937_raw(1, 2)]
+    n1 = u[0] * np.float32(2) - np.float32(1)
+    rxy = np.sqrt(np.float32(1) - n1 * n1)
+    ang = np.float32(2) * np.float32(3.141592653589793) * u[1]
+    end_dir = np.array([n1, np.cos(ang) * rxy, np.sin(ang) * rxy], np.float64)
+    got = np.array([float(t) for t in next(l for l in lines if l.startswith("endDir:")).split()[1:]])
+    assert np.abs(got - end_dir / np.linalg.norm(end_dir)).max() < 1e-5
+    bc = host.boundary13((0, 0, 0), (1, 0, 0), 3.0 * end_dir, end_dir / np.linalg.norm(end_dir), 0.0)
+    mn = host.minimum_arclength(bc, 40)
+    exps = [l.split() for l in lines if l.startswith("experiment ")]
+    assert len(exps) == 2 * 2 * 3
+    ic = _nonzero_stream(5, 2)
+    pc = _nonzero_stream(7, 3)
+    for e in exps:
+        a, i, j = int(e[1]), int(e[2]), int(e[3])
+        assert float(e[5]) == pytest.approx(mn + (9.0 - mn) / 2 * (a + 0.5), rel=1e-5)
+        assert int(e[7]) == (ic[i] << 32) | pc[j]
+
+
+def test_ring_benchmark_plan(tmp_path):
+    """RingBenchmark geometry (RingBenchmark.cpp:186-197) for every height."""
+    cfg = _config()
+    cfg["experiment"]["ringBenchmark"] = {"startPos": [0.0, 0.0, 0.0], "startDir": [1.0, 0.0, 0.0], "zMin": 3.0,
+                                          "zMax": 10.0, "numZValues": 8, "ringRadius": 5}
+    p = tmp_path / "ring.json"
+    p.write_text(json.dumps(cfg))
+    out = _run_driver("twisty_ring_benchmark", p, "--plan")
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = [l.split() for l in out.stdout.splitlines() if l.startswith("experiment ")]
+    assert len(rows) == 8
+    for z, r in enumerate(rows):
+        zz = 3.0 + 1.0 * z
+        assert float(r[5]) == pytest.approx(zz, rel=1e-6) and float(r[3]) == 5.0
+        assert float(r[7]) == pytest.approx(max(math.hypot(5.0, zz) * 1.1, 3.0), rel=1e-6)
+
+
+def test_run_experiments_errors(tmp_path):
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2)
+    short = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 0.5)
+    with pytest.raises(RuntimeError, match="experiment 1: no initial curve"):
+        host.run_experiments(_config(), np.stack([bc, short]), [1, 2])
+    with pytest.raises(ValueError, match="one seed per experiment"):
+        host.run_experiments(_config(), np.stack([bc, bc]), [1])
+    if capi.device_count() == 0:
+        with pytest.raises(RuntimeError, match="twisty_hip_create"):
+            host.run_experiments(_config(), np.stack([bc, bc]), [1, 2])

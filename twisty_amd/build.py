@@ -70,13 +70,16 @@ def build(asm=False, verbose=True):
     return out
 
 
-HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "export.cpp", "host_capi.cpp"]
+HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "export.cpp", "batch_runner.cpp",
+                "host_capi.cpp"]
+DRIVERS = {"twisty_full_experiment": "full_experiment_main.cpp", "twisty_beam_spread": "beam_spread_main.cpp",
+           "twisty_ring_benchmark": "ring_benchmark_main.cpp"}
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               f"-I{os.path.join(ROOT, 'include')}"]
 
 
 def build_host():
-    """libtwisty_host.so (C++ host layer) and the twisty_full_experiment driver."""
+    """libtwisty_host.so (C++ host layer) and the driver executables."""
     cxx = os.environ.get("CXX", "g++")
     hdir = os.path.join(CSRC, "host")
     srcs = [os.path.join(hdir, s) for s in HOST_SOURCES]
@@ -84,9 +87,10 @@ def build_host():
     out = os.path.join(LIBDIR, "libtwisty_host.so")
     run([cxx, *HOST_FLAGS, "-shared", "-o", out + ".tmp", *srcs, *link])
     os.replace(out + ".tmp", out)
-    exe = os.path.join(LIBDIR, "twisty_full_experiment")
-    run([cxx, *HOST_FLAGS, "-o", exe + ".tmp", os.path.join(hdir, "full_experiment_main.cpp"), *srcs, *link])
-    os.replace(exe + ".tmp", exe)
+    for name, main_src in DRIVERS.items():
+        exe = os.path.join(LIBDIR, name)
+        run([cxx, *HOST_FLAGS, "-o", exe + ".tmp", os.path.join(hdir, main_src), "-ltwisty_host", *link])
+        os.replace(exe + ".tmp", exe)
     return out
 
 

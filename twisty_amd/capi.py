@@ -38,6 +38,8 @@ class Experiment(C.Structure):
         ("table_index", C.c_uint32),
         ("positions_offset", C.c_uint32),
         ("arclength", C.c_float),
+        ("chain_length", C.c_uint32),
+        ("burn_in", C.c_uint32),
     ]
 
 
@@ -263,8 +265,9 @@ class Context:
 class Batch:
     """A batch of experiments on one context (twisty_hip_batch_*).
 
-    experiments: list of dicts with N, paths, K, seed, table, arclength and
-    positions ((N+1, 3)); tables: (numTables, bins); meta: (numTables, 5) =
+    experiments: list of dicts with N, paths, K, seed, table, arclength,
+    positions ((N+1, 3)) and optionally chain (paths per chain, default 1) and
+    burn_in (default 0); tables: (numTables, bins); meta: (numTables, 5) =
     minK, maxK, step, ds, absorption.
     """
 
@@ -278,7 +281,8 @@ class Batch:
         for i, x in enumerate(experiments):
             p = np.ascontiguousarray(x["positions"], np.float32).reshape(-1, 3)
             assert p.shape[0] == x["N"] + 1
-            arr[i] = Experiment(x["paths"], x.get("seed", 1), x["N"], x["K"], x["table"], off, x["arclength"])
+            arr[i] = Experiment(x["paths"], x.get("seed", 1), x["N"], x["K"], x["table"], off, x["arclength"],
+                                x.get("chain", 1), x.get("burn_in", 0))
             pos.append(p)
             off += p.shape[0]
             self.paths.append(x["paths"])

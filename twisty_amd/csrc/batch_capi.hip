@@ -100,6 +100,8 @@ int plan(twisty_hip_batch *b)
         d.table = x.table_index;
         d.invDs = (float)N / x.arclength;  // CurvePerturbUtils.cpp:223
         d.absorbLog10PerPath = (float)(b->absorbLog2[e] * 0.30102999566398119521);
+        d.chainLength = std::max<uint32_t>(1u, x.chain_length);
+        d.burnIn = x.burn_in;
         rows = std::max(rows, d.scratchRow + 1);
         pathOut += x.num_paths;
     }
@@ -115,20 +117,26 @@ int plan(twisty_hip_batch *b)
         if (bpc * sc.block_size > 2048) bpc = 2048 / sc.block_size;
         if (bpc < 1) bpc = 1;
     }
-    // work items, longest first (cost ~ paths * (K * sub-chain length + N))
+    // work items of whole chains (about tilePaths paths each), longest first
+    // (cost ~ paths * (K * sub-chain length + N) plus the burn-in)
     std::vector<BatchItem> items;
     for (uint32_t e = 0; e < nE; ++e) {
-        for (uint64_t p0 = 0; p0 < de[e].numPaths; p0 += sc.tile_paths) {
+        const uint64_t L = de[e].chainLength;
+        const uint64_t chains = (de[e].numPaths + L - 1) / L;
+        const uint64_t per = std::max<uint64_t>(1, sc.tile_paths / L);
+        for (uint64_t c0 = 0; c0 < chains; c0 += per) {
             BatchItem it;
-            it.begin = p0;
+            it.begin = c0;
             it.exp = e;
-            it.count = (uint32_t)std::min<uint64_t>(sc.tile_paths, de[e].numPaths - p0);
+            it.count = (uint32_t)std::min<uint64_t>(per, chains - c0);
             items.push_back(it);
         }
     }
     auto cost = [&](const BatchItem &it) {
         const BatchExp &d = de[it.exp];
-        return (double)it.count * ((double)d.K * (double)(d.maxDiff + 2) * 0.5 * (double)d.G + (double)d.N);
+        const double perturb = (double)d.K * (double)(d.maxDiff + 2) * 0.5 * (double)d.G;
+        const double paths = std::min<double>((double)it.count * d.chainLength, (double)d.numPaths);
+        return paths * (perturb + (double)d.N) + (double)it.count * d.burnIn * perturb;
     };
     std::stable_sort(items.begin(), items.end(), [&](const BatchItem &a, const BatchItem &c) { return cost(a) > cost(c); });
     std::vector<uint32_t> start(nE + 1, 0), list(items.size());
@@ -286,7 +294,7 @@ int twisty_hip_batch_create(twisty_hip_ctx *ctx, const twisty_hip_experiment *ex
         const uint32_t N = x.num_segments;
         if (N < 4 || x.num_paths == 0 || x.perturbations_per_path == 0 || x.table_index >= numTables ||
               !(x.arclength > 0.0f) || (uint64_t)x.positions_offset + N >= numPoints ||
-              x.num_paths > (1ull << 40))
+              x.num_paths > (1ull << 40) || x.burn_in > (1u << 24))
             return bail(TWISTY_HIP_ERR_ARG, "bad experiment " + std::to_string(e));
         b->segOffset.push_back((uint32_t)segs.size());
         for (uint32_t i = 0; i < N; ++i) {

@@ -39,7 +39,7 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     for (int w = 0; w < WMAX; ++w) scrB[w] = ldsBase + (scr - (uint32_t)w * SLOT) * 4u;
     const float *init = p.segs + e.segOffset;
     for (uint32_t q = lane / G; q < item.count; q += groupsPerBlock) {
-        const uint64_t local = item.begin + q;
+        const uint64_t chain = item.begin + q;
         for (uint32_t s = 0; s < e.S; ++s) {
             const uint32_t i = s * G + j;
             if (i < e.N) {
@@ -48,30 +48,38 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
                 smem[s * SLOT + 2 * BLOCK + lane] = init[3 * i + 2];
             }
         }
-        perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u, local, e.K, e.N, e.maxDiff);
-        wave_sync_lds();
-        const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, e.N, e.invDs, tv, clamped);
-        wave_sync_lds();  // joints read other lanes' columns before the next reset
-        if (j == 0) {
-            const bool invalid = (pl.flags & 1u) || ((pl.flags & 6u) == 6u);
-            float outv;
-            if (invalid) {
-                ++acc.nInvalid;
-                outv = __int_as_float(0x7fc00000);
-            } else {
-                ++acc.nValid;
-                if (pl.flags & 2u) {
-                    outv = -INFINITY;
-                } else if (pl.flags & 4u) {
-                    outv = INFINITY;
-                    lse_add(acc, INFINITY, 0.0f);
+        // burn-in: the draws of twisty_hip_run's burn-in at pathOffset 0
+        for (uint32_t b = 0; b < e.burnIn; ++b)
+            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u,
+                  (1ull << 63) | (chain * e.burnIn + b), e.K, e.N, e.maxDiff);
+        for (uint32_t sIdx = 0; sIdx < e.chainLength; ++sIdx) {
+            const uint64_t local = chain * e.chainLength + sIdx;
+            if (local >= e.numPaths) break;  // the last chain may be shorter
+            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u, local, e.K, e.N, e.maxDiff);
+            wave_sync_lds();
+            const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, e.N, e.invDs, tv, clamped);
+            wave_sync_lds();  // joints read other lanes' columns before the next reset
+            if (j == 0) {
+                const bool invalid = (pl.flags & 1u) || ((pl.flags & 6u) == 6u);
+                float outv;
+                if (invalid) {
+                    ++acc.nInvalid;
+                    outv = __int_as_float(0x7fc00000);
                 } else {
-                    lse_add(acc, pl.lsum.hi, pl.lsum.lo);
-                    const float l2 = 0.30102999566398119521f;
-                    outv = fmaf(pl.lsum.hi, l2, fmaf(pl.lsum.lo, l2, e.absorbLog10PerPath));
+                    ++acc.nValid;
+                    if (pl.flags & 2u) {
+                        outv = -INFINITY;
+                    } else if (pl.flags & 4u) {
+                        outv = INFINITY;
+                        lse_add(acc, INFINITY, 0.0f);
+                    } else {
+                        lse_add(acc, pl.lsum.hi, pl.lsum.lo);
+                        const float l2 = 0.30102999566398119521f;
+                        outv = fmaf(pl.lsum.hi, l2, fmaf(pl.lsum.lo, l2, e.absorbLog10PerPath));
+                    }
                 }
+                if (p.perPathLog10) p.perPathLog10[e.pathOut + local] = outv;
             }
-            if (p.perPathLog10) p.perPathLog10[e.pathOut + local] = outv;
         }
     }
 }

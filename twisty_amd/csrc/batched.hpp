@@ -21,6 +21,8 @@ struct BatchExp {
     uint32_t table;       // index into the table descriptors
     float invDs;          // N / arclength
     float absorbLog10PerPath;
+    uint32_t chainLength; // paths per chain (>= 1)
+    uint32_t burnIn;      // unweighted paths opening every chain
 };
 
 struct BatchTable {
@@ -29,7 +31,8 @@ struct BatchTable {
     float minK, maxK, step;
 };
 
-// A work item: paths [begin, begin + count) of one experiment.
+// A work item: chains [begin, begin + count) of one experiment (paths when
+// its chain length is 1).
 struct BatchItem {
     uint64_t begin;
     uint32_t exp;

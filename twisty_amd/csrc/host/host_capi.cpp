@@ -169,6 +169,47 @@ int twisty_host_write_path_batches(const char *experimentDir, const char *prepen
     return 0;
 }
 
+int twisty_host_run_experiments(const char *jsonText, const float *boundaries, const uint64_t *seeds,
+      uint32_t numExperiments, twisty_host_result *out, double *deviceMs, char *errorOut, uint32_t errorLen)
+{
+    if (!jsonText || !boundaries || !seeds || !out || numExperiments == 0) return 1;
+    try {
+        const Json cfg = Json::Parse(jsonText);
+        const ExperimentRunner::ExperimentParameters ep = ExperimentRunner::ParseExperimentParamsFromConfig(cfg);
+        std::vector<BatchedExperimentRunner::Experiment> exps(numExperiments);
+        for (uint32_t e = 0; e < numExperiments; ++e) {
+            exps[e].boundaryConditions = from13(boundaries + 13ull * e);
+            exps[e].seed = seeds[e];
+        }
+        BatchedExperimentRunner runner(ep);
+        const auto res = runner.RunExperiments(exps);
+        if (!res) {
+            put_error(errorOut, errorLen, runner.LastError());
+            return 1;
+        }
+        for (uint32_t e = 0; e < numExperiments; ++e) {
+            const ExperimentRunner::ExperimentResults &r = (*res)[e];
+            twisty_host_result &o = out[e];
+            o.log10_experiment_weight = r.experimentWeight.log10;
+            o.log10_avg_path_weight = (r.experimentWeight / (double)r.totalPathsGenerated).log10;
+            o.log10_sum_squares = r.log10SumSquares;
+            o.total_paths = r.totalPathsGenerated;
+            o.failed_paths = r.numFailedPaths;
+            o.clamped_joints = r.numClamped;
+            o.setup_ms = (double)r.setupExperimentMs;
+            o.perturb_ms = (double)r.perturbExperimentMs;
+            o.weighting_ms = (double)r.weightingExperimentMs;
+            o.total_ms = (double)r.totalExperimentMs;
+            o.kernel_ms = r.kernelMs;
+        }
+        if (deviceMs) *deviceMs = runner.DeviceMs();
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int twisty_host_run_config(const char *jsonText, twisty_host_result *out, char *errorOut, uint32_t errorLen)
 {
     if (!jsonText || !out) return 1;

@@ -85,6 +85,7 @@ ExperimentRunner::ExperimentParameters ExperimentRunner::ParseExperimentParamsFr
         if (h.contains("chainLength")) ep.chainLength = (uint32_t)h.at("chainLength").number();
         if (h.contains("device")) ep.device = (int)h.at("device").number();
         if (h.contains("writeFiles")) ep.writeFiles = h.at("writeFiles").boolean();
+        if (h.contains("autotune")) ep.autotune = h.at("autotune").boolean();
     }
     return ep;
 }

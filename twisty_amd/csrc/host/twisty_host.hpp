@@ -265,9 +265,10 @@ class ExperimentRunner {
         bool useGpu = true;
         // twisty_hip settings
         uint32_t perturbationsPerPath = 1;  // K; the reference applies 1 between weightings
-        uint32_t chainLength = 1000000;     // paths sharing one carried state (reference: 1e6/thread)
+        uint32_t chainLength = 0;           // paths sharing one carried state; 0 = about 65536 chains
         int device = 0;
         bool writeFiles = true;             // table CSV, Results-style files
+        bool autotune = false;              // batched runs: time the schedules first
     };
 
     struct ExperimentResults {
@@ -322,6 +323,34 @@ class FullExperimentRunnerOptimalPerturb_HIP : public ExperimentRunner {
 
    private:
     std::string m_lastError;
+};
+
+// Many experiments of one parameter set in one persistent batched launch: the
+// callers are the multi-experiment drivers (BeamSpreadExperiment.cpp:209-284,
+// RingBenchmark.cpp:166-215), which in the reference construct and run one
+// ExperimentRunner per geometry.  Every experiment uses the parameters'
+// path count, segment count, weighting, K, chain length and burn-in; tables
+// are built once per distinct ds.  Results are those RunExperiment gives for
+// the same experiment (seed 0 = from the clock).
+class BatchedExperimentRunner {
+   public:
+    struct Experiment {
+        PerturbUtils::BoundaryConditions boundaryConditions;
+        uint64_t seed = 0;
+    };
+    explicit BatchedExperimentRunner(const ExperimentRunner::ExperimentParameters &experimentParams);
+    // nullopt on failure (LastError() says why).
+    std::optional<std::vector<ExperimentRunner::ExperimentResults>> RunExperiments(
+          const std::vector<Experiment> &experiments);
+    const std::string &LastError() const { return m_lastError; }
+    uint32_t NumTables() const { return m_numTables; }
+    double DeviceMs() const { return m_deviceMs; }
+
+   private:
+    ExperimentRunner::ExperimentParameters m_ep;
+    std::string m_lastError;
+    uint32_t m_numTables = 0;
+    double m_deviceMs = 0.0;
 };
 
 // ---------------------------------------------------------------------------

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
-            "twisty_host_write_path_batches"]
+            "twisty_host_write_path_batches", "twisty_host_run_experiments"]
 
 _lib = None
 
@@ -52,6 +52,8 @@ def lib():
         L.twisty_host_initial_arc_exit_jacobian.argtypes = [vp, u32, vp, vp]
         L.twisty_host_cached_rt_tables.argtypes = [C.c_char_p, f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, f32,
                                                    u32, C.c_int, vp, vp, C.POINTER(C.c_int), C.c_char_p, u32]
+        L.twisty_host_run_experiments.argtypes = [C.c_char_p, vp, vp, u32, C.POINTER(HostResult),
+                                                  C.POINTER(C.c_double), C.c_char_p, u32]
         L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
                                                      C.c_uint64, f64, C.c_char_p, u32]
         _lib = L
@@ -187,6 +189,25 @@ def read_path_batches(experiment_dir, prepend=""):
         "text_weights": np.loadtxt(os.path.join(d, prepend + "Log10PathWeights_Text.pwt"), ndmin=1),
         "five_segment": np.fromfile(os.path.join(d, prepend + "FiveSegment_Binary.bdt"), np.float32).reshape(-1, 3),
     }
+
+
+def run_experiments(config, boundaries, seeds):
+    """Experiments sharing a config's experimentParams, one batched launch.
+
+    boundaries: (E, 13) BoundaryConditions_CudaSafe rows with the arclength;
+    seeds: E seeds (0 = clock).  Returns (list of result dicts, device ms)."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    b = np.ascontiguousarray(boundaries, np.float32).reshape(-1, 13)
+    s = np.ascontiguousarray(seeds, np.uint64)
+    n = b.shape[0]
+    if s.shape != (n,):
+        raise ValueError("one seed per experiment")
+    res = (HostResult * n)()
+    ms = C.c_double()
+    err = C.create_string_buffer(1024)
+    if lib().twisty_host_run_experiments(text.encode(), _p(b), _p(s), n, res, C.byref(ms), err, 1024) != 0:
+        raise RuntimeError(f"run_experiments failed: {err.value.decode()}")
+    return [{k: getattr(r, k) for k, _ in r._fields_} for r in res], ms.value
 
 
 def run_config(config):

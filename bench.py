@@ -109,6 +109,48 @@ def measure_adjoint(ctx, w, reps=3):
     return out
 
 
+def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22, reps=5):
+    """Independent-path estimator (paths built by recursive sphere intersection,
+    ExperimentBase.cpp:1246-1386) at N=66 (the count nearest 64 its recursion
+    allows), 2^22 paths, sigma=0.05; with the oracle timed on a CPU sample."""
+    import numpy as np
+
+    ds = float(np.float32(L) / np.float32(N))
+    table = host.build_rt_table(ds, 1.0, float(np.float32(0.05) ** 2), 0.1, 0.004, numStepsInt=20000,
+                                numCurvatureSteps=4096)
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
+    ctx = capi.Context(0)
+    ctx.set_tables(table.table, table.minK, table.maxK, table.step, table.ds, 0.004)
+    ctx.set_curve(host.initial_arc(bc, N), bc)
+    ctx.run_generated(paths, seed=1)
+    ms = []
+    for _ in range(reps):
+        r = ctx.run_generated(paths, seed=1)
+        ms.append(ctx.last_timing()[0])
+    ctx.close()
+    kms = statistics.fmean(ms)
+    out = {"workload": f"independent paths: N={N}, arclength {L}, 2^22 paths, 4096-bin RT table sigma=0.05",
+           "kernel": "generate_kernel", "kernel_ms": kms, "joint_weights_per_s": paths * (N - 1) / (kms / 1e3),
+           "num_valid": r["num_valid"], "log10_sum_w": r["log10_sum_w"], "cpu_baseline": None}
+    if cpu_seconds:
+        from oracle import pyoracle
+
+        if not os.path.exists(pyoracle.LIB_PATH):
+            pyoracle.build()
+        threads = os.cpu_count() or 1
+        n = 1 << 14
+        secs = pyoracle.generate(bc[:12].astype(np.float64), L, N, n, table, 0.004, seed=1, threads=threads,
+                                 want_w=False)["seconds"]
+        while secs * 2 <= cpu_seconds and n < paths:
+            n *= 2
+            secs = pyoracle.generate(bc[:12].astype(np.float64), L, N, n, table, 0.004, seed=1, threads=threads,
+                                     want_w=False)["seconds"]
+        out["cpu_baseline"] = {"value": n * (N - 1) / secs, "unit": "joint-weights/s", "cores": threads,
+                               "kind": "port", "sample": f"{n} paths of the same workload, oracle/ FP64 restatement "
+                                                         f"on {threads} std::threads, {secs:.1f} s"}
+    return out
+
+
 def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     """The batched workload in one graph-replayed launch vs one plain run per experiment."""
     import numpy as np
@@ -171,6 +213,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-generated", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -235,6 +278,8 @@ def main(argv=None):
     if not a.no_batched:
         out["batched"] = measure_batched(ctx, capi)
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
+    if not a.no_generated:
+        out["generated"] = measure_generated(capi, host, None if a.no_cpu_baseline else a.cpu_seconds / 4)
     if not a.no_cpu_baseline:
         cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
         out["cpu_baseline"] = cb

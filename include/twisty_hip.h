@@ -119,6 +119,23 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
+/* Independent-path estimator: ExperimentBase::MSegmentPathGenerationMC
+ * (Experiments/ExperimentBase.cpp:1246-1386).  Each path is built from scratch
+ * between the hinge points of the curve's boundary (set_curve) by recursive
+ * sphere intersection (ResolveEvenNumberOfSegments / ResolveThreeSegments /
+ * ResolveTwoSegments, :617-983), with ds = arclength / N, and weighted with
+ * the context's table as twisty_hip_run weighs a path.  N must be
+ * 2 + 2^k * 2 or 2 + 2^k * 3 (k >= 1: 6, 8, 10, 14, 18, 26, 34, 50, 66, ...),
+ * as the reference's recursion requires.  Draws: one Philox4x32-10 block per
+ * recursion node, counter (node, 0x80000000 | sub, path lo, path hi), key =
+ * seed.  A step the reference aborts with an exception (spheres that do not
+ * meet) makes the path invalid.  out: the partials over paths
+ * [pathOffset, pathOffset + numPaths) (the reference's estimate is
+ * sum / num_valid); minLog10W (nullable): the smallest valid log10 weight;
+ * perPathLog10 (nullable): every path's log10 weight, NaN when invalid. */
+int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10);
+
 /* Burn-in of the following twisty_hip_run calls: every chain first applies
  * pathsPerChain unweighted paths of K perturbations (draws keyed by path id
  * 2^63 | (chain * pathsPerChain + b), chain = absolute path id / chainLength),

@@ -100,6 +100,24 @@ int twisty_host_write_path_batches(const char *experimentDir, const char *prepen
 int twisty_host_run_experiments(const char *jsonText, const float *boundaries, const uint64_t *seeds,
       uint32_t numExperiments, twisty_host_result *out, double *deviceMs, char *errorOut, uint32_t errorLen);
 
+/* ExperimentBase::Result (Experiments/ExperimentBase.h) with the weight as log10. */
+typedef struct twisty_host_generation_result {
+    double log10_total_weight;   /* log10 of sum_valid 10^(w + normaliser) / num_valid */
+    double min_log10_w, max_log10_w;  /* with the normaliser */
+    uint64_t num_valid, num_total;
+    double kernel_ms;
+} twisty_host_generation_result;
+
+/* ExperimentBase::MSegmentPathGenerationMC (Experiments/ExperimentBase.cpp:
+ * 1246-1386) on a JSON config (text): experimentParams give N, paths,
+ * weighting, device; experiment.basicExperiment the boundary and arclength.
+ * Paths are built by recursive sphere intersection (twisty_hip_run_generated;
+ * N = 2 + 2^k * {2, 3}) and weighted with the table of ds = arclength / N.
+ * pathNormalizerLog10 is added to every path's weight, as the reference's
+ * argument of the same name (its solver passes 0). */
+int twisty_host_path_generation_mc(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
+      twisty_host_generation_result *out, char *errorOut, uint32_t errorLen);
+
 /* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
  * config given as text: parse experiment.experimentParams and
  * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.

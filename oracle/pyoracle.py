@@ -77,6 +77,11 @@ def _declare(L):
     L.oracle_adjoint.argtypes = [_p(np.float64), u32, f32, u32, u64, u64, _p(np.float32), i32, f32, f32, f32, f32,
                                  f32, C.c_int, C.c_int, C.POINTER(f64), C.c_void_p, C.c_void_p, C.c_void_p]
     L.oracle_adjoint.restype = C.c_int
+    L.oracle_generate_levels_c.argtypes = [u32, C.POINTER(u32)]
+    L.oracle_generate_levels_c.restype = C.c_int
+    L.oracle_generate.argtypes = [_p(np.float64), f32, u32, u64, u64, u64, _p(np.float32), i32, f32, f32, f32, f32,
+                                  f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _p(np.float64)]
+    L.oracle_generate.restype = C.c_int
 
 
 # ---------------------------------------------------------------------------
@@ -204,6 +209,37 @@ def adjoint(init_pos, N, arclength, K, n_paths, table, absorption, seed=1, lerp_
     if want_w:
         res["log10w"] = w
     return res
+
+
+def generate_levels(N):
+    """(levels, leaf span) of the independent-path recursion, or None when N is not 2 + 2^k {2, 3}."""
+    leaf = C.c_uint32()
+    k = lib().oracle_generate_levels_c(N, C.byref(leaf))
+    return None if k < 0 else (k, leaf.value)
+
+
+def generate(bc, arclength, N, n_paths, table, absorption, seed=1, path_offset=0, threads=None, want_w=True,
+             want_pos=False):
+    """Independent-path estimator (oracle_generate.cpp): per-path log10 weights and the sums."""
+    threads = threads or os.cpu_count() or 1
+    b = np.ascontiguousarray(bc, np.float64)
+    res = np.zeros(6, np.float64)
+    w = np.zeros(n_paths, np.float64) if want_w else None
+    v = np.zeros(n_paths, np.uint8) if want_w else None
+    pos = np.zeros(n_paths * (N + 1) * 3, np.float64) if want_pos else None
+    ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    rc = lib().oracle_generate(b, arclength, N, seed, path_offset, n_paths, table.table, len(table.table), table.ds,
+                               table.minK, table.maxK, table.step, absorption, threads, ptr(w), ptr(v), ptr(pos), res)
+    if rc != 0:
+        raise ValueError("oracle_generate rejected its arguments")
+    out = {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
+           "max_log10_w": res[4], "seconds": res[5]}
+    if want_w:
+        out["log10w"] = w
+        out["valid"] = v.astype(bool)
+    if want_pos:
+        out["positions"] = pos.reshape(n_paths, N + 1, 3)
+    return out
 
 
 def exit_bins(tangents, e, n_theta, n_phi):

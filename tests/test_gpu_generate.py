@@ -1,0 +1,134 @@
+"""Independent-path estimator (twisty_hip_run_generated) against the FP64 oracle.
+
+ExperimentBase::MSegmentPathGenerationMC (Experiments/ExperimentBase.cpp:
+1246-1386): both sides build every path from the same Philox words by the
+recursion of :617-983 and weigh it with the same table.  FP32 and FP64 can take
+different branches where a span sits within the reference's 0.001 straight
+tolerance, and a joint can fall on either side of a table bin edge, so the
+per-path bar (1e-5 relative) is asserted on 99% of the paths and the median;
+the valid/invalid split, path offsets and the aggregate follow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from twisty_amd import capi, host
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(6, 1.4, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))),
+         (10, 2.5, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))),
+         (14, 1.4, ((0, 0, 0), (1, 0, 0), (0.5, 0.3, 0.2), (0, 1, 0))),
+         (26, 1.8, ((0, 0, 0), (0, 0, 1), (1, 0, 0.2), (0.6, 0.8, 0.0))),
+         (66, 1.6, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))),
+         (130, 2.0, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1)))]
+
+
+def _setup(oracle, N, L, geom):
+    s, sd, e, ed = (np.array(v, np.float64) for v in geom)
+    sd, ed = sd / np.linalg.norm(sd), ed / np.linalg.norm(ed)
+    bc13 = host.boundary13(s, sd, e, ed, L)
+    bc12 = bc13[:12].astype(np.float64)
+    ds = float(np.float32(L) / np.float32(N))
+    t = oracle.build_table(ds, eps=float(np.float32(0.08) ** 2), numStepsInt=2000, numCurvatureSteps=2048)
+    ctx = capi.Context(0)
+    ctx.set_tables(t.table, t.minK, t.maxK, t.step, t.ds, 0.004)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    return ctx, t, bc12
+
+
+@pytest.mark.parametrize("N,L,geom", CASES, ids=[f"N{c[0]}" for c in CASES])
+def test_generated_paths_match_oracle(hip_device, oracle, N, L, geom):
+    n = 4096
+    ctx, t, bc12 = _setup(oracle, N, L, geom)
+    g = ctx.run_generated(n, seed=7, per_path=True)
+    ref = oracle.generate(bc12, L, N, n, t, 0.004, seed=7)
+    gw, rw = g["log10w"].astype(np.float64), ref["log10w"]
+    both = ~np.isnan(gw) & ref["valid"]
+    rel = np.abs(gw[both] - rw[both]) / np.abs(rw[both])
+    print(f"[generated N={N}] valid gpu {g['num_valid']} oracle {ref['n_valid']}, median rel {np.median(rel):.2e}, "
+          f"99% {np.quantile(rel, 0.99):.2e}, max {rel.max():.2e}", flush=True)
+    assert abs(g["num_valid"] - ref["n_valid"]) <= max(2, n // 1000)
+    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.99) <= 1e-5
+    assert g["num_paths"] == n and g["num_valid"] + g["num_invalid"] == n
+    # aggregates over the same valid paths
+    lse = lambda v: v.max() + math.log10(np.sum(10.0 ** (v - v.max())))
+    assert abs(lse(gw[~np.isnan(gw)]) - g["log10_sum_w"]) < 1e-5 * abs(g["log10_sum_w"])
+    assert g["min_log10_w"] == pytest.approx(np.nanmin(gw), rel=1e-6)
+    assert g["max_log10_w"] == pytest.approx(np.nanmax(gw), rel=1e-6)
+    assert abs(g["log10_sum_w"] - ref["log10_sum_w"]) < 1e-4 * abs(ref["log10_sum_w"])
+    ctx.close()
+
+
+def test_generated_path_offset_and_determinism(hip_device, oracle):
+    N, L, geom = CASES[4]
+    ctx, t, bc12 = _setup(oracle, N, L, geom)
+    full = ctx.run_generated(3000, seed=3, per_path=True)
+    part = ctx.run_generated(1000, seed=3, path_offset=1500, per_path=True)
+    again = ctx.run_generated(3000, seed=3, per_path=True)
+    a, b = full["log10w"][1500:2500], part["log10w"]
+    assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    assert np.array_equal(full["log10w"], again["log10w"], equal_nan=True)
+    ctx.close()
+
+
+def test_generated_rejects_unsupported_segment_counts(hip_device, oracle):
+    N, L = 64, 1.5
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
+    ds = float(np.float32(L) / np.float32(N))
+    t = oracle.build_table(ds, eps=0.0025, numStepsInt=2000, numCurvatureSteps=1024)
+    ctx = capi.Context(0)
+    ctx.set_tables(t.table, t.minK, t.maxK, t.step, t.ds, 0.004)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    with pytest.raises(RuntimeError, match="numSegments must be 2 \\+ 2\\^k"):
+        ctx.run_generated(100)
+    ctx.close()
+
+
+def test_generated_full_size_properties(hip_device, oracle):
+    """2^22 paths at N=66: the partials equal the log-sum-exp of the per-path
+    weights, and the valid fraction matches the oracle's on a sample."""
+    N, L, geom = CASES[4]
+    ctx, t, bc12 = _setup(oracle, N, L, geom)
+    n = 1 << 22
+    g = ctx.run_generated(n, seed=1, per_path=True)
+    ms = ctx.last_timing()[0]
+    w = g["log10w"].astype(np.float64)
+    ok = ~np.isnan(w)
+    m = w[ok].max()
+    lse = m + math.log10(np.sum(10.0 ** (w[ok] - m)))
+    print(f"[generated 2^22 N=66] {ms:.3f} ms, {n * (N - 1) / (ms / 1e3):.3e} joint-weights/s, valid "
+          f"{ok.mean():.4f}", flush=True)
+    assert g["num_valid"] == int(ok.sum()) and abs(lse - g["log10_sum_w"]) < 1e-6 * abs(lse)
+    ref = oracle.generate(bc12, L, N, 20000, t, 0.004, seed=1)
+    assert abs(ok.mean() - ref["valid"].mean()) < 0.02
+    ctx.close()
+
+
+def test_host_path_generation_mc(hip_device, oracle):
+    """The C++ host estimator on a config: Result = sum of valid 10^(w + norm) / num_valid
+    (ExperimentBase.cpp:1364-1384), min and max with the normaliser."""
+    N, L = 26, 1.8
+    cfg = {"experiment": {
+        "experimentParams": {
+            "pathsToGenerate": 50000, "pathsToSkip": 0, "numSegments": N, "name": "gen", "experimentDir": "/tmp",
+            "maxPerturbThreads": 0, "maxWeightThreads": 0, "outputBigFloatWeights": False,
+            "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
+            "random": {"bootstrapSeed": 1, "perturbSeed": 1},
+            "weighting": {"weightFunction": 0, "mu": 1.0, "eps": float(np.float32(0.08) ** 2), "numStepsInt": 2000,
+                          "numCurvatureSteps": 2048, "absorption": 0.004, "scatter": 0.1}},
+        "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1], "endDir": [0, 0, 1],
+                            "arclength": L}}}
+    r = host.path_generation_mc(cfg, seed=11, norm_log10=2.5)
+    ctx, t, bc12 = _setup(oracle, N, L, CASES[0][2])
+    g = ctx.run_generated(50000, seed=11, per_path=True)
+    w = g["log10w"][~np.isnan(g["log10w"])].astype(np.float64)
+    m = w.max()
+    want = m + math.log10(np.sum(10.0 ** (w - m))) + 2.5 - math.log10(len(w))
+    print(f"[path_generation_mc] {r}", flush=True)
+    assert r["num_total"] == 50000 and r["num_valid"] == len(w)
+    assert abs(r["log10_total_weight"] - want) < 1e-6 * abs(want)
+    assert r["min_log10_w"] == pytest.approx(w.min() + 2.5, rel=1e-6)
+    assert r["max_log10_w"] == pytest.approx(w.max() + 2.5, rel=1e-6)
+    ctx.close()

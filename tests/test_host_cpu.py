@@ -355,3 +355,11 @@ def test_run_experiments_errors(tmp_path):
     if capi.device_count() == 0:
         with pytest.raises(RuntimeError, match="twisty_hip_create"):
             host.run_experiments(_config(), np.stack([bc, bc]), [1, 2])
+
+
+def test_path_generation_mc_without_gpu():
+    if capi.device_count() > 0:
+        pytest.skip("a device is visible; the GPU tests cover the successful run")
+    cfg = _config(numSegments=26)
+    with pytest.raises(RuntimeError, match="twisty_hip_create"):
+        host.path_generation_mc(cfg)

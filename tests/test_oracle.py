@@ -415,3 +415,129 @@ def test_reference_fp32_close_to_fp64(oracle):
     b = cases.oracle_run(case, oracle, 512, precision=32)
     rel = np.abs(a["log10w"] - b["log10w"]) / np.abs(a["log10w"])
     assert rel.max() < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# Independent-path estimator (oracle_generate.cpp): ExperimentBase.cpp:617-983,
+# 1246-1386.
+# ---------------------------------------------------------------------------
+def _gen_levels_py(N):
+    n0 = N - 2
+    for k in range(1, 20):
+        if n0 in (2 << k, 3 << k):
+            return k, n0 >> k
+    return None
+
+
+def test_generate_segment_count_rule(oracle):
+    """N - 2 must halve down to 2 or 3 (ExperimentBase.cpp:810-817), exhaustively."""
+    for N in range(1, 600):
+        assert oracle.generate_levels(N) == _gen_levels_py(N), N
+
+
+def _words(oracle, seed, path, node, sub=0):
+    r = oracle.philox([node, 0x80000000 | sub, path & 0xFFFFFFFF, path >> 32], [seed & 0xFFFFFFFF, seed >> 32])
+    return [(int(r[0]) >> 8) / 2.0 ** 24, (int(r[1]) >> 8) / 2.0 ** 24, int(r[2]) & 1, (int(r[3]) >> 8) / 2.0 ** 24]
+
+
+def _gen_setup(oracle, N, L, bc):
+    ds = float(np.float32(L) / np.float32(N))
+    t = oracle.build_table(ds, eps=0.0025, numStepsInt=2000, numCurvatureSteps=1024)
+    return ds, t
+
+
+GEN_CASES = [(6, 1.4, (0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1)), (8, 1.7, (0, 0, 0), (1, 0, 0), (0.5, 0.3, 0.2),
+                                                                    (0, 1, 0)),
+             (14, 1.4, (0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1)), (34, 2.5, (0, 0, 0), (0, 0, 1), (1, 0, 0.2),
+                                                                     (0.6, 0.8, 0.0)),
+             (66, 1.6, (0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))]
+
+
+@pytest.mark.parametrize("N,L,s,sd,e,ed", GEN_CASES)
+def test_generated_paths_structure(oracle, N, L, s, sd, e, ed):
+    """Every generated path: fixed start, end and hinge points; each leaf's
+    segments exactly ds unless the leaf is within the reference's 0.001
+    straight tolerance (then its point is the midpoint); each centre within
+    reach (perSide * ds) of both ends of its span."""
+    bc = oracle.bc12(s, np.array(sd) / np.linalg.norm(sd), e, np.array(ed) / np.linalg.norm(ed))
+    ds, t = _gen_setup(oracle, N, L, bc)
+    r = oracle.generate(bc, L, N, 300, t, 0.004, seed=5, want_pos=True)
+    assert r["valid"].all()
+    P = r["positions"]
+    k, leaf = oracle.generate_levels(N)
+    n0 = N - 2
+    assert np.array_equal(P[:, 0], np.broadcast_to(bc[0:3], (300, 3)))
+    assert np.array_equal(P[:, N], np.broadcast_to(bc[6:9], (300, 3)))
+    assert np.abs(P[:, 1] - (bc[0:3] + ds * bc[3:6])).max() < 1e-15
+    assert np.abs(P[:, N - 1] - (bc[6:9] - ds * bc[9:12])).max() < 1e-15
+    exact_leaves = 0
+    for lv in range(k):
+        span = n0 >> lv
+        for q in range(1 << lv):
+            lft = 1 + q * span
+            c, a, b = P[:, lft + span // 2], P[:, lft], P[:, lft + span]
+            reach = ds * (span // 2) + 1e-12
+            assert (np.linalg.norm(c - a, axis=1) <= reach).all() and (np.linalg.norm(b - c, axis=1) <= reach).all()
+    for q in range(1 << k):
+        lft = 1 + q * leaf
+        a, b = P[:, lft + leaf - 2], P[:, lft + leaf]
+        m = P[:, lft + leaf - 1]
+        d = np.linalg.norm(b - a, axis=1)
+        straight = np.abs(2 * ds - d) < 0.001
+        exact_leaves += int((~straight).sum())
+        s1, s2 = np.linalg.norm(m - a, axis=1), np.linalg.norm(b - m, axis=1)
+        assert np.abs(s1[~straight] - ds).max(initial=0) < 1e-12 and np.abs(s2[~straight] - ds).max(initial=0) < 1e-12
+        assert np.abs(m[straight] - 0.5 * (a[straight] + b[straight])).max(initial=0) < 1e-15
+    assert exact_leaves > 0
+
+
+def test_generated_two_segment_angle_is_the_drawn_theta(oracle):
+    """N = 6: the leaf point sits on the circle of radius sqrt(ds^2 - d^2/4)
+    around its chord at the angle 2 pi u1 of its node's Philox words, measured
+    from m = normalize(n x e) towards n x m (ExperimentBase.cpp:672-697); the
+    centre's polar angle from its span start satisfies cos(phi) = 1 - 2 u0 vmax,
+    vmax = (1 - d / (2 rps)) / 2 (:910-935)."""
+    N, L = 6, 1.5
+    bc = oracle.bc12()
+    ds, t = _gen_setup(oracle, N, L, bc)
+    r = oracle.generate(bc, L, N, 64, t, 0.004, seed=9, want_pos=True)
+    P = r["positions"]
+    for p in range(64):
+        # centre (node 1), span 4 between points 1 and 5
+        a, b, c = P[p, 1], P[p, 5], P[p, 3]
+        u0, u1, coin, u3 = _words(oracle, 9, p, 1)
+        d = np.linalg.norm(b - a)
+        z = (b - a) / d
+        vmax = 0.5 * (1 - d / (2 * 2 * ds))
+        rel = (c - a) if coin == 0 else (c - b)
+        cosphi = np.dot(rel, z) / np.linalg.norm(rel) * (1 if coin == 0 else -1)
+        assert cosphi == pytest.approx(1 - 2 * u0 * vmax, abs=1e-12)
+        # leaves (nodes 2, 3)
+        for q, (lft, rgt) in enumerate(((1, 3), (3, 5))):
+            a2, b2, m2 = P[p, lft], P[p, rgt], P[p, lft + 1]
+            d2 = np.linalg.norm(b2 - a2)
+            if abs(2 * ds - d2) < 0.001:
+                continue
+            n = (b2 - a2) / d2
+            e = np.array([0.0, 1.0, 0.0]) if abs(n[0]) >= 0.99 else np.array([1.0, 0.0, 0.0])
+            m = np.cross(n, e)
+            m /= np.linalg.norm(m)
+            v = m2 - 0.5 * (a2 + b2)
+            ang = math.atan2(np.dot(v, np.cross(n, m)), np.dot(v, m)) % (2 * math.pi)
+            theta = 2 * math.pi * _words(oracle, 9, p, 2 + q)[1]
+            assert min(abs(ang - theta), 2 * math.pi - abs(ang - theta)) < 1e-9
+
+
+def test_generated_straight_line_closed_form(oracle):
+    """Arclength equal to the end distance: every node takes the straight
+    branch, the path is the segment from start to end, every curvature is 0 and
+    the weight is (N - 1) log10(T[0] exp(-absorption ds))."""
+    N = 10
+    bc = oracle.bc12()
+    ds, t = _gen_setup(oracle, N, 1.0, bc)
+    r = oracle.generate(bc, 1.0, N, 16, t, 0.004, seed=2, want_pos=True)
+    P = r["positions"]
+    assert np.abs(P[:, :, :2]).max() == 0.0
+    assert np.abs(P[:, :, 2] - np.linspace(0, 1, N + 1)).max() < 1e-7
+    want = (N - 1) * math.log10(float(t.table[0]) * float(np.float32(math.exp(float(np.float32(-0.004 * np.float32(ds)))))))
+    assert np.allclose(r["log10w"], want, rtol=1e-7)

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 
 ARCH = "gfx950"
-HIP_SOURCES = ["forward.hip", "histogram.hip", "adjoint.hip", "batched.hip", "capi.hip", "batch_capi.hip"]
+HIP_SOURCES = ["forward.hip", "histogram.hip", "adjoint.hip", "batched.hip", "generate.hip", "capi.hip", "batch_capi.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -71,6 +71,7 @@ def build(asm=False, verbose=True):
 
 
 HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "export.cpp", "batch_runner.cpp",
+                "path_generation.cpp",
                 "host_capi.cpp"]
 DRIVERS = {"twisty_full_experiment": "full_experiment_main.cpp", "twisty_beam_spread": "beam_spread_main.cpp",
            "twisty_ring_benchmark": "ring_benchmark_main.cpp"}

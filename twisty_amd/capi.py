@@ -22,6 +22,7 @@ EXPORTED = [
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
     "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions", "twisty_hip_set_burn_in",
+    "twisty_hip_run_generated",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -107,6 +108,7 @@ def _declare(L):
     L.twisty_hip_set_weight_histogram.argtypes = [vp, u32, C.c_double, C.c_double, C.c_double]
     L.twisty_hip_set_keep_final_state.argtypes = [vp, C.c_int]
     L.twisty_hip_set_burn_in.argtypes = [vp, u32]
+    L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
     L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
@@ -198,6 +200,19 @@ class Context:
             res["log10w"] = pp
         if per_joint:
             res["curv"] = pj.reshape(num_paths, self.N - 1)
+        return res
+
+    def run_generated(self, num_paths, seed=1, path_offset=0, per_path=False):
+        """Independent-path estimator (paths built by recursive sphere intersection)."""
+        out = Partials()
+        mn = C.c_double()
+        pp = np.empty(num_paths, np.float32) if per_path else None
+        self._check(lib().twisty_hip_run_generated(self._h, num_paths, path_offset, seed, C.byref(out), C.byref(mn),
+                                                   _ptr(pp)))
+        res = out.as_dict()
+        res["min_log10_w"] = mn.value
+        if per_path:
+            res["log10w"] = pp
         return res
 
     def last_timing(self):

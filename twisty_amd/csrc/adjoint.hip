@@ -83,7 +83,12 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
     const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
     const float nx = ax * inv, ny = ay * inv, nz = az * inv;
     const Mat3 R = rotation_from_axis(nx, ny, nz, rec.c, rec.s, rec.C);
-    float g00 = 0.f, g01 = 0.f, g02 = 0.f, g10 = 0.f, g11 = 0.f, g12 = 0.f, g20 = 0.f, g21 = 0.f, g22 = 0.f;
+    // With G = sum_w lambda_w d_pre,w^T, dE/dn_k = s (G_ji - G_ij) + C ((G n)_k
+    // + (G^T n)_k) is linear in the per-slot terms and s, C, n are the
+    // group's, so each lane forms its own share of dE/dn,
+    //   -s sum_w lambda_w x p_w + C sum_w (lambda_w (p_w . n) + p_w (lambda_w . n)),
+    // and the group sums three values instead of the nine entries of G.
+    float cx = 0.f, cy = 0.f, cz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     float px[WMAX], py[WMAX], pz[WMAX];
 #pragma unroll
     for (int w = 0; w < WMAX; ++w) {
@@ -91,34 +96,24 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
         px[w] = fmaf(R.m00, dx[w], fmaf(R.m10, dy[w], R.m20 * dz[w]));
         py[w] = fmaf(R.m01, dx[w], fmaf(R.m11, dy[w], R.m21 * dz[w]));
         pz[w] = fmaf(R.m02, dx[w], fmaf(R.m12, dy[w], R.m22 * dz[w]));
-        g00 = fmaf(lx[w], px[w], g00);
-        g01 = fmaf(lx[w], py[w], g01);
-        g02 = fmaf(lx[w], pz[w], g02);
-        g10 = fmaf(ly[w], px[w], g10);
-        g11 = fmaf(ly[w], py[w], g11);
-        g12 = fmaf(ly[w], pz[w], g12);
-        g20 = fmaf(lz[w], px[w], g20);
-        g21 = fmaf(lz[w], py[w], g21);
-        g22 = fmaf(lz[w], pz[w], g22);
+        cx = fmaf(ly[w], pz[w], fmaf(-lz[w], py[w], cx));
+        cy = fmaf(lz[w], px[w], fmaf(-lx[w], pz[w], cy));
+        cz = fmaf(lx[w], py[w], fmaf(-ly[w], px[w], cz));
+        const float pn = fmaf(px[w], nx, fmaf(py[w], ny, pz[w] * nz));
+        const float ln = fmaf(lx[w], nx, fmaf(ly[w], ny, lz[w] * nz));
+        bx = fmaf(lx[w], pn, fmaf(px[w], ln, bx));
+        by = fmaf(ly[w], pn, fmaf(py[w], ln, by));
+        bz = fmaf(lz[w], pn, fmaf(pz[w], ln, bz));
     }
+    float hx = fmaf(-rec.s, cx, rec.C * bx);
+    float hy = fmaf(-rec.s, cy, rec.C * by);
+    float hz = fmaf(-rec.s, cz, rec.C * bz);
     if constexpr (G > 1) {
-        g00 = group_sum<G>(g00);
-        g01 = group_sum<G>(g01);
-        g02 = group_sum<G>(g02);
-        g10 = group_sum<G>(g10);
-        g11 = group_sum<G>(g11);
-        g12 = group_sum<G>(g12);
-        g20 = group_sum<G>(g20);
-        g21 = group_sum<G>(g21);
-        g22 = group_sum<G>(g22);
+        hx = group_sum<G>(hx);
+        hy = group_sum<G>(hy);
+        hz = group_sum<G>(hz);
     }
-    // dE/dn, then project out the axis direction and divide by |a|.
-    const float gnx = fmaf(g00, nx, fmaf(g01, ny, g02 * nz)) + fmaf(g00, nx, fmaf(g10, ny, g20 * nz));
-    const float gny = fmaf(g10, nx, fmaf(g11, ny, g12 * nz)) + fmaf(g01, nx, fmaf(g11, ny, g21 * nz));
-    const float gnz = fmaf(g20, nx, fmaf(g21, ny, g22 * nz)) + fmaf(g02, nx, fmaf(g12, ny, g22 * nz));
-    const float hx = fmaf(rec.s, g21 - g12, rec.C * gnx);
-    const float hy = fmaf(rec.s, g02 - g20, rec.C * gny);
-    const float hz = fmaf(rec.s, g10 - g01, rec.C * gnz);
+    // project out the axis direction and divide by |a|
     const float hn = fmaf(hx, nx, fmaf(hy, ny, hz * nz));
     const float dax = fmaf(-hn, nx, hx) * inv;
     const float day = fmaf(-hn, ny, hy) * inv;

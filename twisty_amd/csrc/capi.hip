@@ -14,6 +14,7 @@
 #include "../../include/twisty_hip.h"
 #include "adjoint.hpp"
 #include "ctx.hpp"
+#include "generate.hpp"
 #include "forward.hpp"
 
 using namespace twisty_hip;
@@ -326,6 +327,7 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dHistPartial);
     (void)hipFree(ctx->dHist);
     (void)hipFree(ctx->dFinalSeg);
+    (void)hipFree(ctx->dGenMin);
     (void)hipFree(ctx->dGradTPartial);
     (void)hipFree(ctx->dGradSPartial);
     (void)hipFree(ctx->dGradT);
@@ -421,6 +423,7 @@ int twisty_hip_set_curve(twisty_hip_ctx *ctx, const float *positions, uint32_t n
     ctx->arclength = arclength;
     for (int a = 0; a < 3; ++a) ctx->endDir[a] = boundary[9 + a];
     for (int a = 0; a < 3; ++a) ctx->startPoint[a] = positions[a];
+    for (int a = 0; a < 13; ++a) ctx->boundary[a] = boundary[a];
     ctx->haveCurve = true;
     return TWISTY_HIP_OK;
 }
@@ -503,6 +506,99 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastHistBins = hist ? ctx->histTheta * ctx->histPhi : 0;
     ctx->lastFinalPaths = ctx->keepFinalState ? numPaths : 0;
     copy_partials(fo, numPaths, out);
+    ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: out is NULL");
+    if (!ctx->haveTables || !ctx->haveCurve)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_tables and set_curve first");
+    if (numPaths == 0 || numPaths > (1ull << 40))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: need 1 <= numPaths <= 2^40");
+    const uint32_t N = ctx->N;
+    uint32_t leafSpan = 0;
+    const int levels = generate_levels(N, leafSpan);
+    if (levels < 0)
+        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
+              "run_generated: numSegments must be 2 + 2^k * 2 or 2 + 2^k * 3 with k >= 1 (the recursion of "
+              "ResolveEvenNumberOfSegments halves N - 2 down to 2 or 3)");
+    const uint32_t G = generate_lanes(N);
+    const uint32_t rows = (N + 1 + G - 1) / G;
+    if (rows > 16) return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "run_generated: numSegments above 1022");
+    const uint32_t B = 256;
+    const uint32_t lds = generate_lds_bytes(G, B, N, table_stride(ctx->tableSize));
+    if (lds > ctx->ldsPerBlock)
+        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
+              "run_generated: needs " + std::to_string(lds) + " LDS bytes per block, limit " +
+                    std::to_string(ctx->ldsPerBlock));
+    uint32_t bpc = ctx->ldsPerCU / lds;
+    if (bpc * B > 2048) bpc = 2048 / B;
+    if (bpc < 1) bpc = 1;
+    const uint32_t grid = ctx->numCUs * bpc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) return rc;
+    if (perPathLog10 && (rc = ensure_dev(ctx, ctx->dPerPath, ctx->perPathCap, numPaths, "per-path buffer"))) return rc;
+    if ((rc = ensure_dev(ctx, ctx->dGenMin, ctx->genMinCap, 1, "min buffer"))) return rc;
+    const double absorb = (double)std::exp((float)(-ctx->absorption * ctx->ds));
+    const double absorbLog2PerPath = (double)(N - 1) * std::log2(absorb);
+    GenParams p {};
+    p.table = ctx->dTable;
+    p.partials = ctx->dPartials;
+    p.perPathLog10 = perPathLog10 ? ctx->dPerPath : nullptr;
+    p.minLog10 = ctx->dGenMin;
+    p.seed = seed;
+    p.pathOffset = pathOffset;
+    p.numPaths = numPaths;
+    for (int a = 0; a < 3; ++a) {
+        p.start[a] = ctx->boundary[a];
+        p.startDir[a] = ctx->boundary[3 + a];
+        p.end[a] = ctx->boundary[6 + a];
+        p.endDir[a] = ctx->boundary[9 + a];
+    }
+    p.ds = ctx->arclength / (float)N;  // ExperimentBase.cpp:1253
+    p.invDs = (float)N / ctx->arclength;
+    p.N = N;
+    p.levels = (uint32_t)levels;
+    p.leafSpan = leafSpan;
+    p.rows = rows;
+    p.tableSize = (int32_t)ctx->tableSize;
+    p.tableStride = table_stride(ctx->tableSize);
+    p.minK = ctx->minK;
+    p.maxK = ctx->maxK;
+    p.step = ctx->step;
+    p.log10of2 = 0.30102999566398119521f;
+    p.absorbLog10PerPath = (float)(absorbLog2PerPath * 0.30102999566398119521);
+    const float inf = INFINITY;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->dGenMin, &inf, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipError_t e = launch_generate(p, G, grid, B, lds, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "generate kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
+    if (e != hipSuccess) return hipfail(ctx, e, "finalize kernel launch");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
+    FinalOut fo;
+    float mn = INFINITY;
+    HIP_TRY(ctx, hipMemcpyAsync(&fo, ctx->dFinal, sizeof(FinalOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(&mn, ctx->dGenMin, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (perPathLog10)
+        HIP_TRY(ctx, hipMemcpyAsync(perPathLog10, ctx->dPerPath, numPaths * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev2));
+    ctx->lastAdjMs = 0.0f;
+    ctx->lastHistPaths = 0;
+    ctx->lastHistBins = 0;
+    ctx->lastFinalPaths = 0;
+    ctx->lastWeightBins = 0;
+    copy_partials(fo, numPaths, out);
+    if (minLog10W) *minLog10W = fo.nValid ? (double)mn : INFINITY;
     ctx->lastError.clear();
     return TWISTY_HIP_OK;
 }

@@ -40,6 +40,7 @@ struct twisty_hip_ctx {
     float arclength = 0;
     float endDir[3] = { 0, 0, 1 };
     float startPoint[3] = { 0, 0, 0 };  // first point of the initial curve
+    float boundary[13] = {};             // BoundaryConditions_CudaSafe of set_curve
     bool keepFinalState = false;         // run() keeps every path's final segments
     uint32_t burnIn = 0;                 // run(): unweighted paths at the start of every chain
     uint64_t lastFinalPaths = 0;         // paths whose final segments dFinalSeg holds
@@ -76,6 +77,8 @@ struct twisty_hip_ctx {
 
     // Adjoint.
     float *dFinalSeg = nullptr;
+    float *dGenMin = nullptr;  // min log10 weight of the independent-path run
+    uint64_t genMinCap = 0;
     uint64_t finalSegCap = 0;
     float *dGradTPartial = nullptr;
     uint64_t gradTPartialCap = 0;

@@ -210,6 +210,34 @@ int twisty_host_run_experiments(const char *jsonText, const float *boundaries, c
     return 0;
 }
 
+int twisty_host_path_generation_mc(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
+      twisty_host_generation_result *out, char *errorOut, uint32_t errorLen)
+{
+    if (!jsonText || !out) return 1;
+    try {
+        const Json cfg = Json::Parse(jsonText);
+        const ExperimentRunner::ExperimentParameters ep = ExperimentRunner::ParseExperimentParamsFromConfig(cfg);
+        const PerturbUtils::BoundaryConditions bc = ParseBasicExperiment(cfg);
+        std::string err;
+        const auto r = ExperimentBase::MSegmentPathGenerationMC(seed, ep.numPathsInExperiment, bc, ep,
+              pathNormalizerLog10, err);
+        if (!r) {
+            put_error(errorOut, errorLen, err);
+            return 1;
+        }
+        out->log10_total_weight = r->totalWeight.log10;
+        out->min_log10_w = r->minPathWeightLog10;
+        out->max_log10_w = r->maxPathWeightLog10;
+        out->num_valid = r->numValidPaths;
+        out->num_total = r->numPathsTotal;
+        out->kernel_ms = r->kernelMs;
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int twisty_host_run_config(const char *jsonText, twisty_host_result *out, char *errorOut, uint32_t errorLen)
 {
     if (!jsonText || !out) return 1;

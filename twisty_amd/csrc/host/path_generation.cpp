@@ -1,0 +1,79 @@
+// path_generation.cpp -- ExperimentBase::MSegmentPathGenerationMC
+// (Experiments/ExperimentBase.cpp:1246-1386) on the HIP independent-path
+// kernel: the table of ds = arclength / N, the boundary, one
+// twisty_hip_run_generated call, and the reference's Result.
+#include <cmath>
+
+#include "../../../include/twisty_hip.h"
+#include "twisty_host.hpp"
+
+namespace twisty {
+namespace ExperimentBase {
+
+std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperimentPaths,
+      const PerturbUtils::BoundaryConditions &bc, const ExperimentRunner::ExperimentParameters &ep,
+      double pathNormalizerLog10, std::string &error)
+{
+    error.clear();
+    if (ep.weightingParameters.weightingMethod != WeightingMethod::RadiativeTransfer) {
+        error = "only the radiative-transfer weighting (weightFunction 0) runs on the HIP kernels";
+        return std::nullopt;
+    }
+    const uint32_t N = ep.numSegmentsPerCurve;
+    if (!(bc.arclength > 0.0f) || N < 4 || numExperimentPaths == 0) {
+        error = "need arclength > 0, numSegments >= 4 and at least one path";
+        return std::nullopt;
+    }
+    const float ds = bc.arclength / (float)N;  // ExperimentBase.cpp:1253
+    PathWeighting::WeightLookupTableIntegral table(ep.weightingParameters, ds);
+    try {
+        table.Initialize(ep.maxWeightThreads > 0 ? ep.maxWeightThreads : 0);
+    } catch (const std::exception &e) {
+        error = e.what();
+        return std::nullopt;
+    }
+    // set_curve wants a curve: the straight chord when no arc exists (only its
+    // boundary is read by the generated run)
+    std::vector<Vec3> arc;
+    if (!PathGeneration::InitialArc(bc, N, arc)) {
+        arc.resize(N + 1);
+        for (uint32_t i = 0; i <= N; ++i)
+            arc[i] = bc.m_startPos + (bc.m_endPos - bc.m_startPos) * ((float)i / (float)N);
+    }
+    twisty_hip_ctx *ctx = nullptr;
+    if (twisty_hip_create(ep.device, &ctx) != TWISTY_HIP_OK) {
+        error = "twisty_hip_create failed (device " + std::to_string(ep.device) + ")";
+        return std::nullopt;
+    }
+    struct Guard {
+        twisty_hip_ctx *c;
+        ~Guard() { twisty_hip_destroy(c); }
+    } guard { ctx };
+    const auto &tv = table.AccessLookupTable();
+    int rc = twisty_hip_set_tables(ctx, tv.data(), (uint32_t)tv.size(), 1, table.GetMinCurvature(),
+          table.GetMaxCurvature(), table.GetCurvatureStepSize(), table.GetDs(), ep.weightingParameters.absorption);
+    const auto bc13 = PerturbUtils::ToCudaSafe(bc);
+    if (rc == TWISTY_HIP_OK) rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(arc.data()), N, bc13.data());
+    twisty_hip_partials part {};
+    double mn = INFINITY;
+    if (rc == TWISTY_HIP_OK) rc = twisty_hip_run_generated(ctx, numExperimentPaths, 0, seed, &part, &mn, nullptr);
+    if (rc != TWISTY_HIP_OK) {
+        error = twisty_hip_last_error(ctx);
+        return std::nullopt;
+    }
+    float fwdMs = 0.0f, totMs = 0.0f;
+    twisty_hip_last_timing(ctx, &fwdMs, &totMs);
+    Result r;
+    r.numValidPaths = part.num_valid;
+    r.numPathsTotal = numExperimentPaths;
+    r.totalWeight.log10 = part.num_valid
+          ? part.log10_sum_w + pathNormalizerLog10 - std::log10((double)part.num_valid)
+          : -INFINITY;  // the reference returns 0 without valid paths (:1380-1384)
+    r.minPathWeightLog10 = mn + pathNormalizerLog10;
+    r.maxPathWeightLog10 = part.max_log10_w + pathNormalizerLog10;
+    r.kernelMs = totMs;
+    return r;
+}
+
+}  // namespace ExperimentBase
+}  // namespace twisty

@@ -353,6 +353,26 @@ class BatchedExperimentRunner {
     double m_deviceMs = 0.0;
 };
 
+// The independent-path estimator ExperimentBase::MSegmentPathGenerationMC
+// (Experiments/ExperimentBase.cpp:1246-1386) on twisty_hip_run_generated:
+// every path built from scratch by recursive sphere intersection, weighted
+// with the radiative-transfer table of ds = arclength / N.  totalWeight is the
+// reference's sum over valid paths of 10^(w + normaliser) / numValidPaths.
+namespace ExperimentBase {
+struct Result {  // ExperimentBase.h Result
+    uint64_t numValidPaths = 0;
+    uint64_t numPathsTotal = 0;
+    BigWeight totalWeight;
+    double minPathWeightLog10 = INFINITY;
+    double maxPathWeightLog10 = -INFINITY;
+    double kernelMs = 0.0;
+};
+std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperimentPaths,
+      const PerturbUtils::BoundaryConditions &experimentGeometry,
+      const ExperimentRunner::ExperimentParameters &experimentParams, double pathNormalizerLog10,
+      std::string &error);
+}  // namespace ExperimentBase
+
 // ---------------------------------------------------------------------------
 // Minimal JSON value (objects, arrays, numbers, strings, booleans, null).
 // ---------------------------------------------------------------------------

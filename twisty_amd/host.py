@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
-            "twisty_host_write_path_batches", "twisty_host_run_experiments"]
+            "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc"]
 
 _lib = None
 
@@ -29,6 +29,17 @@ class HostResult(C.Structure):
         ("perturb_ms", C.c_double),
         ("weighting_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+class GenerationResult(C.Structure):
+    _fields_ = [
+        ("log10_total_weight", C.c_double),
+        ("min_log10_w", C.c_double),
+        ("max_log10_w", C.c_double),
+        ("num_valid", C.c_uint64),
+        ("num_total", C.c_uint64),
         ("kernel_ms", C.c_double),
     ]
 
@@ -54,6 +65,8 @@ def lib():
                                                    u32, C.c_int, vp, vp, C.POINTER(C.c_int), C.c_char_p, u32]
         L.twisty_host_run_experiments.argtypes = [C.c_char_p, vp, vp, u32, C.POINTER(HostResult),
                                                   C.POINTER(C.c_double), C.c_char_p, u32]
+        L.twisty_host_path_generation_mc.argtypes = [C.c_char_p, C.c_uint64, f64, C.POINTER(GenerationResult),
+                                                     C.c_char_p, u32]
         L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
                                                      C.c_uint64, f64, C.c_char_p, u32]
         _lib = L
@@ -208,6 +221,16 @@ def run_experiments(config, boundaries, seeds):
     if lib().twisty_host_run_experiments(text.encode(), _p(b), _p(s), n, res, C.byref(ms), err, 1024) != 0:
         raise RuntimeError(f"run_experiments failed: {err.value.decode()}")
     return [{k: getattr(r, k) for k, _ in r._fields_} for r in res], ms.value
+
+
+def path_generation_mc(config, seed=1, norm_log10=0.0):
+    """ExperimentBase::MSegmentPathGenerationMC on a reference-schema config (independent paths)."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    res = GenerationResult()
+    err = C.create_string_buffer(1024)
+    if lib().twisty_host_path_generation_mc(text.encode(), seed, norm_log10, C.byref(res), err, 1024) != 0:
+        raise RuntimeError(f"path_generation_mc failed: {err.value.decode()}")
+    return {k: getattr(res, k) for k, _ in res._fields_}
 
 
 def run_config(config):

@@ -96,16 +96,32 @@ def cpu_baseline(table, pos, w, target_seconds):
 
 
 def measure_adjoint(ctx, w, reps=3):
-    """Forward + adjoint device time at the same workload (HIP events)."""
+    """Forward + adjoint device time at the same workload (HIP events): exact
+    (every path swept) and with Russian roulette (kappa = 1) in front of the
+    adjoint, with the roulette gradient's deviation from the exact one."""
+    import numpy as np
+
     out = {}
-    for seg, key in ((False, "table_only"), (True, "full")):
-        ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
+    exact = None
+    for seg, key, kappa in ((False, "table_only", 0.0), (True, "full", 0.0), (True, "full_roulette", 1.0)):
+        ctx.set_adjoint_roulette(kappa)
+        r = ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
         tot, adj = [], []
         for _ in range(reps):
             ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
             tot.append(ctx.last_timing()[1])
             adj.append(ctx.last_adjoint_ms())
-        out[key] = {"fwd_adjoint_ms": statistics.fmean(tot), "adjoint_ms": statistics.fmean(adj)}
+        out[key] = {"fwd_adjoint_ms": statistics.fmean(tot), "adjoint_ms": statistics.fmean(adj),
+                    "paths_swept": ctx.last_adjoint_paths()}
+        if key == "full":
+            exact = r
+        if key == "full_roulette":
+            out[key]["kappa"] = kappa
+            out[key]["d_seg_rel_to_exact"] = float(np.linalg.norm(r["d_seg"] - exact["d_seg"]) /
+                                                   np.linalg.norm(exact["d_seg"]))
+            out[key]["d_table_running_sum_rel_to_exact"] = float(
+                np.abs(np.cumsum(r["d_table"]) - np.cumsum(exact["d_table"])).max() / np.abs(exact["d_table"]).sum())
+    ctx.set_adjoint_roulette(0.0)
     return out
 
 
@@ -273,8 +289,10 @@ def main(argv=None):
         out["adjoint"] = {"forward_ms": fwd_ms, **adj,
                           "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
                           "ratio_table_only": adj["table_only"]["fwd_adjoint_ms"] / fwd_ms,
+                          "ratio_full_roulette": adj["full_roulette"]["fwd_adjoint_ms"] / fwd_ms,
                           "gradients": "full: dE/dT (4096) + dE/d(initial segments) -> sigma, exit angles; "
-                                       "table_only: dE/dT (-> sigma)"}
+                                       "table_only: dE/dT (-> sigma); full_roulette: full, paths kept with "
+                                       "probability min(1, omega n / kappa) and reweighted (unbiased)"}
     if not a.no_batched:
         out["batched"] = measure_batched(ctx, capi)
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)

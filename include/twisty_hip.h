@@ -119,6 +119,21 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
+/* Russian roulette in front of the following twisty_hip_run_adjoint calls:
+ * kappa > 0 keeps path p with probability q_p = min(1, omega_p * numPaths /
+ * kappa) (omega_p = w_p / sum w from the forward pass; draw: Philox4x32-10,
+ * counter (0xA0000000, 0, path lo, path hi), key = seed) and scales its
+ * contribution by 1 / q_p, so the gradient keeps its expectation while about
+ * numPaths / kappa + (paths with omega_p >= kappa / numPaths) paths are swept.
+ * The added variance is at most kappa / numPaths times the omega-weighted mean
+ * square of the per-path gradients.  kappa = 0 (the default) visits every path
+ * and gives the exact gradient of the paths' estimate.  No counterpart in the
+ * reference, which has no adjoint. */
+int twisty_hip_set_adjoint_roulette(twisty_hip_ctx *ctx, float kappa);
+
+/* Paths the last twisty_hip_run_adjoint visited (numPaths without roulette). */
+int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited);
+
 /* Independent-path estimator: ExperimentBase::MSegmentPathGenerationMC
  * (Experiments/ExperimentBase.cpp:1246-1386).  Each path is built from scratch
  * between the hinge points of the curve's boundary (set_curve) by recursive

@@ -22,7 +22,7 @@ EXPORTED = [
     "twisty_hip_set_schedule", "twisty_hip_get_schedule", "twisty_hip_last_timing",
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
     "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions", "twisty_hip_set_burn_in",
-    "twisty_hip_run_generated",
+    "twisty_hip_run_generated", "twisty_hip_set_adjoint_roulette", "twisty_hip_last_adjoint_paths",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -108,6 +108,8 @@ def _declare(L):
     L.twisty_hip_set_weight_histogram.argtypes = [vp, u32, C.c_double, C.c_double, C.c_double]
     L.twisty_hip_set_keep_final_state.argtypes = [vp, C.c_int]
     L.twisty_hip_set_burn_in.argtypes = [vp, u32]
+    L.twisty_hip_set_adjoint_roulette.argtypes = [vp, f32]
+    L.twisty_hip_last_adjoint_paths.argtypes = [vp, C.POINTER(u64)]
     L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
@@ -201,6 +203,15 @@ class Context:
         if per_joint:
             res["curv"] = pj.reshape(num_paths, self.N - 1)
         return res
+
+    def set_adjoint_roulette(self, kappa):
+        """Russian roulette over paths before the adjoint (0 = exact, every path)."""
+        self._check(lib().twisty_hip_set_adjoint_roulette(self._h, kappa))
+
+    def last_adjoint_paths(self):
+        v = C.c_uint64()
+        self._check(lib().twisty_hip_last_adjoint_paths(self._h, C.byref(v)))
+        return v.value
 
     def run_generated(self, num_paths, seed=1, path_offset=0, per_path=False):
         """Independent-path estimator (paths built by recursive sphere intersection)."""

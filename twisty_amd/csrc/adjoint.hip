@@ -204,13 +204,16 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
     for (int w = 0; w < WMAX; ++w) scrB[w] = ldsBase + (scr - (uint32_t)w * SLOT) * 4u;
     const float M = (float)p.fin->maxLog2;
     const bool finiteM = M > -INFINITY && M < INFINITY;
+    const uint64_t numVisits = p.pick ? (uint64_t)*p.pickCount : p.numPaths;
 
-    for (uint64_t local = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; local < p.numPaths;
-          local += (uint64_t)gridDim.x * groupsPerBlock) {
+    for (uint64_t visit = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; visit < numVisits;
+          visit += (uint64_t)gridDim.x * groupsPerBlock) {
+        const uint64_t local = p.pick ? p.pick[visit].local : visit;
         const float2 lw = p.pathLog2[local];
         // Scaled weight 2^(log2 w - M) in (0, 1]; invalid (NaN) and zero-weight
         // paths, and those below FP32 range relative to the largest, add nothing.
-        const float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
+        float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
+        if (p.pick) wp *= p.pick[visit].factor;
         if (!(wp > 0.0f)) continue;  // uniform across the path's group
         const uint64_t path = p.pathOffset + local;
         for (uint32_t s = 0; s < S; ++s) {
@@ -319,6 +322,52 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
     }
 }
 
+__global__ void __launch_bounds__(256) adjoint_pick_kernel(const float2 *pathLog2, const FinalOut *fin,
+      uint64_t numPaths, uint64_t pathOffset, uint64_t seed, float kappa, AdjPick *pick, uint32_t *count)
+{
+    const float M = (float)fin->maxLog2;
+    const bool finiteM = M > -INFINITY && M < INFINITY;
+    // the loop condition is uniform across the block, so every lane of a wave
+    // reaches the ballot in every iteration
+    for (uint64_t first = (uint64_t)blockIdx.x * 256u; first < numPaths; first += (uint64_t)gridDim.x * 256u) {
+        const uint64_t local = first + threadIdx.x;
+        bool keep = false;
+        float factor = 0.0f;
+        if (local < numPaths) {
+            const float2 lw = pathLog2[local];
+            const float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
+            if (wp > 0.0f) {
+                const double q = (double)wp * (double)numPaths / ((double)kappa * fin->sumScaled);
+                if (q >= 1.0) {
+                    keep = true;
+                    factor = 1.0f;
+                } else {
+                    const uint64_t path = pathOffset + local;
+                    const U4 r = philox4x32_10(0xA0000000u, 0u, (uint32_t)path, (uint32_t)(path >> 32), (uint32_t)seed,
+                          (uint32_t)(seed >> 32));
+                    const double u = (double)(r.x >> 8) * 0x1p-24;
+                    if (u < q) {
+                        keep = true;
+                        factor = (float)(1.0 / q);
+                    }
+                }
+            }
+        }
+        // one counter update per wave; every lane of the wave reaches the ballot
+        const uint64_t mask = __ballot(keep);
+        const uint32_t wl = threadIdx.x & 63u;
+        uint32_t base = 0;
+        if (wl == 0 && mask) base = atomicAdd(count, (uint32_t)__popcll(mask));
+        base = __shfl(base, 0, 64);
+        if (keep) {
+            const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << wl) - 1ull));
+            pick[slot].local = local;
+            pick[slot].factor = factor;
+            pick[slot].pad = 0u;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) grad_reduce_kernel(const float *partial, uint32_t numBlocks,
       uint32_t numEntries, const FinalOut *fin, double *out)
 {
@@ -364,6 +413,15 @@ hipError_t launch_adjoint(const AdjParams &p, uint32_t G, bool sweep, uint32_t g
     TWISTY_ADJ_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_adjoint_pick(const float2 *pathLog2, const FinalOut *fin, uint64_t numPaths, uint64_t pathOffset,
+      uint64_t seed, float kappa, AdjPick *pick, uint32_t *count, hipStream_t s)
+{
+    const uint64_t blocks = std::min<uint64_t>((numPaths + 255) / 256, 65536);
+    hipLaunchKernelGGL(adjoint_pick_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, pathLog2, fin, numPaths, pathOffset,
+          seed, kappa, pick, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_grad_reduce(const float *partial, uint32_t numBlocks, uint32_t numEntries, const FinalOut *fin,

@@ -9,6 +9,13 @@
 
 namespace twisty_hip {
 
+// A path the roulette keeps, and the factor its contribution is scaled by.
+struct AdjPick {
+    uint64_t local;
+    float factor;
+    uint32_t pad;
+};
+
 struct AdjParams {
     const float *finalSeg;    // forward's final segments: (3i + c)*numPaths + local
     const float2 *pathLog2;   // forward's per-path (hi, lo) log2 weights
@@ -26,6 +33,8 @@ struct AdjParams {
     uint32_t tableStride;
     uint32_t segStride;   // floats reserved for the 3N segment gradient (multiple of 64)
     float minK, maxK, step, invDs, invStep;
+    const AdjPick *pick;         // nullable: visit only these paths (roulette)
+    const uint32_t *pickCount;   // their number (written by adjoint_pick_kernel)
 };
 
 // Dynamic LDS bytes: segment rows (and adjoint rows for the sweep), table,
@@ -42,6 +51,13 @@ inline uint32_t adjoint_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32
 int adjoint_block_sizes(uint32_t G, bool sweep, uint32_t *out, int cap);
 hipError_t launch_adjoint(const AdjParams &p, uint32_t G, bool sweep, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s);
+// Russian roulette over the paths before the adjoint: a path of scaled weight
+// w (omega = w / S) is kept with probability q = min(1, omega * numPaths / kappa)
+// and its contribution scaled by 1 / q; kept paths are appended to pick
+// (*count reset by the caller).  The draw is Philox4x32-10 with counter
+// (0xA0000000, 0, path lo, path hi), key = seed.
+hipError_t launch_adjoint_pick(const float2 *pathLog2, const FinalOut *fin, uint64_t numPaths, uint64_t pathOffset,
+      uint64_t seed, float kappa, AdjPick *pick, uint32_t *count, hipStream_t s);
 // out[e] = (sum over blocks of partial[block][e]) / (S ln 10), compensated FP32
 // sums over blocks, FP64 scale.
 hipError_t launch_grad_reduce(const float *partial, uint32_t numBlocks, uint32_t numEntries, const FinalOut *fin,

@@ -328,6 +328,8 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dHist);
     (void)hipFree(ctx->dFinalSeg);
     (void)hipFree(ctx->dGenMin);
+    (void)hipFree(ctx->dPick);
+    (void)hipFree(ctx->dPickCount);
     (void)hipFree(ctx->dGradTPartial);
     (void)hipFree(ctx->dGradSPartial);
     (void)hipFree(ctx->dGradT);
@@ -507,6 +509,22 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastFinalPaths = ctx->keepFinalState ? numPaths : 0;
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_adjoint_roulette(twisty_hip_ctx *ctx, float kappa)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!(kappa >= 0.0f) || !std::isfinite(kappa))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_roulette: kappa must be finite and >= 0");
+    ctx->adjKappa = kappa;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited)
+{
+    if (!ctx || !pathsVisited) return TWISTY_HIP_ERR_ARG;
+    *pathsVisited = ctx->lastPicked;
     return TWISTY_HIP_OK;
 }
 
@@ -749,7 +767,20 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
             return rc;
         if ((rc = ensure_dev(ctx, ctx->dGradS, ctx->gradSCap, 3ull * N, "segment gradient"))) return rc;
     }
+    const bool roulette = ctx->adjKappa > 0.0f;
+    if (roulette) {
+        if (numPaths > 0xFFFFFFFFull)
+            return fail(ctx, TWISTY_HIP_ERR_ARG, "run_adjoint: the roulette handles at most 2^32 - 1 paths");
+        if ((rc = ensure_dev(ctx, ctx->dPick, ctx->pickCap, numPaths, "roulette list"))) return rc;
+        if ((rc = ensure_dev(ctx, ctx->dPickCount, ctx->pickCountCap, 1, "roulette count"))) return rc;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->dPickCount, 0, sizeof(uint32_t), ctx->stream));
+        const hipError_t pe = launch_adjoint_pick(ctx->dPathLog2, ctx->dFinal, numPaths, pathOffset, seed,
+              ctx->adjKappa, ctx->dPick, ctx->dPickCount, ctx->stream);
+        if (pe != hipSuccess) return hipfail(ctx, pe, "roulette kernel launch");
+    }
     AdjParams a {};
+    a.pick = roulette ? ctx->dPick : nullptr;
+    a.pickCount = roulette ? ctx->dPickCount : nullptr;
     a.finalSeg = ctx->dFinalSeg;
     a.pathLog2 = ctx->dPathLog2;
     a.fin = ctx->dFinal;
@@ -789,7 +820,11 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     if (sweep)
         HIP_TRY(ctx, hipMemcpyAsync(dLog10EstimateDSegments, ctx->dGradS, 3ull * N * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t picked = 0;
+    if (roulette)
+        HIP_TRY(ctx, hipMemcpyAsync(&picked, ctx->dPickCount, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->lastPicked = roulette ? picked : numPaths;
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev3));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastAdjMs, ctx->ev2, ctx->ev3));

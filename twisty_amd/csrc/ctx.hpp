@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/twisty_hip.h"
+#include "adjoint.hpp"
 #include "forward.hpp"
 
 using namespace twisty_hip;
@@ -78,6 +79,12 @@ struct twisty_hip_ctx {
     // Adjoint.
     float *dFinalSeg = nullptr;
     float *dGenMin = nullptr;  // min log10 weight of the independent-path run
+    float adjKappa = 0.0f;     // adjoint roulette: 0 = every path
+    AdjPick *dPick = nullptr;
+    uint64_t pickCap = 0;
+    uint32_t *dPickCount = nullptr;
+    uint64_t pickCountCap = 0;
+    uint64_t lastPicked = 0;   // paths the last adjoint visited
     uint64_t genMinCap = 0;
     uint64_t finalSegCap = 0;
     float *dGradTPartial = nullptr;

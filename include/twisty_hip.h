@@ -151,6 +151,24 @@ int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited);
 int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
       twisty_hip_partials *out, double *minLog10W, float *perPathLog10);
 
+/* Multi-ds tables for twisty_hip_run_generated_multi: numTables tables of
+ * numBins floats, tableMeta 4 floats per table (ds, minCurvature,
+ * maxCurvature, curvatureStepSize), ds strictly increasing on a uniform grid
+ * (CachedMultiArclengthWeightLookupTable, PathWeightUtils.cpp:123-278), the
+ * absorption coefficient and the segment count the tables will weigh. */
+int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
+      const float *tableMeta, float absorption, uint32_t numSegments);
+
+/* The multi-arclength form of the independent-path estimator
+ * (Experiments/ExperimentBase.cpp:1388-1597): every path draws its own
+ * ds = u (maxDs - minDs) + minDs (u from node 0's Philox words), is built with
+ * that ds and weighted with the cached table nearest to it (first on ties,
+ * PathWeightUtils.cpp:223-236), its curvature taken with arclength ds N and its
+ * absorption with the table's ds.  Otherwise as twisty_hip_run_generated (the
+ * boundary comes from set_curve; set_tables is not needed). */
+int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10);
+
 /* Burn-in of the following twisty_hip_run calls: every chain first applies
  * pathsPerChain unweighted paths of K perturbations (draws keyed by path id
  * 2^63 | (chain * pathsPerChain + b), chain = absolute path id / chainLength),

@@ -118,6 +118,15 @@ typedef struct twisty_host_generation_result {
 int twisty_host_path_generation_mc(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
       twisty_host_generation_result *out, char *errorOut, uint32_t errorLen);
 
+/* The multi-arclength overload (Experiments/ExperimentBase.cpp:1388-1597):
+ * cached tables for numDsSteps ds values on [minDsTables, maxDsTables] (the
+ * .cwt cache of twisty_host_cached_rt_tables under cacheRoot), ds drawn per
+ * path on [minDs, maxDs] with minDs the quadratic minimum arclength over N
+ * (:1437-1462); num_total = 0 when maxDs < minDs, as the reference returns. */
+int twisty_host_path_generation_mc_multi(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
+      float minDsTables, float maxDsTables, uint32_t numDsSteps, float maxDs, const char *cacheRoot,
+      twisty_host_generation_result *out, char *errorOut, uint32_t errorLen);
+
 /* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
  * config given as text: parse experiment.experimentParams and
  * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.

@@ -24,10 +24,12 @@
 // extent) marks the path invalid instead; the result keeps the reference's
 // sum over valid paths and their count, min and max.
 //
+// The multi-arclength overload (:1388-1597) is oracle_generate_multi below.
+//
 // Pinned by (tests/test_oracle.py): exact segment lengths and hinge points of
 // every generated path, the closed forms of the straight-line case, the
-// exhaustive segment-count rule, and the draw-distribution checks of the
-// two-segment step.
+// exhaustive segment-count rule, the drawn theta and phi recovered from the
+// geometry, and (multi-ds) the table choice read back from flat tables.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -284,6 +286,95 @@ int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t
                     }
                 PathWeightValue pw { false, NAN };
                 if (ok) pw = WeightState(p, ps, wt, absorption, tan, curvT, curvF);
+                if (log10wOut) log10wOut[li] = pw.isValid ? pw.weight : NAN;
+                if (validOut) validOut[li] = pw.isValid ? 1 : 0;
+                if (!pw.isValid) {
+                    ++a.ni;
+                    continue;
+                }
+                ++a.nv;
+                a.sum = LogSumExp10(a.sum, (long double)pw.weight);
+                a.mn = std::min(a.mn, pw.weight);
+                a.mx = std::max(a.mx, pw.weight);
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+    Acc tot;
+    for (const Acc &a : accs) {
+        tot.sum = LogSumExp10(tot.sum, a.sum);
+        tot.nv += a.nv;
+        tot.ni += a.ni;
+        tot.mn = std::min(tot.mn, a.mn);
+        tot.mx = std::max(tot.mx, a.mx);
+    }
+    result[0] = (double)tot.sum;
+    result[1] = (double)tot.nv;
+    result[2] = (double)tot.ni;
+    result[3] = tot.mn;
+    result[4] = tot.mx;
+    result[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// The multi-arclength form (ExperimentBase.cpp:1388-1597): per path ds =
+// u (maxDs - minDs) + minDs in float (u from node 0's words), the table whose
+// ds is nearest (first on ties, PathWeightUtils.cpp:223-236), curvature with
+// arclength ds N, absorption with the table's ds.  tables: T x bins floats;
+// meta: 4 per table (ds, minK, maxK, step).  Outputs as oracle_generate.
+int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint64_t pathOffset, uint64_t nPaths,
+      const float *tables, int32_t bins, const float *meta, uint32_t numTables, float absorption, float minDs,
+      float maxDs, int threads, double *log10wOut, uint8_t *validOut, double *result)
+{
+    uint32_t leafSpan = 0;
+    const int levels = oracle_generate_levels(N, &leafSpan);
+    if (levels < 0 || bins < 2 || numTables < 1 || !(minDs > 0.0f) || !(maxDs >= minDs)) return 1;
+    std::vector<WeightTable> wts(numTables);
+    for (uint32_t t = 0; t < numTables; ++t) {
+        wts[t].table.assign(tables + (uint64_t)t * bins, tables + (uint64_t)(t + 1) * bins);
+        wts[t].ds = meta[4 * t];
+        wts[t].minCurvature = meta[4 * t + 1];
+        wts[t].maxCurvature = meta[4 * t + 2];
+        wts[t].curvatureStepSize = meta[4 * t + 3];
+    }
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > nPaths) threads = (int)std::max<uint64_t>(1, nPaths);
+    struct Acc {
+        long double sum = -std::numeric_limits<long double>::infinity();
+        uint64_t nv = 0, ni = 0;
+        double mn = INFINITY, mx = -INFINITY;
+    };
+    std::vector<Acc> accs(threads);
+    std::vector<std::thread> pool;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int th = 0; th < threads; ++th) {
+        const uint64_t b = nPaths * (uint64_t)th / (uint64_t)threads;
+        const uint64_t e = nPaths * (uint64_t)(th + 1) / (uint64_t)threads;
+        pool.emplace_back([&, th, b, e] {
+            std::vector<V> p;
+            std::vector<V> tan;
+            std::vector<double> curvT;
+            std::vector<float> curvF;
+            Acc &a = accs[th];
+            for (uint64_t li = b; li < e; ++li) {
+                const uint64_t path = pathOffset + li;
+                const float u = (float)gen_words(seed, path, 0, 0).u0;
+                const float ds = std::fma(u, maxDs - minDs, minDs);
+                uint32_t best = 0;
+                float bestDist = std::numeric_limits<float>::max();
+                for (uint32_t t = 0; t < numTables; ++t) {
+                    const float dist = std::fabs(wts[t].ds - ds);
+                    if (dist < bestDist) {
+                        bestDist = dist;
+                        best = t;
+                    }
+                }
+                PathSpec ps;
+                ps.N = N;
+                ps.arclength = ds * (float)N;
+                const bool ok = GeneratePath(bc12, (double)ds, N, levels, leafSpan, seed, path, p);
+                PathWeightValue pw { false, NAN };
+                if (ok) pw = WeightState(p, ps, wts[best], absorption, tan, curvT, curvF);
                 if (log10wOut) log10wOut[li] = pw.isValid ? pw.weight : NAN;
                 if (validOut) validOut[li] = pw.isValid ? 1 : 0;
                 if (!pw.isValid) {

@@ -82,6 +82,9 @@ def _declare(L):
     L.oracle_generate.argtypes = [_p(np.float64), f32, u32, u64, u64, u64, _p(np.float32), i32, f32, f32, f32, f32,
                                   f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _p(np.float64)]
     L.oracle_generate.restype = C.c_int
+    L.oracle_generate_multi.argtypes = [_p(np.float64), u32, u64, u64, u64, _p(np.float32), i32, _p(np.float32), u32,
+                                        f32, f32, f32, C.c_int, C.c_void_p, C.c_void_p, _p(np.float64)]
+    L.oracle_generate_multi.restype = C.c_int
 
 
 # ---------------------------------------------------------------------------
@@ -240,6 +243,23 @@ def generate(bc, arclength, N, n_paths, table, absorption, seed=1, path_offset=0
     if want_pos:
         out["positions"] = pos.reshape(n_paths, N + 1, 3)
     return out
+
+
+def generate_multi(bc, N, n_paths, tables, meta, absorption, min_ds, max_ds, seed=1, path_offset=0, threads=None):
+    """Multi-arclength independent-path estimator (oracle_generate.cpp): tables (T, bins), meta (T, 4)."""
+    threads = threads or os.cpu_count() or 1
+    t = np.ascontiguousarray(tables, np.float32)
+    m = np.ascontiguousarray(meta, np.float32)
+    res = np.zeros(6, np.float64)
+    w = np.zeros(n_paths, np.float64)
+    v = np.zeros(n_paths, np.uint8)
+    rc = lib().oracle_generate_multi(np.ascontiguousarray(bc, np.float64), N, seed, path_offset, n_paths, t,
+                                     t.shape[1], m, t.shape[0], absorption, min_ds, max_ds, threads,
+                                     w.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), res)
+    if rc != 0:
+        raise ValueError("oracle_generate_multi rejected its arguments")
+    return {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
+            "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool)}
 
 
 def exit_bins(tangents, e, n_theta, n_phi):

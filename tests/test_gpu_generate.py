@@ -132,3 +132,63 @@ def test_host_path_generation_mc(hip_device, oracle):
     assert r["min_log10_w"] == pytest.approx(w.min() + 2.5, rel=1e-6)
     assert r["max_log10_w"] == pytest.approx(w.max() + 2.5, rel=1e-6)
     ctx.close()
+
+
+def test_generated_multi_ds_matches_oracle(hip_device, oracle):
+    """Multi-ds form against the oracle: per-path ds, nearest cached table
+    (CachedMultiArclengthWeightLookupTable from the host), curvature with
+    arclength ds N, the table's own absorption."""
+    N = 34
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
+    bc13[9:12] /= np.linalg.norm(bc13[9:12])
+    tabs, _, _ = host.cached_rt_tables(0.035, 0.08, 12, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
+                                       numStepsInt=2000, numCurvatureSteps=1024)
+    tables = np.stack([t.table for t in tabs])
+    meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+    ctx = capi.Context(0)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    ctx.set_multi_tables(tables, meta, 0.004, N)
+    lo, hi = 0.045, 0.075
+    n = 4096
+    g = ctx.run_generated_multi(n, lo, hi, seed=5, per_path=True)
+    ref = oracle.generate_multi(bc13[:12].astype(np.float64), N, n, tables, meta, 0.004, lo, hi, seed=5)
+    gw, rw = g["log10w"].astype(np.float64), ref["log10w"]
+    both = ~np.isnan(gw) & ref["valid"]
+    rel = np.abs(gw[both] - rw[both]) / np.abs(rw[both])
+    print(f"[generated multi-ds N={N}] valid gpu {g['num_valid']} oracle {ref['n_valid']}, median rel "
+          f"{np.median(rel):.2e}, 99% {np.quantile(rel, 0.99):.2e}", flush=True)
+    assert abs(g["num_valid"] - ref["n_valid"]) <= max(2, n // 1000)
+    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.99) <= 1e-5
+    assert abs(g["log10_sum_w"] - ref["log10_sum_w"]) < 1e-4 * abs(ref["log10_sum_w"])
+    ctx.close()
+
+
+def test_host_path_generation_mc_multi(hip_device, tmp_path):
+    """The C++ multi-arclength overload on a config equals the device call it makes."""
+    N, L = 34, 2.0
+    cfg = {"experiment": {
+        "experimentParams": {
+            "pathsToGenerate": 20000, "pathsToSkip": 0, "numSegments": N, "name": "gen", "experimentDir": "/tmp",
+            "maxPerturbThreads": 0, "maxWeightThreads": 0, "outputBigFloatWeights": False,
+            "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
+            "random": {"bootstrapSeed": 1, "perturbSeed": 1},
+            "weighting": {"weightFunction": 0, "mu": 1.0, "eps": float(np.float32(0.08) ** 2), "numStepsInt": 2000,
+                          "numCurvatureSteps": 1024, "absorption": 0.004, "scatter": 0.1}},
+        "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1], "endDir": [0, 0, 1],
+                            "arclength": L}}}
+    r = host.path_generation_mc_multi(cfg, 0.03, 0.08, 11, 0.07, seed=3, cache_root=tmp_path)
+    tabs, loaded, _ = host.cached_rt_tables(0.03, 0.08, 11, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
+                                            numStepsInt=2000, numCurvatureSteps=1024, cache_root=tmp_path)
+    assert loaded
+    min_ds = float(np.float32(np.float32(1.0) / np.float32(N)))  # the quadratic's root is the end distance here
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 0.07 * N)
+    ctx = capi.Context(0)
+    ctx.set_curve(np.stack([np.zeros(N + 1), np.zeros(N + 1), np.linspace(0, 1, N + 1)], 1).astype(np.float32), bc13)
+    ctx.set_multi_tables(np.stack([t.table for t in tabs]), np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs],
+                                                                     np.float32), 0.004, N)
+    g = ctx.run_generated_multi(20000, min_ds, 0.07, seed=3)
+    ctx.close()
+    want = g["log10_sum_w"] - math.log10(g["num_valid"])
+    print(f"[path_generation_mc_multi] {r}", flush=True)
+    assert r["num_total"] == 20000 and r["num_valid"] == g["num_valid"]
+    assert abs(r["log10_total_weight"] - want) < 1e-5 * abs(want)

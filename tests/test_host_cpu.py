@@ -363,3 +363,18 @@ def test_path_generation_mc_without_gpu():
     cfg = _config(numSegments=26)
     with pytest.raises(RuntimeError, match="twisty_hip_create"):
         host.path_generation_mc(cfg)
+
+
+def test_path_generation_mc_multi_minimum_ds(tmp_path):
+    """Multi-arclength overload: its minimum arclength (the quadratic of
+    ExperimentBase.cpp:1437-1462) is the end distance for a straight geometry,
+    so maxDs just below D / N gives the reference's empty result
+    (:1466-1469) and just above reaches the device."""
+    N, D = 26, 1.3
+    cfg = _config(numSegments=N, pathsToGenerate=1000)
+    cfg["experiment"]["basicExperiment"]["endPos"] = [0, 0, D]
+    empty = host.path_generation_mc_multi(cfg, 0.03, 0.07, 5, D / N * 0.999, cache_root=tmp_path)
+    assert empty["num_total"] == 0 and empty["num_valid"] == 0
+    if capi.device_count() == 0:
+        with pytest.raises(RuntimeError, match="twisty_hip_create"):
+            host.path_generation_mc_multi(cfg, 0.03, 0.07, 5, D / N * 1.001, cache_root=tmp_path)

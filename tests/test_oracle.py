@@ -541,3 +541,28 @@ def test_generated_straight_line_closed_form(oracle):
     assert np.abs(P[:, :, 2] - np.linspace(0, 1, N + 1)).max() < 1e-7
     want = (N - 1) * math.log10(float(t.table[0]) * float(np.float32(math.exp(float(np.float32(-0.004 * np.float32(ds)))))))
     assert np.allclose(r["log10w"], want, rtol=1e-7)
+
+
+def test_generated_multi_ds_table_choice(oracle):
+    """Multi-ds form (ExperimentBase.cpp:1388-1597): with flat tables of
+    distinct constants the path weight names the table, which must be the one
+    whose ds is nearest to u (maxDs - minDs) + minDs in float (u from node 0's
+    words), first on ties (PathWeightUtils.cpp:223-236)."""
+    N, T, bins = 10, 7, 64
+    dss = np.linspace(0.09, 0.15, T).astype(np.float32)
+    consts = np.float32(1.0) + np.arange(T, dtype=np.float32)
+    tables = np.repeat(consts[:, None], bins, axis=1)
+    meta = np.stack([dss, np.zeros(T, np.float32), np.full(T, 100.0, np.float32), np.full(T, 100.0 / (bins - 1),
+                                                                                        np.float32)], axis=1)
+    lo, hi = np.float32(0.1), np.float32(0.14)
+    r = oracle.generate_multi(oracle.bc12(), N, 200, tables, meta, 0.004, float(lo), float(hi), seed=4)
+    assert r["valid"].all()
+    for p in range(200):
+        u = np.float32(_words(oracle, 4, p, 0)[0])
+        # fmaf(u, hi - lo, lo): the product of two floats is exact in double
+        ds = np.float32(float(u) * float(hi - lo) + float(lo))
+        d = np.abs(dss - ds)
+        t = int(np.argmin(d))
+        absorb = float(np.float32(math.exp(float(np.float32(np.float32(-0.004) * dss[t])))))
+        want = (N - 1) * math.log10(float(consts[t]) * absorb)
+        assert r["log10w"][p] == pytest.approx(want, rel=1e-6), p  # float lerp and product per joint

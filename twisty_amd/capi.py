@@ -23,6 +23,7 @@ EXPORTED = [
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
     "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions", "twisty_hip_set_burn_in",
     "twisty_hip_run_generated", "twisty_hip_set_adjoint_roulette", "twisty_hip_last_adjoint_paths",
+    "twisty_hip_set_multi_tables", "twisty_hip_run_generated_multi",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -110,6 +111,8 @@ def _declare(L):
     L.twisty_hip_set_burn_in.argtypes = [vp, u32]
     L.twisty_hip_set_adjoint_roulette.argtypes = [vp, f32]
     L.twisty_hip_last_adjoint_paths.argtypes = [vp, C.POINTER(u64)]
+    L.twisty_hip_set_multi_tables.argtypes = [vp, vp, u32, u32, vp, f32, u32]
+    L.twisty_hip_run_generated_multi.argtypes = [vp, u64, u64, u64, f32, f32, vp, C.POINTER(C.c_double), vp]
     L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
@@ -220,6 +223,26 @@ class Context:
         pp = np.empty(num_paths, np.float32) if per_path else None
         self._check(lib().twisty_hip_run_generated(self._h, num_paths, path_offset, seed, C.byref(out), C.byref(mn),
                                                    _ptr(pp)))
+        res = out.as_dict()
+        res["min_log10_w"] = mn.value
+        if per_path:
+            res["log10w"] = pp
+        return res
+
+    def set_multi_tables(self, tables, meta, absorption, num_segments):
+        """Multi-ds tables: tables (T, bins), meta (T, 4) = ds, minK, maxK, step."""
+        t = np.ascontiguousarray(tables, np.float32)
+        m = np.ascontiguousarray(meta, np.float32)
+        self._check(lib().twisty_hip_set_multi_tables(self._h, _ptr(t), t.shape[1], t.shape[0], _ptr(m), absorption,
+                                                      num_segments))
+
+    def run_generated_multi(self, num_paths, min_ds, max_ds, seed=1, path_offset=0, per_path=False):
+        """Independent paths with ds drawn per path and the nearest cached table."""
+        out = Partials()
+        mn = C.c_double()
+        pp = np.empty(num_paths, np.float32) if per_path else None
+        self._check(lib().twisty_hip_run_generated_multi(self._h, num_paths, path_offset, seed, min_ds, max_ds,
+                                                         C.byref(out), C.byref(mn), _ptr(pp)))
         res = out.as_dict()
         res["min_log10_w"] = mn.value
         if per_path:

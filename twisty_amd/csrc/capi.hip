@@ -329,6 +329,8 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dFinalSeg);
     (void)hipFree(ctx->dGenMin);
     (void)hipFree(ctx->dPick);
+    (void)hipFree(ctx->dMultiTables);
+    (void)hipFree(ctx->dMultiMeta);
     (void)hipFree(ctx->dPickCount);
     (void)hipFree(ctx->dGradTPartial);
     (void)hipFree(ctx->dGradSPartial);
@@ -528,13 +530,16 @@ int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited)
     return TWISTY_HIP_OK;
 }
 
-int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      bool multi, float dsLo, float dsHi, twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
 {
-    if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: out is NULL");
-    if (!ctx->haveTables || !ctx->haveCurve)
-        return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_tables and set_curve first");
+    if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_curve first");
+    if (!multi && !ctx->haveTables) return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_tables first");
+    if (multi && ctx->multiCount == 0)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated_multi: set_multi_tables first");
+    if (multi && !(dsLo > 0.0f && dsHi >= dsLo && std::isfinite(dsHi)))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_multi: need 0 < minDs <= maxDs");
     if (numPaths == 0 || numPaths > (1ull << 40))
         return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: need 1 <= numPaths <= 2^40");
     const uint32_t N = ctx->N;
@@ -548,7 +553,8 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
     const uint32_t rows = (N + 1 + G - 1) / G;
     if (rows > 16) return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "run_generated: numSegments above 1022");
     const uint32_t B = 256;
-    const uint32_t lds = generate_lds_bytes(G, B, N, table_stride(ctx->tableSize));
+    const uint32_t bins = multi ? ctx->multiBins : ctx->tableSize;
+    const uint32_t lds = generate_lds_bytes(G, B, N, multi ? 0u : table_stride(bins));
     if (lds > ctx->ldsPerBlock)
         return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
               "run_generated: needs " + std::to_string(lds) + " LDS bytes per block, limit " +
@@ -562,8 +568,9 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
     if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) return rc;
     if (perPathLog10 && (rc = ensure_dev(ctx, ctx->dPerPath, ctx->perPathCap, numPaths, "per-path buffer"))) return rc;
     if ((rc = ensure_dev(ctx, ctx->dGenMin, ctx->genMinCap, 1, "min buffer"))) return rc;
-    const double absorb = (double)std::exp((float)(-ctx->absorption * ctx->ds));
-    const double absorbLog2PerPath = (double)(N - 1) * std::log2(absorb);
+    // the multi-ds form adds each path's own table absorption in the kernel
+    const double absorbLog2PerPath =
+          multi ? 0.0 : (double)(N - 1) * std::log2((double)std::exp((float)(-ctx->absorption * ctx->ds)));
     GenParams p {};
     p.table = ctx->dTable;
     p.partials = ctx->dPartials;
@@ -584,8 +591,15 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
     p.levels = (uint32_t)levels;
     p.leafSpan = leafSpan;
     p.rows = rows;
-    p.tableSize = (int32_t)ctx->tableSize;
-    p.tableStride = table_stride(ctx->tableSize);
+    p.tableSize = (int32_t)bins;
+    p.tableStride = table_stride(bins);
+    if (multi) {
+        p.multiTables = ctx->dMultiTables;
+        p.multiMeta = ctx->dMultiMeta;
+        p.numTables = ctx->multiCount;
+        p.dsLo = dsLo;
+        p.dsHi = dsHi;
+    }
     p.minK = ctx->minK;
     p.maxK = ctx->maxK;
     p.step = ctx->step;
@@ -594,7 +608,7 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
     const float inf = INFINITY;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->dGenMin, &inf, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipError_t e = launch_generate(p, G, grid, B, lds, ctx->stream);
+    hipError_t e = launch_generate(p, G, multi, grid, B, lds, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "generate kernel launch");
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
@@ -619,6 +633,59 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
     if (minLog10W) *minLog10W = fo.nValid ? (double)mn : INFINITY;
     ctx->lastError.clear();
     return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, false, 0.0f, 0.0f, out, minLog10W, perPathLog10);
+}
+
+int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
+      const float *tableMeta, float absorption, uint32_t numSegments)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!tables || !tableMeta || numBins < 2 || numBins > 65536 || numTables < 1 || numTables > 4096 ||
+          numSegments < 2)
+        return fail(ctx, TWISTY_HIP_ERR_ARG,
+              "set_multi_tables: need tables, meta, 2 <= numBins <= 65536, 1 <= numTables <= 4096, numSegments >= 2");
+    const uint32_t ts = table_stride(numBins);
+    std::vector<float> data((size_t)ts * numTables, 0.0f), meta((size_t)5 * numTables);
+    for (uint32_t t = 0; t < numTables; ++t) {
+        const float *m = tableMeta + 4 * t;  // ds, minK, maxK, step
+        if (!(m[0] > 0.0f) || !(m[3] > 0.0f) || !std::isfinite(m[1]) || !std::isfinite(m[2]) || !(m[2] >= m[1]) ||
+              (t > 0 && !(m[0] > tableMeta[4 * (t - 1)])))
+            return fail(ctx, TWISTY_HIP_ERR_ARG, "set_multi_tables: table " + std::to_string(t) +
+                                                         " has a bad grid or its ds does not increase");
+        for (uint32_t b = 0; b < numBins; ++b) {
+            const float v = tables[(uint64_t)t * numBins + b];
+            if (std::isinf(v)) return fail(ctx, TWISTY_HIP_ERR_ARG, "set_multi_tables: table contains inf");
+            data[(size_t)t * ts + b] = v;
+        }
+        for (int k = 0; k < 4; ++k) meta[5 * t + k] = m[k];
+        // per path: (N - 1) joints of expf(-absorption * table ds) (PathWeighters.cpp:37)
+        meta[5 * t + 4] = (float)((double)(numSegments - 1) * std::log2((double)std::exp((float)(-absorption * m[0]))));
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_dev(ctx, ctx->dMultiTables, ctx->multiTablesCap, data.size(), "multi-ds tables"))) return rc;
+    if ((rc = ensure_dev(ctx, ctx->dMultiMeta, ctx->multiMetaCap, meta.size(), "multi-ds meta"))) return rc;
+    HIP_TRY(ctx, hipMemcpy(ctx->dMultiTables, data.data(), data.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ctx->dMultiMeta, meta.data(), meta.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->multiBins = numBins;
+    ctx->multiCount = numTables;
+    ctx->multiSegments = numSegments;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
+      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (ctx->multiCount && ctx->haveCurve && ctx->multiSegments != ctx->N)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_multi: set_multi_tables was given another numSegments");
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, true, minDs, maxDs, out, minLog10W, perPathLog10);
 }
 
 int twisty_hip_set_burn_in(twisty_hip_ctx *ctx, uint32_t pathsPerChain)

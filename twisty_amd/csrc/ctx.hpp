@@ -79,6 +79,10 @@ struct twisty_hip_ctx {
     // Adjoint.
     float *dFinalSeg = nullptr;
     float *dGenMin = nullptr;  // min log10 weight of the independent-path run
+    float *dMultiTables = nullptr;  // multi-ds tables of run_generated_multi
+    float *dMultiMeta = nullptr;
+    uint64_t multiTablesCap = 0, multiMetaCap = 0;
+    uint32_t multiBins = 0, multiCount = 0, multiSegments = 0;
     float adjKappa = 0.0f;     // adjoint roulette: 0 = every path
     AdjPick *dPick = nullptr;
     uint64_t pickCap = 0;

@@ -174,7 +174,30 @@ __device__ __forceinline__ void atomic_min_f32(float *addr, float v)
 
 }  // namespace
 
-template<int G, int BLOCK>
+// Nearest table to ds in the multi-ds set: the tables' ds are a uniform grid,
+// so the nearest index is the rounded one or a neighbour; the three are
+// compared in increasing order with a strict < (first table on ties).
+__device__ __forceinline__ uint32_t nearest_table(const GenParams &p, float ds)
+{
+    const uint32_t n = p.numTables;
+    if (n == 1) return 0;
+    const float d0 = p.multiMeta[0], d1 = p.multiMeta[5 * (n - 1)];
+    const float pos = (ds - d0) / (d1 - d0) * (float)(n - 1);
+    int i0 = (int)rintf(pos);
+    i0 = min(max(i0, 0), (int)n - 1);
+    uint32_t best = 0;
+    float bestDist = INFINITY;
+    for (int i = max(i0 - 1, 0); i <= min(i0 + 1, (int)n - 1); ++i) {
+        const float dist = fabsf(p.multiMeta[5 * i] - ds);
+        if (dist < bestDist) {
+            bestDist = dist;
+            best = (uint32_t)i;
+        }
+    }
+    return best;
+}
+
+template<int G, int BLOCK, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
 {
     constexpr uint32_t SLOT = 3u * BLOCK;
@@ -183,18 +206,20 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
     extern __shared__ float smem[];
     const uint32_t N = p.N;
     const uint32_t tabOff = p.rows * SLOT;
-    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
-    __syncthreads();
-    const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    if constexpr (!MULTI) {
+        for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
+        __syncthreads();
+    }
+    TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
     const uint32_t groupBase = lane - j;
     const uint32_t S = (N + G - 1) / G;
-    const float ds = p.ds;
+    float ds = p.ds, invDs = p.invDs, absorbLog2 = 0.0f;
     const F3 start = f3(p.start[0], p.start[1], p.start[2]);
     const F3 end = f3(p.end[0], p.end[1], p.end[2]);
-    const F3 hinge0 = start + f3(p.startDir[0], p.startDir[1], p.startDir[2]) * ds;  // :1255-1261
-    const F3 hinge1 = end - f3(p.endDir[0], p.endDir[1], p.endDir[2]) * ds;
+    const F3 startDir = f3(p.startDir[0], p.startDir[1], p.startDir[2]);
+    const F3 endDir = f3(p.endDir[0], p.endDir[1], p.endDir[2]);
     auto at = [&](uint32_t i) { return (i / G) * SLOT + groupBase + (i & (uint32_t)(G - 1)); };
     auto ld = [&](uint32_t i) {
         const uint32_t a = at(i);
@@ -215,10 +240,20 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
     for (uint64_t local = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; local < p.numPaths;
           local += (uint64_t)gridDim.x * groupsPerBlock) {
         const uint64_t path = p.pathOffset + local;
+        if constexpr (MULTI) {
+            // ds = U(0,1) (maxDs - minDs) + minDs (ExperimentBase.cpp:1486-1487) from node 0's
+            // words, the nearest cached table, curvature with arclength ds N (:1512-1518)
+            ds = fmaf(gen_words(p.seed, path, 0, 0).u0, p.dsHi - p.dsLo, p.dsLo);
+            const uint32_t t = nearest_table(p, ds);
+            const float *m = p.multiMeta + 5 * t;
+            tv = TableView { p.multiTables + (uint64_t)t * p.tableStride, p.tableSize, m[1], m[2], m[3] };
+            absorbLog2 = m[4];
+            invDs = (float)N / (ds * (float)N);
+        }
         if (j == 0) {
             st(0, start);
-            st(1, hinge0);
-            st(N - 1, hinge1);
+            st(1, start + startDir * ds);  // hinge points, ExperimentBase.cpp:1255-1261
+            st(N - 1, end - endDir * ds);
             st(N, end);
         }
         wave_sync_lds();
@@ -268,7 +303,8 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
             }
         }
         wave_sync_lds();
-        const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, p.invDs, tv, clamped);
+        PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, invDs, tv, clamped);
+        if constexpr (MULTI) pl.lsum = ff_add_f(pl.lsum, absorbLog2);  // the table's own absorption
         wave_sync_lds();  // joints read other lanes' columns before the next path's points
         if (j == 0) {
             const bool invalid = (pl.flags & 1u) || ((pl.flags & 6u) == 6u);
@@ -301,21 +337,22 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
 
 #define TWISTY_GEN_INSTANCES(X) X(2, 256) X(4, 256) X(8, 256) X(16, 256) X(32, 256) X(64, 256)
 
-template<int G, int B>
+template<int G, int B, bool M>
 static hipError_t launch_gen_gb(const GenParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    auto *fn = generate_kernel<G, B>;
+    auto *fn = generate_kernel<G, B, M>;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_generate(const GenParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_generate(const GenParams &p, uint32_t G, bool multi, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s)
 {
 #define X(g, b)                                                                                                   \
-    if (G == g && block == b) return launch_gen_gb<g, b>(p, grid, lds, s);
+    if (G == g && block == b) return multi ? launch_gen_gb<g, b, true>(p, grid, lds, s)                          \
+                                           : launch_gen_gb<g, b, false>(p, grid, lds, s);
     TWISTY_GEN_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;

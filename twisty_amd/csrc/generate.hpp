@@ -29,6 +29,13 @@ struct GenParams {
     float minK, maxK, step;
     float log10of2;
     float absorbLog10PerPath;
+    // Multi-ds form (ExperimentBase.cpp:1388-1597): ds ~ U[dsLo, dsHi) per path
+    // and the table whose ds is nearest (CachedMultiArclengthWeightLookupTable::
+    // GetWeightLookupTable, PathWeightUtils.cpp:223-236), read from HBM.
+    const float *multiTables;  // numTables x tableStride floats
+    const float *multiMeta;    // per table: ds, minK, maxK, step, absorption log2 per path
+    uint32_t numTables;
+    float dsLo, dsHi;
 };
 
 // Levels of the recursion (N - 2 halved until 2 or 3), -1 when N is not
@@ -56,6 +63,7 @@ inline uint32_t generate_lanes(uint32_t N)
     return G;
 }
 
+// tableStride 0 for the multi-ds form (its tables stay in HBM).
 inline uint32_t generate_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride)
 {
     const uint64_t rows = (N + 1 + G - 1) / G;
@@ -64,7 +72,7 @@ inline uint32_t generate_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint3
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);
 }
 
-hipError_t launch_generate(const GenParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_generate(const GenParams &p, uint32_t G, bool multi, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s);
 
 }  // namespace twisty_hip

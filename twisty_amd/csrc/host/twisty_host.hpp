@@ -371,6 +371,18 @@ std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperi
       const PerturbUtils::BoundaryConditions &experimentGeometry,
       const ExperimentRunner::ExperimentParameters &experimentParams, double pathNormalizerLog10,
       std::string &error);
+// The multi-arclength overload (ExperimentBase.cpp:1388-1597): ds drawn per path
+// on [minDs, maxDs] with minDs = the quadratic minimum arclength of :1437-1462
+// over N (an empty Result when maxDs < minDs, :1466-1469), the nearest table of
+// the cached set.
+std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperimentPaths,
+      const PerturbUtils::BoundaryConditions &experimentGeometry,
+      const ExperimentRunner::ExperimentParameters &experimentParams, double pathNormalizerLog10,
+      const PathWeighting::CachedMultiArclengthWeightLookupTable &cachedWeightLookupTable, float maxDs,
+      std::string &error);
+// ExperimentBase.cpp:1437-1462: the smaller positive root of the quadratic in
+// the arclength for M segments (1e6 when neither root is positive).
+float QuadraticMinimumArclength(const PerturbUtils::BoundaryConditions &experimentGeometry, uint32_t M);
 }  // namespace ExperimentBase
 
 // ---------------------------------------------------------------------------

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
-            "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc"]
+            "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc",
+            "twisty_host_path_generation_mc_multi"]
 
 _lib = None
 
@@ -67,6 +68,8 @@ def lib():
                                                   C.POINTER(C.c_double), C.c_char_p, u32]
         L.twisty_host_path_generation_mc.argtypes = [C.c_char_p, C.c_uint64, f64, C.POINTER(GenerationResult),
                                                      C.c_char_p, u32]
+        L.twisty_host_path_generation_mc_multi.argtypes = [C.c_char_p, C.c_uint64, f64, f32, f32, u32, f32,
+                                                           C.c_char_p, C.POINTER(GenerationResult), C.c_char_p, u32]
         L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
                                                      C.c_uint64, f64, C.c_char_p, u32]
         _lib = L
@@ -230,6 +233,19 @@ def path_generation_mc(config, seed=1, norm_log10=0.0):
     err = C.create_string_buffer(1024)
     if lib().twisty_host_path_generation_mc(text.encode(), seed, norm_log10, C.byref(res), err, 1024) != 0:
         raise RuntimeError(f"path_generation_mc failed: {err.value.decode()}")
+    return {k: getattr(res, k) for k, _ in res._fields_}
+
+
+def path_generation_mc_multi(config, min_ds_tables, max_ds_tables, num_ds_steps, max_ds, seed=1, norm_log10=0.0,
+                             cache_root=None):
+    """The multi-arclength overload: ds drawn per path, nearest cached table."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    res = GenerationResult()
+    err = C.create_string_buffer(1024)
+    root = None if cache_root is None else str(cache_root).encode()
+    if lib().twisty_host_path_generation_mc_multi(text.encode(), seed, norm_log10, min_ds_tables, max_ds_tables,
+                                                  num_ds_steps, max_ds, root, C.byref(res), err, 1024) != 0:
+        raise RuntimeError(f"path_generation_mc_multi failed: {err.value.decode()}")
     return {k: getattr(res, k) for k, _ in res._fields_}
 
 

@@ -73,29 +73,32 @@ def cpu_baseline(table, pos, w, target_seconds):
                               precision=32, threads=th, want_w=False)
         return r["seconds"]
 
-    # Sample: paths 0..n-1 of the same workload, n a power of two sized for
-    # about target_seconds of wall time (at most 4x the workload's paths).
+    # Sample: paths 0..n-1 of the same workload, n a power of two sized so that
+    # three repeats take about target_seconds (at most 4x the workload's
+    # paths); the median repeat is reported.  Threads are pinned one per core.
     n0 = 1 << 12
     s0 = run(n0, threads)
     n = n0
-    while n < 4 * w["paths"] and s0 * (n / n0) * 2 <= target_seconds * 1.5:
+    while n < 4 * w["paths"] and s0 * (n / n0) * 2 * 3 <= target_seconds * 1.5:
         n *= 2
-    secs = run(n, threads) if n != n0 else s0
+    secs = statistics.median(run(n, threads) for _ in range(3))
     jw = n * (w["N"] - 1) / secs
     out = {"value": jw, "unit": "joint-weights/s", "cores": threads, "kind": "port",
            "sample": f"{n} paths of the same workload (paths 0..{n - 1}), reference CPU path restated in oracle/ "
-                     f"(FP32 positions, FP64 weights, same Philox draws), {threads} std::threads, {secs:.1f} s"}
+                     f"(FP32 positions, FP64 weights, same Philox draws, -O3), {threads} pinned std::threads, "
+                     f"median of 3 runs {secs:.2f} s"}
     # The reference's own split: 1e6-path granules per thread, so at 2^22 paths
     # only ceil(2^22/1e6) = 5 threads work (FullExperimentRunnerOptimalPerturb.cpp:255-277).
     ref_threads = min(threads, math.ceil(w["paths"] / 1_000_000))
     n_ref = max(n0, n * ref_threads // threads)
-    secs_ref = run(n_ref, ref_threads)
+    secs_ref = statistics.median(run(n_ref, ref_threads) for _ in range(3))
     ref_part = {"value": n_ref * (w["N"] - 1) / secs_ref, "unit": "joint-weights/s", "cores": ref_threads,
-                "kind": "port", "sample": f"{n_ref} paths on {ref_threads} threads ({secs_ref:.1f} s)"}
+                "kind": "port", "sample": f"{n_ref} paths on {ref_threads} pinned threads (median of 3, "
+                                          f"{secs_ref:.2f} s)"}
     return out, ref_part
 
 
-def measure_adjoint(ctx, w, reps=3):
+def measure_adjoint(ctx, w, reps=3, roulette=True):
     """Forward + adjoint device time at the same workload (HIP events): exact
     (every path swept) and with Russian roulette (kappa = 1) in front of the
     adjoint, with the roulette gradient's deviation from the exact one."""
@@ -103,7 +106,8 @@ def measure_adjoint(ctx, w, reps=3):
 
     out = {}
     exact = None
-    for seg, key, kappa in ((False, "table_only", 0.0), (True, "full", 0.0), (True, "full_roulette", 1.0)):
+    legs = [(False, "table_only", 0.0), (True, "full", 0.0)] + ([(True, "full_roulette", 1.0)] if roulette else [])
+    for seg, key, kappa in legs:
         ctx.set_adjoint_roulette(kappa)
         r = ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
         tot, adj = [], []
@@ -230,6 +234,7 @@ def main(argv=None):
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-generated", action="store_true")
+    ap.add_argument("--no-roulette", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -284,12 +289,13 @@ def main(argv=None):
         "batched": None,
     }
     if not a.no_adjoint:
-        adj = measure_adjoint(ctx, w)
+        adj = measure_adjoint(ctx, w, roulette=not a.no_roulette)
         out["fwd_adjoint_ms"] = adj["full"]["fwd_adjoint_ms"]
         out["adjoint"] = {"forward_ms": fwd_ms, **adj,
                           "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
                           "ratio_table_only": adj["table_only"]["fwd_adjoint_ms"] / fwd_ms,
-                          "ratio_full_roulette": adj["full_roulette"]["fwd_adjoint_ms"] / fwd_ms,
+                          "ratio_full_roulette": (adj["full_roulette"]["fwd_adjoint_ms"] / fwd_ms
+                                                  if "full_roulette" in adj else None),
                           "gradients": "full: dE/dT (4096) + dE/d(initial segments) -> sigma, exit angles; "
                                        "table_only: dE/dT (-> sigma); full_roulette: full, paths kept with "
                                        "probability min(1, omega n / kappa) and reweighted (unbiased)"}

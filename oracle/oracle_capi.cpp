@@ -327,6 +327,7 @@ int oracle_simulate(int precision, const double *initialPos, uint32_t N, float a
         const uint64_t b = nPaths * (uint64_t)th / (uint64_t)threads;
         const uint64_t e = nPaths * (uint64_t)(th + 1) / (uint64_t)threads;
         pool.emplace_back([&, th, b, e] {
+            PinToAllowedCpu(th);
             if (precision == 32)
                 SimulateRange<float>(init, ps, chainLength, pathOffset, b, e, st, log10wOut, validOut, curvOut,
                       exitOut, accs[th]);

@@ -271,6 +271,7 @@ int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t
         const uint64_t b = nPaths * (uint64_t)th / (uint64_t)threads;
         const uint64_t e = nPaths * (uint64_t)(th + 1) / (uint64_t)threads;
         pool.emplace_back([&, th, b, e] {
+            PinToAllowedCpu(th);
             std::vector<V> p;
             std::vector<V> tan;
             std::vector<double> curvT;
@@ -351,6 +352,7 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
         const uint64_t b = nPaths * (uint64_t)th / (uint64_t)threads;
         const uint64_t e = nPaths * (uint64_t)(th + 1) / (uint64_t)threads;
         pool.emplace_back([&, th, b, e] {
+            PinToAllowedCpu(th);
             std::vector<V> p;
             std::vector<V> tan;
             std::vector<double> curvT;

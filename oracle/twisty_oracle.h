@@ -21,6 +21,9 @@
 // section; a function without an independent pin says "parity unpinned".
 #pragma once
 
+#include <pthread.h>
+#include <sched.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -31,6 +34,28 @@
 namespace twisty_oracle {
 
 constexpr float TwistyPi = 3.141592653589793f;  // MathConsts.h:8
+
+// Pins the calling thread to the th-th CPU this process may run on (the CPU
+// baseline runs one std::thread per allowed core, each on its own core).
+inline void PinToAllowedCpu(int th)
+{
+    cpu_set_t allowed;
+    CPU_ZERO(&allowed);
+    if (sched_getaffinity(0, sizeof(allowed), &allowed) != 0) return;
+    const int n = CPU_COUNT(&allowed);
+    if (n <= 0) return;
+    int k = th % n;
+    for (int c = 0; c < CPU_SETSIZE; ++c) {
+        if (!CPU_ISSET(c, &allowed)) continue;
+        if (k-- == 0) {
+            cpu_set_t one;
+            CPU_ZERO(&one);
+            CPU_SET(c, &one);
+            pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+            return;
+        }
+    }
+}
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------------------------

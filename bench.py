@@ -35,6 +35,11 @@ WORKLOAD = {"N": 64, "L": 1.5, "K": 100, "paths": 1 << 22, "sigma": 0.05, "bins"
             "scatter": 0.1, "absorption": 0.004, "numStepsInt": 20000, "seed": 1}
 
 
+def progress(msg):
+    """One line per finished leg, on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def mean_rotated_points(N):
     """E[diff] - 1 for diff ~ U{2..min(N-2,25)}."""
     md = min(N - 2, 25)
@@ -268,6 +273,7 @@ def main(argv=None):
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     value = w["paths"] * (N - 1) / (ms_per_step / 1e3)
     fwd_ms = statistics.fmean(fwd)
+    progress(f"forward: {a.steps} steps, {fwd_ms:.3f} ms per launch")
     F = flops_per_path(N, K)
     achieved = F * w["paths"] / (fwd_ms / 1e3) / 1e12
     mean_l, rel_se = capi.estimator(res)
@@ -290,6 +296,7 @@ def main(argv=None):
     }
     if not a.no_adjoint:
         adj = measure_adjoint(ctx, w, roulette=not a.no_roulette)
+        progress(f"adjoint: full {adj['full']['fwd_adjoint_ms']:.3f} ms")
         out["fwd_adjoint_ms"] = adj["full"]["fwd_adjoint_ms"]
         out["adjoint"] = {"forward_ms": fwd_ms, **adj,
                           "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
@@ -301,11 +308,14 @@ def main(argv=None):
                                        "probability min(1, omega n / kappa) and reweighted (unbiased)"}
     if not a.no_batched:
         out["batched"] = measure_batched(ctx, capi)
+        progress(f"batched: {out['batched']['batched_ms']:.3f} ms")
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
     if not a.no_generated:
         out["generated"] = measure_generated(capi, host, None if a.no_cpu_baseline else a.cpu_seconds / 4)
+        progress(f"independent paths: {out['generated']['kernel_ms']:.3f} ms")
     if not a.no_cpu_baseline:
         cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
+        progress(f"cpu baseline: {cb['value']:.3e} joint-weights/s on {cb['cores']} threads")
         out["cpu_baseline"] = cb
         out["cpu_baseline_reference_partitioning"] = refpart
         out["speedup_vs_cpu"] = value / cb["value"]

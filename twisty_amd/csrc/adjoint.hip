@@ -322,48 +322,70 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
     }
 }
 
+// Each block takes 4096 consecutive paths (16 per thread, coalesced), keeps
+// its paths' decisions in registers, reserves its slice of the list with one
+// atomic and writes the kept paths there.
+constexpr uint32_t kPickPerThread = 16;
+
 __global__ void __launch_bounds__(256) adjoint_pick_kernel(const float2 *pathLog2, const FinalOut *fin,
       uint64_t numPaths, uint64_t pathOffset, uint64_t seed, float kappa, AdjPick *pick, uint32_t *count)
 {
+    __shared__ uint32_t waveTotal[4];
+    __shared__ uint32_t blockBase;
     const float M = (float)fin->maxLog2;
     const bool finiteM = M > -INFINITY && M < INFINITY;
-    // the loop condition is uniform across the block, so every lane of a wave
-    // reaches the ballot in every iteration
-    for (uint64_t first = (uint64_t)blockIdx.x * 256u; first < numPaths; first += (uint64_t)gridDim.x * 256u) {
-        const uint64_t local = first + threadIdx.x;
-        bool keep = false;
-        float factor = 0.0f;
-        if (local < numPaths) {
-            const float2 lw = pathLog2[local];
-            const float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
-            if (wp > 0.0f) {
-                const double q = (double)wp * (double)numPaths / ((double)kappa * fin->sumScaled);
-                if (q >= 1.0) {
-                    keep = true;
-                    factor = 1.0f;
-                } else {
-                    const uint64_t path = pathOffset + local;
-                    const U4 r = philox4x32_10(0xA0000000u, 0u, (uint32_t)path, (uint32_t)(path >> 32), (uint32_t)seed,
-                          (uint32_t)(seed >> 32));
-                    const double u = (double)(r.x >> 8) * 0x1p-24;
-                    if (u < q) {
-                        keep = true;
-                        factor = (float)(1.0 / q);
-                    }
-                }
-            }
+    // q_p = w_p * scale, scale = numPaths / (kappa S)
+    const double scale = (double)numPaths / ((double)kappa * fin->sumScaled);
+    const uint64_t first = (uint64_t)blockIdx.x * (256u * kPickPerThread);
+    uint32_t keepBits = 0;
+    float factor[kPickPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kPickPerThread; ++k) {
+        factor[k] = 0.0f;
+        const uint64_t local = first + k * 256u + threadIdx.x;
+        if (local >= numPaths) continue;
+        const float2 lw = pathLog2[local];
+        const float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
+        if (!(wp > 0.0f)) continue;
+        const double q = (double)wp * scale;
+        if (q >= 1.0) {
+            keepBits |= 1u << k;
+            factor[k] = 1.0f;
+            continue;
         }
-        // one counter update per wave; every lane of the wave reaches the ballot
-        const uint64_t mask = __ballot(keep);
-        const uint32_t wl = threadIdx.x & 63u;
-        uint32_t base = 0;
-        if (wl == 0 && mask) base = atomicAdd(count, (uint32_t)__popcll(mask));
-        base = __shfl(base, 0, 64);
-        if (keep) {
-            const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << wl) - 1ull));
-            pick[slot].local = local;
-            pick[slot].factor = factor;
+        const uint64_t path = pathOffset + local;
+        const U4 r = philox4x32_10(0xA0000000u, 0u, (uint32_t)path, (uint32_t)(path >> 32), (uint32_t)seed,
+              (uint32_t)(seed >> 32));
+        if ((double)(r.x >> 8) * 0x1p-24 < q) {
+            keepBits |= 1u << k;
+            factor[k] = (float)(1.0 / q);
+        }
+    }
+    // exclusive prefix of the kept counts over the block
+    const uint32_t mine = (uint32_t)__popc(keepBits);
+    const uint32_t wl = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if (wl >= (uint32_t)o) incl += v;
+    }
+    if (wl == 63) waveTotal[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = waveTotal[0] + waveTotal[1] + waveTotal[2] + waveTotal[3];
+        blockBase = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = blockBase + incl - mine;
+    for (uint32_t w = 0; w < wave; ++w) slot += waveTotal[w];
+#pragma unroll
+    for (uint32_t k = 0; k < kPickPerThread; ++k) {
+        if (keepBits & (1u << k)) {
+            pick[slot].local = first + k * 256u + threadIdx.x;
+            pick[slot].factor = factor[k];
             pick[slot].pad = 0u;
+            ++slot;
         }
     }
 }
@@ -418,7 +440,9 @@ hipError_t launch_adjoint(const AdjParams &p, uint32_t G, bool sweep, uint32_t g
 hipError_t launch_adjoint_pick(const float2 *pathLog2, const FinalOut *fin, uint64_t numPaths, uint64_t pathOffset,
       uint64_t seed, float kappa, AdjPick *pick, uint32_t *count, hipStream_t s)
 {
-    const uint64_t blocks = std::min<uint64_t>((numPaths + 255) / 256, 65536);
+    const uint64_t per = 256u * kPickPerThread;
+    const uint64_t blocks = (numPaths + per - 1) / per;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(adjoint_pick_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, pathLog2, fin, numPaths, pathOffset,
           seed, kappa, pick, count);
     return hipGetLastError();

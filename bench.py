@@ -176,6 +176,41 @@ def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22,
     return out
 
 
+def measure_c5(capi, host, reps=2):
+    """Large beam spread: N=1024, arclength 2.0 between endpoints 1.0 apart,
+    K=200, 2^24 paths, the sum of 3 tables (sigma 0.02/0.05/0.1), 256x256
+    exit-direction histogram."""
+    import numpy as np
+
+    N, L, K, P = 1024, 2.0, 200, 1 << 24
+    ds = float(np.float32(L) / np.float32(N))
+    tabs = [host.build_rt_table(ds, 1.0, float(np.float32(s) ** 2), 0.1, 0.004, numStepsInt=20000,
+                                numCurvatureSteps=4096) for s in (0.02, 0.05, 0.1)]
+    if len({(t.minK, t.maxK, t.step) for t in tabs}) != 1:
+        raise RuntimeError("C5 tables do not share one curvature grid")
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
+    ctx = capi.Context(0)
+    t0 = tabs[0]
+    ctx.set_tables(np.stack([t.table for t in tabs]), t0.minK, t0.maxK, t0.step, t0.ds, 0.004)
+    ctx.set_curve(host.initial_arc(bc, N), bc)
+    ctx.set_exit_histogram(256, 256)
+    ctx.run(P, K, seed=1)
+    fwd, tot = [], []
+    for _ in range(reps):
+        r = ctx.run(P, K, seed=1)
+        f, t = ctx.last_timing()
+        fwd.append(f)
+        tot.append(t)
+    h, _ = ctx.exit_histogram()
+    sched = ctx.get_schedule()
+    ctx.close()
+    return {"workload": "C5: N=1024, arclength 2.0, endpoints 1.0 apart, K=200, 2^24 paths, 3 summed tables "
+                        "sigma 0.02/0.05/0.1, 256x256 exit histogram",
+            "forward_ms": statistics.fmean(fwd), "forward_plus_histogram_ms": statistics.fmean(tot),
+            "joint_weights_per_s": P * (N - 1) / (statistics.fmean(tot) / 1e3), "num_valid": r["num_valid"],
+            "histogram_sum": float(h.sum()), "schedule": sched}
+
+
 def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     """The batched workload in one graph-replayed launch vs one plain run per experiment."""
     import numpy as np
@@ -240,6 +275,7 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-generated", action="store_true")
     ap.add_argument("--no-roulette", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -310,6 +346,9 @@ def main(argv=None):
         out["batched"] = measure_batched(ctx, capi)
         progress(f"batched: {out['batched']['batched_ms']:.3f} ms")
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
+    if not a.no_c5:
+        out["c5"] = measure_c5(capi, host)
+        progress(f"C5: {out['c5']['forward_plus_histogram_ms']:.2f} ms")
     if not a.no_generated:
         out["generated"] = measure_generated(capi, host, None if a.no_cpu_baseline else a.cpu_seconds / 4)
         progress(f"independent paths: {out['generated']['kernel_ms']:.3f} ms")

@@ -6,6 +6,7 @@
 The library lands in twisty_amd/lib/ so that it travels with the source tree.
 """
 import argparse
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -49,16 +50,18 @@ def build(asm=False, verbose=True):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        run([cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
-        if asm:
-            adir = os.path.join(BUILD, "asm")
-            os.makedirs(adir, exist_ok=True)
-            run([cc, *HIPCC_FLAGS, "--cuda-device-only", "-S", os.path.join(CSRC, src), "-o",
-                 os.path.join(adir, os.path.splitext(src)[0] + ".s")])
+    objs = [os.path.join(BUILD, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+    cmds = [[cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj] for src, obj in zip(HIP_SOURCES, objs)]
+    if asm:
+        adir = os.path.join(BUILD, "asm")
+        os.makedirs(adir, exist_ok=True)
+        cmds += [[cc, *HIPCC_FLAGS, "--cuda-device-only", "-S", os.path.join(CSRC, src), "-o",
+                  os.path.join(adir, os.path.splitext(src)[0] + ".s")] for src in HIP_SOURCES]
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(run, c) for c in cmds]:
+            f.result()
     out = os.path.join(LIBDIR, "libtwisty_hip.so")
     tmp = out + ".tmp"
     run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -100,6 +100,24 @@ int twisty_host_write_path_batches(const char *experimentDir, const char *prepen
 int twisty_host_run_experiments(const char *jsonText, const float *boundaries, const uint64_t *seeds,
       uint32_t numExperiments, twisty_host_result *out, double *deviceMs, char *errorOut, uint32_t errorLen);
 
+/* Gradients of one experiment's log-estimator E = log10(sum_p w_p / P). */
+typedef struct twisty_host_gradients {
+    double log10_estimate;  /* E, without the path normaliser (1 for N >= 7) */
+    double d_eps;           /* dE / d eps, the config's eps slot (maxBound = 10/eps moves with it) */
+    double d_theta, d_phi;  /* dE / d angles of the end direction (sin t cos p, sin t sin p, cos t) */
+    double kernel_ms;       /* forward + adjoint device time */
+} twisty_host_gradients;
+
+/* The adjoint of the experiment of a JSON config (text; experiment.
+ * basicExperiment, experimentParams with twisty_hip.perturbationsPerPath as K,
+ * paths restarting from the initial curve): dE/dT into dTableOut (nullable,
+ * numCurvatureSteps doubles) and, chained on the host in FP64 through dT/d eps
+ * and the initial arc's Jacobian, dE/d eps and dE/d(theta, phi).  No
+ * counterpart in the reference, which has no gradients; it differentiates the
+ * estimate RunnerSpecificRunExperiment returns (ExperimentRunner.h:99-101). */
+int twisty_host_experiment_gradients(const char *jsonText, twisty_host_gradients *out, double *dTableOut,
+      uint32_t dTableLen, char *errorOut, uint32_t errorLen);
+
 /* ExperimentBase::Result (Experiments/ExperimentBase.h) with the weight as log10. */
 typedef struct twisty_host_generation_result {
     double log10_total_weight;   /* log10 of sum_valid 10^(w + normaliser) / num_valid */

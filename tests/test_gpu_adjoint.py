@@ -220,3 +220,48 @@ def test_adjoint_roulette_against_exact(hip_device, oracle):
     assert e_seg <= 5e-3 and e_run <= 1e-3
     # same seed, same kept set: the second run agrees up to float atomics order
     assert rel_norm(again["d_seg"], r["d_seg"]) < 1e-5
+
+
+def test_host_experiment_gradients_match_autograd(hip_device, oracle):
+    """The C++ chain (twisty_host_experiment_gradients) equals the torch.autograd
+    composition of the same pieces: dE/d eps * 2 sigma = dE/d sigma, and the
+    exit angles, for e = +z and a tilted exit."""
+    import torch
+
+    from twisty_amd import autograd as ag
+
+    case, paths = cases.C4, 1 << 13
+    N, L, K = case.N, case.L, case.K
+    ds = cases.ds_of(case)
+    for edir in ((0.0, 0.0, 1.0), (0.6, 0.0, 0.8)):
+        bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), edir, L)
+        th0, ph0 = math.acos(edir[2]), math.atan2(edir[1], edir[0])
+        sigma = torch.tensor(case.sigma, dtype=torch.float64, requires_grad=True)
+        ang = torch.tensor([th0, ph0], dtype=torch.float64, requires_grad=True)
+        hctx = capi.Context(0)
+        table, grid = ag.rt_table_with_grid(sigma, ds, numStepsInt=case.numStepsInt, bins=case.bins)
+        E = ag.log10_estimate(table, ag.exit_curve(ang, bc, N), hctx, grid, bc, paths, K, seed=case.seed)
+        E.backward()
+        hctx.close()
+        cfg = {"experiment": {
+            "experimentParams": {
+                "pathsToGenerate": paths, "pathsToSkip": 0, "numSegments": N, "name": "grad", "experimentDir": "/tmp",
+                "maxPerturbThreads": 0, "maxWeightThreads": 0, "outputBigFloatWeights": False,
+                "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
+                "random": {"bootstrapSeed": 1, "perturbSeed": case.seed},
+                "weighting": {"weightFunction": 0, "mu": 1.0, "eps": float(np.float32(case.sigma) ** 2),
+                              "numStepsInt": case.numStepsInt, "numCurvatureSteps": case.bins,
+                              "absorption": case.absorption, "scatter": 0.1},
+                "twisty_hip": {"perturbationsPerPath": K, "chainLength": 1, "writeFiles": False}},
+            "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1],
+                                "endDir": list(edir), "arclength": L}}}
+        g = host.experiment_gradients(cfg, table_bins=case.bins)
+        d_sigma = g["d_eps"] * 2.0 * float(np.float32(case.sigma))
+        print(f"[host gradients e={edir}] E {g['log10_estimate']:.9f} vs {float(E):.9f}; dsigma {d_sigma:.6e} vs "
+              f"{float(sigma.grad):.6e}; dtheta {g['d_theta']:.6e} vs {float(ang.grad[0]):.6e}; dphi {g['d_phi']:.6e} "
+              f"vs {float(ang.grad[1]):.6e}", flush=True)
+        assert abs(g["log10_estimate"] - float(E)) < 1e-9 * abs(float(E))
+        assert d_sigma == pytest.approx(float(sigma.grad), rel=1e-4)
+        scale = max(abs(float(ang.grad[0])), abs(float(ang.grad[1])))
+        assert abs(g["d_theta"] - float(ang.grad[0])) < 1e-4 * scale
+        assert abs(g["d_phi"] - float(ang.grad[1])) < 1e-4 * scale

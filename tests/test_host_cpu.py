@@ -378,3 +378,11 @@ def test_path_generation_mc_multi_minimum_ds(tmp_path):
     if capi.device_count() == 0:
         with pytest.raises(RuntimeError, match="twisty_hip_create"):
             host.path_generation_mc_multi(cfg, 0.03, 0.07, 5, D / N * 1.001, cache_root=tmp_path)
+
+
+def test_experiment_gradients_errors():
+    with pytest.raises(RuntimeError, match="dTableLen"):
+        host.experiment_gradients(_config(), table_bins=7)
+    if capi.device_count() == 0:
+        with pytest.raises(RuntimeError, match="twisty_hip_create"):
+            host.experiment_gradients(_config())

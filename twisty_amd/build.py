@@ -74,7 +74,7 @@ def build(asm=False, verbose=True):
 
 
 HOST_SOURCES = ["weighting.cpp", "geometry.cpp", "json.cpp", "runner.cpp", "export.cpp", "batch_runner.cpp",
-                "path_generation.cpp",
+                "path_generation.cpp", "gradients.cpp",
                 "host_capi.cpp"]
 DRIVERS = {"twisty_full_experiment": "full_experiment_main.cpp", "twisty_beam_spread": "beam_spread_main.cpp",
            "twisty_ring_benchmark": "ring_benchmark_main.cpp"}

@@ -1,4 +1,5 @@
 // host_capi.cpp -- C ABI of libtwisty_host.so (include/twisty_host.h).
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 
@@ -262,6 +263,37 @@ int twisty_host_path_generation_mc_multi(const char *jsonText, uint64_t seed, do
         out->num_valid = r->numValidPaths;
         out->num_total = r->numPathsTotal;
         out->kernel_ms = r->kernelMs;
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
+int twisty_host_experiment_gradients(const char *jsonText, twisty_host_gradients *out, double *dTableOut,
+      uint32_t dTableLen, char *errorOut, uint32_t errorLen)
+{
+    if (!jsonText || !out) return 1;
+    try {
+        const Json cfg = Json::Parse(jsonText);
+        const ExperimentRunner::ExperimentParameters ep = ExperimentRunner::ParseExperimentParamsFromConfig(cfg);
+        const PerturbUtils::BoundaryConditions bc = ParseBasicExperiment(cfg);
+        if (dTableOut && dTableLen != ep.weightingParameters.numCurvatureSteps) {
+            put_error(errorOut, errorLen, "dTableLen must equal numCurvatureSteps");
+            return 1;
+        }
+        std::string err;
+        const auto g = ComputeExperimentGradients(ep, bc, err);
+        if (!g) {
+            put_error(errorOut, errorLen, err);
+            return 1;
+        }
+        out->log10_estimate = g->log10Estimate;
+        out->d_eps = g->dEps;
+        out->d_theta = g->dTheta;
+        out->d_phi = g->dPhi;
+        out->kernel_ms = g->kernelMs;
+        if (dTableOut) std::copy(g->dTable.begin(), g->dTable.end(), dTableOut);
     } catch (const std::exception &e) {
         put_error(errorOut, errorLen, e.what());
         return 1;

@@ -353,6 +353,24 @@ class BatchedExperimentRunner {
     double m_deviceMs = 0.0;
 };
 
+// Gradients of one experiment's log-estimator E = log10(sum_p w_p / P) (no
+// normaliser; it is 1 for N >= 7, PathWeightUtils.cpp:721-752) through the
+// adjoint: dE/dT, dE/d eps (the config's eps slot, maxBound = 10/eps moving
+// with it, ExperimentRunner.cpp:168-171) and dE/d(theta, phi) of the end
+// direction e = (sin theta cos phi, sin theta sin phi, cos theta), the last two
+// chained on the host in FP64 through dT/d eps and the initial arc's Jacobian.
+// Paths restart from the initial curve (chain length 1); K from
+// perturbationsPerPath; seed 0 = from the clock.  The reference has no
+// gradients.
+struct ExperimentGradients {
+    double log10Estimate = 0.0;
+    double dEps = 0.0, dTheta = 0.0, dPhi = 0.0;
+    std::vector<double> dTable;
+    double kernelMs = 0.0;
+};
+std::optional<ExperimentGradients> ComputeExperimentGradients(const ExperimentRunner::ExperimentParameters &ep,
+      const PerturbUtils::BoundaryConditions &bc, std::string &error);
+
 // The independent-path estimator ExperimentBase::MSegmentPathGenerationMC
 // (Experiments/ExperimentBase.cpp:1246-1386) on twisty_hip_run_generated:
 // every path built from scratch by recursive sphere intersection, weighted

@@ -13,7 +13,7 @@ EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_hos
             "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
             "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc",
-            "twisty_host_path_generation_mc_multi"]
+            "twisty_host_path_generation_mc_multi", "twisty_host_experiment_gradients"]
 
 _lib = None
 
@@ -30,6 +30,16 @@ class HostResult(C.Structure):
         ("perturb_ms", C.c_double),
         ("weighting_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+class Gradients(C.Structure):
+    _fields_ = [
+        ("log10_estimate", C.c_double),
+        ("d_eps", C.c_double),
+        ("d_theta", C.c_double),
+        ("d_phi", C.c_double),
         ("kernel_ms", C.c_double),
     ]
 
@@ -70,6 +80,7 @@ def lib():
                                                      C.c_char_p, u32]
         L.twisty_host_path_generation_mc_multi.argtypes = [C.c_char_p, C.c_uint64, f64, f32, f32, u32, f32,
                                                            C.c_char_p, C.POINTER(GenerationResult), C.c_char_p, u32]
+        L.twisty_host_experiment_gradients.argtypes = [C.c_char_p, C.POINTER(Gradients), vp, u32, C.c_char_p, u32]
         L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
                                                      C.c_uint64, f64, C.c_char_p, u32]
         _lib = L
@@ -247,6 +258,21 @@ def path_generation_mc_multi(config, min_ds_tables, max_ds_tables, num_ds_steps,
                                                   num_ds_steps, max_ds, root, C.byref(res), err, 1024) != 0:
         raise RuntimeError(f"path_generation_mc_multi failed: {err.value.decode()}")
     return {k: getattr(res, k) for k, _ in res._fields_}
+
+
+def experiment_gradients(config, table_bins=None):
+    """dE/d eps, dE/d(theta, phi) and dE/dT of a config's experiment (C++ chain of the adjoint)."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    res = Gradients()
+    err = C.create_string_buffer(1024)
+    dT = np.zeros(table_bins, np.float64) if table_bins else None
+    if lib().twisty_host_experiment_gradients(text.encode(), C.byref(res), None if dT is None else _p(dT),
+                                              table_bins or 0, err, 1024) != 0:
+        raise RuntimeError(f"experiment_gradients failed: {err.value.decode()}")
+    out = {k: getattr(res, k) for k, _ in res._fields_}
+    if dT is not None:
+        out["d_table"] = dT
+    return out
 
 
 def run_config(config):

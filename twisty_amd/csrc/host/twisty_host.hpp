@@ -265,7 +265,12 @@ class ExperimentRunner {
         bool useGpu = true;
         // twisty_hip settings
         uint32_t perturbationsPerPath = 1;  // K; the reference applies 1 between weightings
-        uint32_t chainLength = 0;           // paths sharing one carried state; 0 = about 65536 chains
+        // Paths sharing one carried state.  The default is the reference CPU
+        // runner's chain of 1e6 paths per thread (FullExperimentRunnerOptimalPerturb.cpp:
+        // 255-277), so a reference config keeps its Markov-chain statistics;
+        // 0 = about 65536 chains (fills the GPU; paths near each chain start
+        // stay close to the initial curve unless pathsToSkip burns them in).
+        uint32_t chainLength = 1000000;
         int device = 0;
         bool writeFiles = true;             // table CSV, Results-style files
         bool autotune = false;              // batched runs: time the schedules first

@@ -32,7 +32,8 @@ def config(N=20, L=1.2, paths=1 << 16, K=50, sigma=0.1, chain=1, seed=1, bins=40
             "random": {"bootstrapSeed": 1, "perturbSeed": seed},
             "weighting": {"weightFunction": 0, "mu": 1.0, "eps": float(np.float32(sigma) ** 2),
                           "numStepsInt": 20000, "numCurvatureSteps": bins, "absorption": 0.004, "scatter": 0.1},
-            "twisty_hip": {"perturbationsPerPath": K, "chainLength": chain, "writeFiles": write}},
+            "twisty_hip": {"perturbationsPerPath": K, "writeFiles": write,
+                           **({} if chain is None else {"chainLength": chain})}},
         "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1],
                             "endDir": [0, 0, 1], "arclength": L}}}
 
@@ -58,6 +59,27 @@ def test_runner_matches_oracle(hip_device, oracle, N, L, K, sigma, chain, skip):
           f"oracle={ref_avg:.9f} kernel_ms={r['kernel_ms']:.3f} setup_ms={r['setup_ms']}", flush=True)
     assert r["total_paths"] == paths and r["failed_paths"] == 0
     assert abs(10 ** (r["log10_avg_path_weight"] - ref_avg) - 1.0) < 1e-3
+
+
+def test_runner_default_chain_is_the_reference_chain(hip_device, oracle):
+    """No chainLength in the config: one chain of 1e6 paths as the reference's
+    CPU runner, with pathsToSkip burn-in, against the oracle on the same draws.
+    Over 32768 chained perturbations FP32 and FP64 states drift apart (a
+    near-folded sub-chain has a short chord, and its axis amplifies rounding),
+    so the two chains agree statistically: within 3 standard errors of the
+    oracle chain's batch means."""
+    N, L, K, sigma, paths = 20, 1.2, 1, 0.1, 1 << 15
+    r = host.run_config(config(N=N, L=L, paths=paths, K=K, sigma=sigma, chain=None, skip=100))
+    ref = oracle_same_inputs(oracle, N, L, K, paths, sigma, chain=1_000_000, skip=100)
+    ref_avg = ref["log10_sum_w_exact"] - math.log10(paths)
+    w = ref["log10w"]
+    m = np.nanmax(w)
+    batch = np.nanmean(10.0 ** (w - m).reshape(32, -1), axis=1)
+    rel_se = batch.std(ddof=1) / math.sqrt(len(batch)) / batch.mean()
+    diff = abs(10 ** (r["log10_avg_path_weight"] - ref_avg) - 1.0)
+    print(f"[runner default chain] gpu {r['log10_avg_path_weight']:.9f} oracle {ref_avg:.9f} rel diff {diff:.2e} "
+          f"oracle rel SE {rel_se:.2e}", flush=True)
+    assert r["total_paths"] == paths and diff < 3 * rel_se + 1e-3
 
 
 def test_full_experiment_driver(hip_device, tmp_path):

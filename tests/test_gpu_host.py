@@ -245,3 +245,44 @@ def test_ring_benchmark_driver(hip_device, tmp_path):
     ref = host.run_config(one)
     assert abs(ref["log10_experiment_weight"] - float(rows[-1][2])) < 1e-6
     assert "Avg path weight" in out.stdout
+
+
+def _convergence_lines(path):
+    out = []
+    for line in path.read_text().splitlines():
+        n, w = line.split(",")
+        out.append((int(n), float(w)))
+    return out
+
+
+def test_runner_convergence_weights(hip_device, tmp_path):
+    """ConvergenceWeights/: 2.5e6 paths = three accumulators of 1e6 consecutive
+    paths, dealt to maxPerturbThreads=2 threads in blocks of ceil(3/2) = 2
+    (FullExperimentRunnerOptimalPerturb.cpp:221-266, 342-367).  Thread 0's
+    file has two cumulative lines, thread 1's one; the global file (written
+    with outputBigFloatWeights) has three.  Its first line equals a run of the
+    first 1e6 paths alone (independent chains, same draws) and its last the
+    experiment's average, to the FP32 rounding of the per-path log10 weights, and thread sums add up to the global ones."""
+    N, L, K, sigma, paths = 20, 1.2, 20, 0.1, 2_500_000
+    cfg = config(N=N, L=L, paths=paths, K=K, sigma=sigma, write=True, outdir=str(tmp_path / "full"))
+    cfg["experiment"]["experimentParams"]["maxPerturbThreads"] = 2
+    cfg["experiment"]["experimentParams"]["outputBigFloatWeights"] = True
+    r = host.run_config(cfg)
+    first = host.run_config(config(N=N, L=L, paths=1_000_000, K=K, sigma=sigma))
+    d = list((tmp_path / "full").rglob("ConvergenceWeights"))
+    assert len(d) == 1
+    t0 = _convergence_lines(d[0] / "ConvergenceWeights_0.txt")
+    t1 = _convergence_lines(d[0] / "ConvergenceWeights_1.txt")
+    g = _convergence_lines(d[0] / "ConvergenceWeights.txt")
+    assert sorted(p.name for p in d[0].iterdir()) == ["ConvergenceWeights.txt", "ConvergenceWeights_0.txt",
+                                                        "ConvergenceWeights_1.txt"]
+    print(f"[convergence] thread0 {t0} thread1 {t1} global {g}", flush=True)
+    valid = paths - r["failed_paths"]
+    assert [n for n, _ in g][-1] == valid and len(t0) == 2 and len(t1) == 1 and len(g) == 3
+    assert g[0][0] == 1_000_000 - first["failed_paths"]
+    # the files sum the per-path FP32 log10 weights, the run its device partials
+    assert abs(math.log10(g[0][1] * g[0][0]) - first["log10_experiment_weight"]) < 1e-5
+    assert abs(math.log10(g[-1][1] * valid) - r["log10_experiment_weight"]) < 1e-5
+    assert t0[0] == g[0] and t0[1] == g[1]
+    assert t1[0][0] == g[2][0] - g[1][0]
+    assert abs(t1[0][1] * t1[0][0] + g[1][1] * g[1][0] - g[2][1] * g[2][0]) < 1e-9 * g[2][1] * g[2][0]

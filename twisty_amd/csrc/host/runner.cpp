@@ -6,6 +6,7 @@
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "../../../include/twisty_hip.h"
 #include "twisty_host.hpp"
@@ -148,6 +149,104 @@ double log10_add(double a, double b)
     return m + std::log10(std::pow(10.0, a - m) + std::pow(10.0, b - m));
 }
 
+// ConvergenceWeights/ (FullExperimentRunnerOptimalPerturb.cpp:90-109, 342-367;
+// ExperimentRunner.cpp:399-422): the paths are summed per accumulator of 1e6
+// consecutive path ids, the accumulators of every dispatch of 2e10 paths are
+// dealt to T "threads" in blocks of ceil(accumulators / T) as the reference's
+// dispatch does (:221-266), and after every
+// accumulator a line "paths, weight / paths" goes to the thread's
+// ConvergenceWeights_<t>.txt and, with outputBigFloatWeights, to the global
+// ConvergenceWeights.txt (paths = valid paths so far, weight = sum of
+// 10^(w + normaliser) so far).  Fed the per-path log10 weights in path order.
+class ConvergenceWriter {
+   public:
+    ConvergenceWriter(const std::string &experimentDir, uint64_t numPaths, uint32_t threads, double normLog10,
+          bool global)
+        : m_norm(normLog10)
+    {
+        const std::filesystem::path dir = std::filesystem::path(experimentDir) / "ConvergenceWeights";
+        std::filesystem::create_directories(dir);
+        m_threads = threads ? threads : 1;
+        m_pathsLeft = numPaths;
+        StartDispatch();
+        m_files.resize(m_threads);
+        for (uint32_t t = 0; t < m_threads; ++t) {
+            m_files[t].open((dir / ("ConvergenceWeights_" + std::to_string(t) + ".txt")).string());
+            if (!m_files[t].is_open()) throw std::runtime_error("Failed to open ConvergenceWeights_" + std::to_string(t));
+        }
+        m_threadPaths.assign(m_threads, 0);
+        m_threadW.assign(m_threads, -INFINITY);
+        if (global) {
+            m_global.open((dir / "ConvergenceWeights.txt").string());
+            if (!m_global.is_open()) throw std::runtime_error("Failed to open weight convergence file");
+        }
+    }
+    void Add(const float *log10w, uint64_t n)
+    {
+        for (uint64_t q = 0; q < n; ++q) {
+            const float w = log10w[q];
+            if (!std::isnan(w)) {  // invalid paths add nothing (:547-549)
+                // running max m and sum of 10^(w - m): one pow per path
+                if (m_accValid == 0) {
+                    m_accMax = w;
+                    m_accSum = 1.0;
+                } else if (w > m_accMax) {
+                    m_accSum = m_accSum * std::pow(10.0, m_accMax - (double)w) + 1.0;
+                    m_accMax = w;
+                } else {
+                    m_accSum += std::pow(10.0, (double)w - m_accMax);
+                }
+                ++m_accValid;
+            }
+            ++m_accPaths;
+            if (m_accPaths == kAccPaths || m_accPaths == m_dispatchLeft) Close();
+        }
+    }
+    void Finish()
+    {
+        if (m_accPaths) Close();
+    }
+
+   private:
+    static constexpr uint64_t kAccPaths = 1000000;       // MaxNumPathsPerCombinedWeight
+    static constexpr uint64_t kDispatchPaths = 20000000000ull;  // MaxNumberOfPathsInDispatch
+    void StartDispatch()
+    {
+        m_dispatchLeft = std::min(m_pathsLeft, kDispatchPaths);
+        m_pathsLeft -= m_dispatchLeft;
+        const uint64_t numAcc = (m_dispatchLeft + kAccPaths - 1) / kAccPaths;
+        m_perThread = std::max<uint64_t>(1, (numAcc + m_threads - 1) / m_threads);
+        m_acc = 0;
+    }
+    void Close()
+    {
+        const double accW = m_accValid ? m_accMax + std::log10(m_accSum) + m_norm : -INFINITY;
+        const uint32_t t = (uint32_t)std::min<uint64_t>(m_acc / m_perThread, m_threads - 1);
+        m_threadPaths[t] += m_accValid;
+        m_threadW[t] = log10_add(m_threadW[t], accW);
+        m_files[t] << m_threadPaths[t] << ", " << (BigWeight { m_threadW[t] } / (double)m_threadPaths[t]).ToString()
+                   << '\n';
+        if (m_global.is_open()) {
+            m_globalPaths += m_accValid;
+            m_globalW = log10_add(m_globalW, accW);
+            m_global << m_globalPaths << ", " << (BigWeight { m_globalW } / (double)m_globalPaths).ToString() << '\n';
+        }
+        ++m_acc;
+        m_dispatchLeft -= m_accPaths;
+        m_accPaths = m_accValid = 0;
+        if (m_dispatchLeft == 0 && m_pathsLeft) StartDispatch();
+    }
+    double m_norm;
+    uint32_t m_threads = 1;
+    uint64_t m_perThread = 1, m_acc = 0, m_accPaths = 0, m_accValid = 0, m_globalPaths = 0;
+    uint64_t m_pathsLeft = 0, m_dispatchLeft = 0;
+    double m_accMax = 0.0, m_accSum = 0.0, m_globalW = -INFINITY;
+    std::vector<std::ofstream> m_files;
+    std::ofstream m_global;
+    std::vector<uint64_t> m_threadPaths;
+    std::vector<double> m_threadW;
+};
+
 // Partials of two disjoint path ranges of one experiment.
 twisty_hip_partials merge_partials(const twisty_hip_partials &a, const twisty_hip_partials &b)
 {
@@ -242,13 +341,47 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
     }
     twisty_hip_partials part {};
     std::vector<uint64_t> counts(kHistBins, 0);
-    if (nPaths > 0 && !ep.outputPathBatches) {
-        rc = twisty_hip_run(ctx, nPaths, 0, ep.perturbationsPerPath, L, seed, &part, nullptr, nullptr);
-        if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
-        if (ep.writeFiles) {
-            rc = twisty_hip_get_weight_histogram(ctx, counts.data());
-            if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+    float totMs = 0.0f;  // kernel time summed over the runs
+    auto addTiming = [&]() {
+        float f = 0.0f, t = 0.0f;
+        twisty_hip_last_timing(ctx, &f, &t);
+        totMs += t;
+    };
+    // the reference's thread count: maxPerturbThreads, 0 or above the
+    // hardware's = hardware_concurrency (FullExperimentRunnerOptimalPerturb.cpp:73-85)
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t convThreads =
+          (ep.maxPerturbThreads <= 0 || (uint32_t)ep.maxPerturbThreads > hw) ? hw : (uint32_t)ep.maxPerturbThreads;
+    std::unique_ptr<ConvergenceWriter> conv;
+    if (ep.writeFiles && nPaths > 0) {
+        try {
+            conv = std::make_unique<ConvergenceWriter>(m_experimentDirPath, nPaths, convThreads, normLog10,
+                  ep.outputBigFloatWeights);
+        } catch (const std::exception &e) {
+            return fail(e.what());
         }
+    }
+    if (nPaths > 0 && !ep.outputPathBatches) {
+        // One run, or with the convergence files chunks of whole chains (the
+        // draws of one run) of about 2^26 paths whose per-path weights come
+        // back to the host.
+        const uint64_t chunk = conv ? std::max<uint64_t>(1, (uint64_t(1) << 26) / L) * L : nPaths;
+        std::vector<float> perPath(conv ? std::min(chunk, nPaths) : 0);
+        std::vector<uint64_t> chunkCounts(kHistBins);
+        for (uint64_t off = 0; off < nPaths; off += chunk) {
+            const uint64_t n = std::min(chunk, nPaths - off);
+            twisty_hip_partials c {};
+            rc = twisty_hip_run(ctx, n, off, ep.perturbationsPerPath, L, seed, &c, conv ? perPath.data() : nullptr,
+                  nullptr);
+            if (rc == TWISTY_HIP_OK && ep.writeFiles) rc = twisty_hip_get_weight_histogram(ctx, chunkCounts.data());
+            if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+            addTiming();
+            if (ep.writeFiles)
+                for (uint32_t i = 0; i < kHistBins; ++i) counts[i] += chunkCounts[i];
+            if (conv) conv->Add(perPath.data(), n);
+            part = off == 0 ? c : merge_partials(part, c);
+        }
+        if (conv) conv->Finish();
     } else if (nPaths > 0) {
         // Path-batch export: chunks of whole chains (pathOffset a multiple of
         // L, so the draws are those of one run), final points and per-path
@@ -270,13 +403,16 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
                 if (rc == TWISTY_HIP_OK) rc = twisty_hip_get_final_positions(ctx, points.data(), n);
                 if (rc == TWISTY_HIP_OK && ep.writeFiles) rc = twisty_hip_get_weight_histogram(ctx, chunkCounts.data());
                 if (rc != TWISTY_HIP_OK) return fail(twisty_hip_last_error(ctx));
+                addTiming();
                 if (ep.writeFiles)
                     for (uint32_t i = 0; i < kHistBins; ++i) counts[i] += chunkCounts[i];
                 writer.Add(points.data(), log10w.data(), n, normLog10);
+                if (conv) conv->Add(log10w.data(), n);
                 part = first ? c : merge_partials(part, c);
                 first = false;
             }
             writer.Finish();
+            if (conv) conv->Finish();
         } catch (const std::exception &e) {
             return fail(e.what());
         }
@@ -294,8 +430,6 @@ ExperimentRunner::RunnerSpecificResults FullExperimentRunnerOptimalPerturb_HIP::
         }
     }
     const auto runEnd = std::chrono::high_resolution_clock::now();
-    float fwdMs = 0.0f, totMs = 0.0f;
-    twisty_hip_last_timing(ctx, &fwdMs, &totMs);
 
     ExperimentResults res;
     res.experimentWeight.log10 = part.log10_sum_w + normLog10;

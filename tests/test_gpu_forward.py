@@ -118,6 +118,34 @@ def test_schedules(hip_device, oracle, lanes):
     check_against_oracle(case, oracle, gpu, ref, ref32, table, f"C2 lanes={lanes}")
 
 
+def test_lean_instance_matches_lds_instance(hip_device, oracle):
+    """C2 resolves to the lean G=4 instance (768 lanes, table read from global
+    memory, draw records exchanged with ds_bpermute).  It computes what the
+    512-lane instance with the table and records in LDS computes: per-path
+    weights, per-joint curvatures and weight-histogram counts bit-identical,
+    with chains and burn-in; aggregates differ only through the grid's
+    summation order."""
+    case = cases.C2
+    n, L = 4096, 16
+    out = {}
+    for block in (0, 512):
+        with cases.hip_context(case, oracle, schedule=(0, block, 0) if block else None) as ctx:
+            ctx.set_weight_histogram(500, 1e-90, 1e-80, 0.0)
+            ctx.set_burn_in(5)
+            r = ctx.run(n, 2, seed=case.seed, chain_length=L, per_path=True, per_joint=True)
+            r["hist"] = ctx.weight_histogram()
+            r["schedule"] = ctx.get_schedule()
+            out[block] = r
+        _report(f"block={block}", **out[block]["schedule"])
+    assert out[0]["schedule"]["block_size"] == 768 and out[0]["schedule"]["lanes_per_path"] == 4
+    a, b = out[0], out[512]
+    assert np.array_equal(a["log10w"].view(np.uint32), b["log10w"].view(np.uint32))
+    assert np.array_equal(a["curv"].view(np.uint32), b["curv"].view(np.uint32))
+    assert np.array_equal(a["hist"], b["hist"]) and a["hist"].sum() > 0
+    # FP32 rescaling of the block accumulators: ~1e-7 relative, order dependent
+    assert a["num_valid"] == b["num_valid"] and abs(a["log10_sum_w"] - b["log10_sum_w"]) <= 1e-6
+
+
 @pytest.mark.parametrize("case", [cases.SMALL, cases.MID, cases.C4, cases.BIG], ids=lambda c: c.name)
 def test_other_shapes(hip_device, oracle, case):
     table, _ = cases.setup(case, oracle)

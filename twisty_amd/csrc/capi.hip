@@ -25,9 +25,10 @@ namespace {
 uint32_t hist_stride(const twisty_hip_ctx *ctx) { return ctx->whBins ? ((ctx->whBins + 63u) & ~63u) : 0u; }
 
 // Resolves a schedule for segment count N.  Zero fields are filled in: G from N
-// (about 16 segments per lane), then the largest compiled block size whose LDS
-// fits one block, then as many blocks per CU as the per-CU LDS holds (at most
-// 2048 lanes per CU).
+// (about 16 segments per lane), then the largest compiled block size with the
+// table in LDS whose LDS fits one block (a lean instance instead when it keeps
+// more lanes resident per CU), then as many blocks per CU as the per-CU LDS
+// holds (at most 2048 lanes per CU).
 int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_t &block, uint32_t &bpc,
       uint32_t &lds, std::string &why)
 {
@@ -43,27 +44,36 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
         why = "lanesPerPath must be a power of two <= 64";
         return TWISTY_HIP_ERR_ARG;
     }
-    uint32_t sizes[32];
-    const int ns = forward_block_sizes(G, sizes, 32);
+    FwdInstance inst[32];
+    const int ns = forward_instances(G, inst, 32);
+    auto ldsOf = [&](const FwdInstance &x) { return forward_lds_bytes(G, x.block, N, ts, hist_stride(ctx), x.lean); };
+    auto lanesPerCU = [&](const FwdInstance &x) {
+        uint32_t b = ctx->ldsPerCU / ldsOf(x);
+        if (b * x.block > 2048) b = 2048 / x.block;
+        return (b < 1 ? 1u : b) * x.block;
+    };
     if (block == 0) {
-        for (int k = ns - 1; k >= 0; --k) {
-            if (forward_lds_bytes(G, sizes[k], N, ts, hist_stride(ctx)) <= ctx->ldsPerBlock) {
-                block = sizes[k];
-                break;
-            }
-        }
-        if (block == 0) {
+        int pick = -1;
+        for (int k = ns - 1; k >= 0 && pick < 0; --k)
+            if (!inst[k].lean && ldsOf(inst[k]) <= ctx->ldsPerBlock) pick = k;
+        for (int k = 0; k < ns; ++k)
+            if (inst[k].lean && ldsOf(inst[k]) <= ctx->ldsPerBlock &&
+                  (pick < 0 || lanesPerCU(inst[k]) > lanesPerCU(inst[pick])))
+                pick = k;
+        if (pick < 0) {
             why = "no compiled block size fits the LDS of one block for N=" + std::to_string(N);
             return TWISTY_HIP_ERR_UNSUPPORTED;
         }
+        block = inst[pick].block;
     }
-    bool compiled = false;
-    for (int k = 0; k < ns; ++k) compiled |= (sizes[k] == block);
-    if (!compiled) {
+    int chosen = -1;
+    for (int k = 0; k < ns; ++k)
+        if (inst[k].block == block) chosen = k;
+    if (chosen < 0) {
         why = "no kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
         return TWISTY_HIP_ERR_UNSUPPORTED;
     }
-    lds = forward_lds_bytes(G, block, N, ts, hist_stride(ctx));
+    lds = ldsOf(inst[chosen]);
     if (lds > ctx->ldsPerBlock) {
         why = "schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
               std::to_string(ctx->ldsPerBlock);

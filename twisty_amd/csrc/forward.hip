@@ -57,7 +57,11 @@ __device__ __forceinline__ uint32_t weight_bin(const FwdParams &p, uint32_t flag
     return bi < p.weightBins ? bi : p.weightBins - 1u;
 }
 
-template<int G, int BLOCK>
+// LEAN: the table is read from global memory (through the caches; it is only
+// touched by the weighting, once per path) and the draw records go through
+// ds_bpermute, so the LDS holds only path state and histogram bins and more
+// paths stay resident per CU.
+template<int G, int BLOCK, bool LEAN>
 __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
 {
     constexpr uint32_t SLOT = 3u * BLOCK;  // floats per slot row
@@ -66,16 +70,17 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
     // Rows 0..S-1 hold segments; row p.scratchRow (>= S and >= WMAX-1, so that
     // scr - w*SLOT never underflows) is the lane's zero row.
     const uint32_t tabOff = (p.scratchRow + 1) * SLOT;
-    const uint32_t histOff = tabOff + p.tableStride;
+    const uint32_t histOff = tabOff + (LEAN ? 0u : p.tableStride);
     const uint32_t drawOff = histOff + p.histStride;
-    uint32_t *drawbuf = reinterpret_cast<uint32_t *>(smem + drawOff);
+    uint32_t *drawbuf = LEAN ? nullptr : reinterpret_cast<uint32_t *>(smem + drawOff);
     uint32_t *whist = reinterpret_cast<uint32_t *>(smem + histOff);
 
-    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
+    if constexpr (!LEAN)
+        for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
     if (p.weightHist)
         for (uint32_t i = threadIdx.x; i < p.weightBins; i += BLOCK) whist[i] = 0u;
     __syncthreads();
-    const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    const TableView tv { LEAN ? p.table : smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
 
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
@@ -116,11 +121,11 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                 // the top bit set, which weighted paths never reach.
                 const uint64_t burnBase = (p.pathOffset / p.chainLength + chain) * (uint64_t)p.burnIn;
                 for (uint32_t b = 0; b < p.burnIn; ++b)
-                    perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream,
+                    perturb_path<G, BLOCK, LEAN>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream,
                           (1ull << 63) | (burnBase + b), K, N, p.maxDiff);
             }
 
-            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream, path, K, N,
+            perturb_path<G, BLOCK, LEAN>(laneB, lane, j, groupBase, scrB, drawbuf, p.seed, p.stream, path, K, N,
                   p.maxDiff);
 
             // Weighting: tangents (CurvePerturbUtils.cpp:147-154), curvature
@@ -233,32 +238,34 @@ __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts
 }
 
 // ---------------------------------------------------------------------------
-// Instances: (lanes per path, block size).  Sub-chain slot offsets are
+// Instances: (lanes per path, block size, lean).  Sub-chain slot offsets are
 // immediates, so BLOCK is a template parameter; the set below covers the
-// shapes the schedule resolver chooses from (the largest block that fits the
-// LDS).  For N = 64 (G = 4) a 640-lane block measured slower than 512.
+// shapes the schedule resolver chooses from.  A block size names one instance
+// per G (lean or not).  For N = 64 (G = 4) a 640-lane block with the table in
+// LDS measured slower than 512; the lean 768-lane block runs 12 waves per CU.
 // ---------------------------------------------------------------------------
 #define TWISTY_FWD_INSTANCES(X)                                                                                   \
-    X(1, 128) X(1, 192) X(2, 192) X(2, 256) X(2, 320) X(4, 256) X(4, 384) X(4, 512) X(8, 512)                   \
-    X(8, 768) X(8, 1024) X(16, 512) X(16, 1024) X(32, 256) X(32, 384) X(32, 512) X(32, 1024) X(64, 256)          \
-    X(64, 512) X(64, 640)                                                                                         \
-    X(64, 704) X(64, 1024)
+    X(1, 128, false) X(1, 192, false) X(2, 192, false) X(2, 256, false) X(2, 320, false) X(4, 256, false)        \
+    X(4, 384, false) X(4, 512, false) X(4, 768, true) X(8, 512, false) X(8, 768, false) X(8, 1024, false)         \
+    X(16, 512, false) X(16, 1024, false) X(32, 256, false) X(32, 384, false) X(32, 512, false)                   \
+    X(32, 1024, false) X(64, 256, false) X(64, 512, false) X(64, 640, false) X(64, 704, false)                   \
+    X(64, 1024, false)
 
-template<int G, int B>
+template<int G, int B, bool L>
 static hipError_t launch_gb(const FwdParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    auto *fn = forward_kernel<G, B>;
+    auto *fn = forward_kernel<G, B, L>;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, s, p);
     return hipGetLastError();
 }
 
-int forward_block_sizes(uint32_t G, uint32_t *out, int cap)
+int forward_instances(uint32_t G, FwdInstance *out, int cap)
 {
     int n = 0;
-#define X(g, b)                                                                                                   \
-    if (G == g && n < cap) out[n++] = b;
+#define X(g, b, l)                                                                                                \
+    if (G == g && n < cap) out[n++] = FwdInstance { b, l };
     TWISTY_FWD_INSTANCES(X)
 #undef X
     return n;
@@ -267,8 +274,8 @@ int forward_block_sizes(uint32_t G, uint32_t *out, int cap)
 hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s)
 {
-#define X(g, b)                                                                                                   \
-    if (G == g && block == b) return launch_gb<g, b>(p, grid, lds, s);
+#define X(g, b, l)                                                                                                \
+    if (G == g && block == b) return launch_gb<g, b, l>(p, grid, lds, s);
     TWISTY_FWD_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -55,8 +55,16 @@ struct FinalOut {
     double sumScaled;   // S = sum over valid paths of 2^(log2 w - M)
 };
 
-// Block sizes compiled for G lanes per path (ascending); returns the count.
-int forward_block_sizes(uint32_t G, uint32_t *out, int cap);
+// A compiled forward kernel for some G: its block size and whether it is lean
+// (table read from global memory, draw records exchanged with ds_bpermute, so
+// neither takes LDS).
+struct FwdInstance {
+    uint32_t block;
+    bool lean;
+};
+// Instances compiled for G lanes per path (ascending block size); returns the
+// count.
+int forward_instances(uint32_t G, FwdInstance *out, int cap);
 
 // Sub-chain window slots per lane for G lanes per path (diff <= 25).
 inline uint32_t forward_window(uint32_t G) { return (25u + G - 1u) / G; }
@@ -69,15 +77,15 @@ inline uint32_t forward_scratch_row(uint32_t G, uint32_t N)
     return S > w - 1 ? S : w - 1;
 }
 
-// Dynamic LDS bytes of the forward kernel: segment rows + zero row, table,
-// weight-histogram bins, draw records (G > 1); at least the block reduction's
-// scratch.
+// Dynamic LDS bytes of the forward kernel: segment rows + zero row, table
+// (not lean), weight-histogram bins, draw records (G > 1, not lean); at least
+// the block reduction's scratch.
 inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride,
-      uint32_t histStride = 0)
+      uint32_t histStride = 0, bool lean = false)
 {
     const uint64_t rows = (uint64_t)forward_scratch_row(G, N) + 1;
-    const uint64_t floats =
-          rows * 3ull * block + (uint64_t)tableStride + histStride + (G > 1 ? 4ull * block : 0ull);
+    const uint64_t floats = rows * 3ull * block + (lean ? 0ull : (uint64_t)tableStride) + histStride +
+          (G > 1 && !lean ? 4ull * block : 0ull);
     const uint64_t reduceBytes = (uint64_t)(block / 64) * sizeof(LseAcc);
     const uint64_t bytes = floats * 4ull;
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);

@@ -69,10 +69,22 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
 }
 
 
+// Draw record of the lane whose wave-lane byte index is srcB (ds_bpermute: a
+// cross-lane read through the LDS crossbar that uses no LDS storage).
+__device__ __forceinline__ DrawRec bperm_rec(int srcB, const DrawRec &r)
+{
+    return { (uint32_t)__builtin_amdgcn_ds_bpermute(srcB, (int)r.packed),
+        __int_as_float(__builtin_amdgcn_ds_bpermute(srcB, __float_as_int(r.c))),
+        __int_as_float(__builtin_amdgcn_ds_bpermute(srcB, __float_as_int(r.s))),
+        __int_as_float(__builtin_amdgcn_ds_bpermute(srcB, __float_as_int(r.C))) };
+}
+
 // K perturbations of one path, applied in order by its group of G lanes.
 // For G > 1 lane j draws perturbation kb + j of each batch of G and the
-// records are exchanged through the group's slice of drawbuf (4 words per lane).
-template<int G, int BLOCK>
+// records are exchanged through the group's slice of drawbuf (4 words per
+// lane), or with ds_bpermute when LEAN (no drawbuf: the LDS it would take goes
+// to more resident paths).
+template<int G, int BLOCK, bool LEAN = false>
 __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint32_t j, uint32_t groupBase,
       const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t *drawbuf, uint64_t seed, uint32_t stream, uint64_t path,
       uint32_t K, uint32_t N, uint32_t maxDiff)
@@ -81,6 +93,24 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
         const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
         if constexpr (G == 1) {
             perturb_one<G, BLOCK>(laneB, j, scrB, pack_draw(make_draw(seed, stream, path, kb, N, maxDiff)));
+        } else if constexpr (LEAN) {
+            DrawRec mine { 0u, 0.0f, 0.0f, 0.0f };  // past K: never read (kk < kcount)
+            if (kb + j < K) mine = pack_draw(make_draw(seed, stream, path, kb + j, N, maxDiff));
+            // byte index of the group's first lane within the wave
+            const int firstB = (int)(((lane & 63u) - j) * 4u);
+            if constexpr (G <= 8) {
+                DrawRec recs[G];
+#pragma unroll
+                for (int kk = 0; kk < G; ++kk) recs[kk] = bperm_rec(firstB + 4 * kk, mine);
+#pragma unroll
+                for (int kk = 0; kk < G; ++kk) {
+                    if ((uint32_t)kk >= kcount) break;
+                    perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
+                }
+            } else {
+                for (uint32_t kk = 0; kk < kcount; ++kk)
+                    perturb_one<G, BLOCK>(laneB, j, scrB, bperm_rec(firstB + 4 * (int)kk, mine));
+            }
         } else {
             if (kb + j < K) {
                 const DrawRec r = pack_draw(make_draw(seed, stream, path, kb + j, N, maxDiff));

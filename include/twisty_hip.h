@@ -266,12 +266,15 @@ typedef struct twisty_hip_experiment {
 /* Launch schedule of a batch: blockSize 256 or 512; blocksPerCU 0 = as many as
  * LDS allows; segmentsPerLane picks each experiment's lanes per path (the
  * smallest power of two >= 2 with ceil(N / lanes) <= segmentsPerLane);
- * tilePaths = paths per work item. */
+ * tilePaths = paths per work item; lean must be 0 (tables and draw records in
+ * LDS): lean 1 (tables read from global memory, draw records exchanged without
+ * LDS) is reserved and currently returns TWISTY_HIP_ERR_UNSUPPORTED. */
 typedef struct twisty_hip_batch_schedule {
     uint32_t block_size;
     uint32_t blocks_per_cu;
     uint32_t segments_per_lane;
     uint32_t tile_paths;
+    uint32_t lean;
 } twisty_hip_batch_schedule;
 
 /* Uploads a batch: experiments, numTables tables of numBins floats with their

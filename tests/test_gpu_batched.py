@@ -116,7 +116,14 @@ def test_batched_replays_are_deterministic(hip_device):
     ctx.close()
 
 
-@pytest.mark.parametrize("sched", [(256, 0, 8, 1024), (512, 1, 16, 333), (256, 2, 16, 4096), (256, 0, 32, 100000)])
+LEAN_OFF = pytest.mark.skip(reason="lean batched schedules are switched off: a lean batch run did not complete "
+                                   "and its cause has not been shown")
+
+
+@pytest.mark.parametrize("sched", [(256, 0, 8, 1024), (512, 1, 16, 333), (256, 2, 16, 4096), (256, 0, 32, 100000),
+                                   pytest.param((256, 0, 16, 4096, 1), marks=LEAN_OFF),
+                                   pytest.param((256, 3, 16, 333, 1), marks=LEAN_OFF),
+                                   pytest.param((512, 0, 8, 1024, 1), marks=LEAN_OFF)])
 def test_batched_schedules_agree(hip_device, sched):
     w = hetero_batch()
     ctx = capi.Context(0)
@@ -131,6 +138,21 @@ def test_batched_schedules_agree(hip_device, sched):
         ok = np.isfinite(a)
         if batch_lanes(x["N"], sched[2]) == batch_lanes(x["N"], 16):
             assert np.array_equal(a[ok], c[ok])
+    b.close()
+    ctx.close()
+
+
+def test_batched_lean_schedule_rejected(hip_device):
+    """lean = 1 is switched off: set_schedule fails before any launch and the
+    batch keeps its previous schedule."""
+    w = hetero_batch()
+    ctx = capi.Context(0)
+    b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
+    before = b.get_schedule()
+    with pytest.raises(capi.TwistyHipError) as e:
+        b.set_schedule(256, 0, 16, 4096, 1)
+    assert e.value.code == capi.ERR_UNSUPPORTED
+    assert b.get_schedule() == before and before["lean"] == 0
     b.close()
     ctx.close()
 

@@ -51,6 +51,7 @@ class BatchSchedule(C.Structure):
         ("blocks_per_cu", C.c_uint32),
         ("segments_per_lane", C.c_uint32),
         ("tile_paths", C.c_uint32),
+        ("lean", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -358,8 +359,8 @@ class Batch:
         except Exception:
             pass
 
-    def set_schedule(self, block_size=0, blocks_per_cu=0, segments_per_lane=0, tile_paths=0):
-        s = BatchSchedule(block_size, blocks_per_cu, segments_per_lane, tile_paths)
+    def set_schedule(self, block_size=0, blocks_per_cu=0, segments_per_lane=0, tile_paths=0, lean=0):
+        s = BatchSchedule(block_size, blocks_per_cu, segments_per_lane, tile_paths, lean)
         self.ctx._check(lib().twisty_hip_batch_set_schedule(self._h, C.byref(s)))
 
     def get_schedule(self):

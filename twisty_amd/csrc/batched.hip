@@ -10,14 +10,16 @@
 // per-path code as the plain kernel (perturb_path / weigh_state) with the
 // lanes-per-path G of its experiment, chosen from its segment count; the
 // block keeps the experiment's table in LDS and reloads it only when the next
-// item belongs to an experiment with another table.  Each item leaves one
-// LSE partial; a second kernel merges an experiment's partials in FP64.
+// item belongs to an experiment with another table (lean: tables are read from
+// global memory and draw records go through ds_bpermute, so the LDS holds path
+// state only).  Each item leaves one LSE partial; a second kernel merges an
+// experiment's partials in FP64.
 #include "batched.hpp"
 #include "path_kernels.hpp"
 
 namespace twisty_hip {
 
-template<int G, int BLOCK>
+template<int G, int BLOCK, bool LEAN>
 __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e, const BatchItem &item,
       float *smem, const TableView &tv, LseAcc &acc, uint32_t &clamped)
 {
@@ -27,7 +29,7 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
     const uint32_t groupBase = lane - j;
-    uint32_t *drawbuf = reinterpret_cast<uint32_t *>(smem + p.rows * SLOT + p.tableStride);
+    uint32_t *drawbuf = LEAN ? nullptr : reinterpret_cast<uint32_t *>(smem + p.rows * SLOT + p.tableStride);
     const uint32_t scr = e.scratchRow * SLOT + lane;
     smem[scr] = 0.0f;
     smem[scr + BLOCK] = 0.0f;
@@ -50,12 +52,13 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
         }
         // burn-in: the draws of twisty_hip_run's burn-in at pathOffset 0
         for (uint32_t b = 0; b < e.burnIn; ++b)
-            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u,
+            perturb_path<G, BLOCK, LEAN>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u,
                   (1ull << 63) | (chain * e.burnIn + b), e.K, e.N, e.maxDiff);
         for (uint32_t sIdx = 0; sIdx < e.chainLength; ++sIdx) {
             const uint64_t local = chain * e.chainLength + sIdx;
             if (local >= e.numPaths) break;  // the last chain may be shorter
-            perturb_path<G, BLOCK>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u, local, e.K, e.N, e.maxDiff);
+            perturb_path<G, BLOCK, LEAN>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u, local, e.K, e.N,
+                  e.maxDiff);
             wave_sync_lds();
             const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, e.N, e.invDs, tv, clamped);
             wave_sync_lds();  // joints read other lanes' columns before the next reset
@@ -84,14 +87,14 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     }
 }
 
-template<int BLOCK>
+template<int BLOCK, bool LEAN>
 __global__ void __launch_bounds__(BLOCK) batched_kernel(const BatchParams p)
 {
     // No static LDS: the byte addressing of perturb_one takes the dynamic
     // block to start at groupstaticsize(), so the item slot lives at its end.
     extern __shared__ float smem[];
     const uint32_t tabOff = p.rows * 3u * BLOCK;
-    uint32_t &sItem = *reinterpret_cast<uint32_t *>(smem + tabOff + p.tableStride + 4u * BLOCK);
+    uint32_t &sItem = *reinterpret_cast<uint32_t *>(smem + tabOff + (LEAN ? 0u : p.tableStride + 4u * BLOCK));
     uint32_t curTable = 0xFFFFFFFFu;
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(p.queue, 1u);
@@ -102,24 +105,24 @@ __global__ void __launch_bounds__(BLOCK) batched_kernel(const BatchParams p)
         const BatchItem item = p.items[it];
         const BatchExp e = p.exps[item.exp];
         const BatchTable bt = p.tables[e.table];
-        if (e.table != curTable) {
+        if (!LEAN && e.table != curTable) {
             // The previous item ended with a block barrier, so no lane still
             // reads the old table.
             for (int i = threadIdx.x; i < bt.size; i += BLOCK) smem[tabOff + i] = p.tableData[bt.offset + i];
             __syncthreads();
             curTable = e.table;
         }
-        const TableView tv { smem + tabOff, bt.size, bt.minK, bt.maxK, bt.step };
+        const TableView tv { LEAN ? p.tableData + bt.offset : smem + tabOff, bt.size, bt.minK, bt.maxK, bt.step };
         LseAcc acc;
         lse_init(acc);
         uint32_t clamped = 0;
         switch (e.G) {
-        case 2: run_item<2, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
-        case 4: run_item<4, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
-        case 8: run_item<8, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
-        case 16: run_item<16, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
-        case 32: run_item<32, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
-        default: run_item<64, BLOCK>(p, e, item, smem, tv, acc, clamped); break;
+        case 2: run_item<2, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        case 4: run_item<4, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        case 8: run_item<8, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        case 16: run_item<16, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        case 32: run_item<32, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        default: run_item<64, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
         }
         acc.nClamped = clamped;
         block_reduce_lse(acc, smem, p.itemPartials + it);
@@ -138,20 +141,31 @@ __global__ void __launch_bounds__(256) batch_finalize_kernel(const BlockPartial 
 
 bool batched_block_supported(uint32_t block) { return block == 256 || block == 512; }
 
-hipError_t batched_prepare(uint32_t block, uint32_t lds)
+static const void *batched_fn(uint32_t block, bool lean)
 {
-    const void *fn = block == 256 ? (const void *)batched_kernel<256> : (const void *)batched_kernel<512>;
+    if (block == 256) return lean ? (const void *)batched_kernel<256, true> : (const void *)batched_kernel<256, false>;
+    if (block == 512) return lean ? (const void *)batched_kernel<512, true> : (const void *)batched_kernel<512, false>;
+    return nullptr;
+}
+
+hipError_t batched_prepare(uint32_t block, bool lean, uint32_t lds)
+{
+    const void *fn = batched_fn(block, lean);
+    if (!fn) return hipErrorInvalidValue;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, uint32_t lds, const uint32_t *expItemStart,
-      const uint32_t *expItems, const double *absorbLog2, uint32_t numExps, FinalOut *outs, hipStream_t s)
+hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, bool lean, uint32_t lds,
+      const uint32_t *expItemStart, const uint32_t *expItems, const double *absorbLog2, uint32_t numExps,
+      FinalOut *outs, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, p.queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (block == 256) hipLaunchKernelGGL(batched_kernel<256>, dim3(grid), dim3(256), lds, s, p);
-    else if (block == 512) hipLaunchKernelGGL(batched_kernel<512>, dim3(grid), dim3(512), lds, s, p);
+    if (block == 256 && lean) hipLaunchKernelGGL((batched_kernel<256, true>), dim3(grid), dim3(256), lds, s, p);
+    else if (block == 256) hipLaunchKernelGGL((batched_kernel<256, false>), dim3(grid), dim3(256), lds, s, p);
+    else if (block == 512 && lean) hipLaunchKernelGGL((batched_kernel<512, true>), dim3(grid), dim3(512), lds, s, p);
+    else if (block == 512) hipLaunchKernelGGL((batched_kernel<512, false>), dim3(grid), dim3(512), lds, s, p);
     else return hipErrorInvalidValue;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(numExps), dim3(256), 0, s, p.itemPartials, expItemStart, expItems,

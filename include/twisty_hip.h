@@ -103,9 +103,12 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
       float *perJointCurvature);
 
 /* Launch schedule of twisty_hip_run: lanesPerPath in {1,2,4,8,16,32,64},
- * blockSize a multiple of 64, blocksPerCU >= 1; 0 for any value selects the
- * built-in choice for the current curve.  Replaces the compile-time
- * PerturbGridSize / PerturbBlockSize (...Optimized_GPU.cu:45-47). */
+ * blockSize one of the compiled sizes for that lanesPerPath, blocksPerCU >= 1;
+ * 0 for any value selects the built-in choice for the current curve.  For
+ * lanesPerPath 4, blockSize 768 is the lean kernel (table read from global
+ * memory, draw records exchanged between lanes without LDS), the built-in
+ * choice whenever it keeps more lanes per CU resident.  Replaces the
+ * compile-time PerturbGridSize / PerturbBlockSize (...Optimized_GPU.cu:45-47). */
 int twisty_hip_set_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
       uint32_t blocksPerCU);
 

@@ -278,6 +278,38 @@ def test_full_size_c2_properties(hip_device, oracle):
     assert abs(10.0 ** (mg - mr) - 1.0) <= 3.0 * math.hypot(sg, sr)
 
 
+LONGEST = cases.Case("N3072", N=3072, L=3.0, K=30, sigma=0.05, bins=2048, numStepsInt=2000)
+TOO_LONG = cases.Case("N3136", N=3136, L=3.0, K=30, sigma=0.05, bins=2048, numStepsInt=2000)
+
+
+def test_size_limits(hip_device, oracle):
+    """The longest curve whose state fits one block's LDS (N = 3072: 64 lanes
+    per path, 48 segment rows per lane, block 256) runs and matches the oracle
+    path by path; one more row per lane (N = 3136) is reported as unsupported
+    before any launch.  No paths, no perturbations and more than 2^40 paths
+    are argument errors."""
+    table, _ = cases.setup(LONGEST, oracle)
+    n = 64
+    with cases.hip_context(LONGEST, oracle) as ctx:
+        sched = ctx.get_schedule()
+        _report("N3072 schedule", **sched)
+        assert sched["lanes_per_path"] == 64
+        gpu = ctx.run(n, LONGEST.K, seed=LONGEST.seed, per_path=True, per_joint=True)
+        for bad in [(0, 1), (16, 0), ((1 << 40) + 1, 1)]:
+            with pytest.raises(capi.TwistyHipError) as e:
+                ctx.run(*bad)
+            assert e.value.code == capi.ERR_ARG
+    ref, ref32 = both_refs(LONGEST, oracle, n)
+    check_against_oracle(LONGEST, oracle, gpu, ref, ref32, table, "N3072 64 paths")
+    with cases.hip_context(TOO_LONG, oracle) as ctx:
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.get_schedule()
+        assert e.value.code == capi.ERR_UNSUPPORTED
+        with pytest.raises(capi.TwistyHipError) as e:
+            ctx.run(16, 1)
+        assert e.value.code == capi.ERR_UNSUPPORTED
+
+
 def test_error_codes(hip_device, oracle):
     case = cases.SMALL
     with capi.Context(0) as ctx:

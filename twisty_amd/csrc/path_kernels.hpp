@@ -14,14 +14,20 @@ namespace twisty_hip {
 // scratch row: they read zeros, add nothing to the axis and write zeros back,
 // so every load and store is unconditional with a compile-time offset.
 // scrB[w] is the byte address that lands on the scratch row after slot w's
-// offset is added.  (Skipping the last slot behind a wave-uniform branch when
-// no path of the wave needs it measured slower: the compiler copies the window
-// registers around the branch.)
-template<int G, int BLOCK>
+// offset is added.  With SPLIT the last slot, needed only by a lane holding
+// WMAX segments of the sub-chain (diff = 25 for G = 4: one path in 24), sits
+// behind a wave-uniform branch taken when some lane of the wave needs it; it
+// is loaded and added to the axis last (the same summation order as an
+// unconditional slot) and reloaded for the rotation, so no window register
+// stays live across the branch.  That pays at 12 waves per CU (lean forward:
+// C2 8.3 -> 8.05 ms) and costs 1-4% at 8 (table-in-LDS forward, batched), so
+// only the lean kernels use it.
+template<int G, int BLOCK, bool SPLIT = false>
 __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
 {
     constexpr int WMAX = (25 + G - 1) / G;       // sub-chain slots per lane (diff <= 25)
+    constexpr int WMAIN = (SPLIT && WMAX > 1) ? WMAX - 1 : WMAX;  // slots every perturbation runs
     constexpr uint32_t SLOTB = 12u * BLOCK;      // bytes per slot row
     constexpr uint32_t COMPB = 4u * BLOCK;       // bytes between components
     const uint32_t diff = rec.packed & 0xFFFFu;
@@ -32,10 +38,10 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
     const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
     const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
     const uint32_t baseB = (i0 / G) * SLOTB + laneB;
-    uint32_t addr[WMAX];
-    float dx[WMAX], dy[WMAX], dz[WMAX];
+    uint32_t addr[WMAIN];
+    float dx[WMAIN], dy[WMAIN], dz[WMAIN];
 #pragma unroll
-    for (int w = 0; w < WMAX; ++w) {
+    for (int w = 0; w < WMAIN; ++w) {
         addr[w] = ((uint32_t)w < cnt) ? baseB : scrB[w];
         const uint32_t a = addr[w] + w * SLOTB;
         dx[w] = lds_ld(a);
@@ -44,10 +50,18 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
     }
     float ax = dx[0], ay = dy[0], az = dz[0];
 #pragma unroll
-    for (int w = 1; w < WMAX; ++w) {
+    for (int w = 1; w < WMAIN; ++w) {
         ax += dx[w];
         ay += dy[w];
         az += dz[w];
+    }
+    // wave-uniform: some lane of the wave holds WMAX segments of its sub-chain
+    const bool last = WMAIN < WMAX && __builtin_amdgcn_ballot_w64(cnt > (uint32_t)WMAIN) != 0;
+    const uint32_t lastA = ((uint32_t)WMAIN < cnt ? baseB : scrB[WMAX - 1]) + (uint32_t)WMAIN * SLOTB;
+    if (last) {
+        ax += lds_ld(lastA);
+        ay += lds_ld(lastA + COMPB);
+        az += lds_ld(lastA + 2 * COMPB);
     }
     if constexpr (G > 1) {
         ax = group_sum<G>(ax);
@@ -59,12 +73,18 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
     const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
     const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec.c, rec.s, rec.C);
 #pragma unroll
-    for (int w = 0; w < WMAX; ++w) {
+    for (int w = 0; w < WMAIN; ++w) {
         const float x = dx[w], y = dy[w], z = dz[w];
         const uint32_t a = addr[w] + w * SLOTB;
         lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
         lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
         lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
+    }
+    if (last) {
+        const float x = lds_ld(lastA), y = lds_ld(lastA + COMPB), z = lds_ld(lastA + 2 * COMPB);
+        lds_st(lastA, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
+        lds_st(lastA + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
+        lds_st(lastA + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
     }
 }
 
@@ -105,11 +125,11 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
 #pragma unroll
                 for (int kk = 0; kk < G; ++kk) {
                     if ((uint32_t)kk >= kcount) break;
-                    perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
+                    perturb_one<G, BLOCK, true>(laneB, j, scrB, recs[kk]);
                 }
             } else {
                 for (uint32_t kk = 0; kk < kcount; ++kk)
-                    perturb_one<G, BLOCK>(laneB, j, scrB, bperm_rec(firstB + 4 * (int)kk, mine));
+                    perturb_one<G, BLOCK, true>(laneB, j, scrB, bperm_rec(firstB + 4 * (int)kk, mine));
             }
         } else {
             if (kb + j < K) {

@@ -19,9 +19,9 @@ namespace twisty_hip {
 // behind a wave-uniform branch taken when some lane of the wave needs it; it
 // is loaded and added to the axis last (the same summation order as an
 // unconditional slot) and reloaded for the rotation, so no window register
-// stays live across the branch.  That pays at 12 waves per CU (lean forward:
-// C2 8.3 -> 8.05 ms) and costs 1-4% at 8 (table-in-LDS forward, batched), so
-// only the lean kernels use it.
+// stays live across the branch.  That pays at 12 waves per CU (lean forward,
+// C2: about 2% faster) and costs 1-4% at 8 (table-in-LDS forward, batched),
+// so only the lean kernels use it.
 template<int G, int BLOCK, bool SPLIT = false>
 __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)

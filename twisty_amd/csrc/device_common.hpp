@@ -68,6 +68,35 @@ __device__ __forceinline__ DrawRec pack_draw(const Draw &d)
     return r;
 }
 
+// sin(pi x) and cos(pi x) for |x| < 1/2, the range of the half-angle draw.
+// For |x| > 1/4 the argument becomes y = 1/2 - |x| (exact in float) and sin
+// and cos swap, so both polynomials only cover |y| <= 1/4: sin(pi y) =
+// y P(y^2) with P of degree 3 and cos(pi y) = Q(y^2) with Q of degree 4, the
+// coefficients a relative-weighted least-squares fit at 4000 Chebyshev nodes
+// of [0, 1/4] rounded to float, evaluated by Horner's rule with FMA.  Over
+// every x the draw produces (odd multiples of 2^-25) the error is at most 1.7
+// ulp (mean 0.35 ulp) against FP64 sin/cos.  About 15 VALU instead of the
+// general sincospif's ~40 with its full-range reduction.
+__device__ __forceinline__ void sincospi_half(float x, float &s, float &c)
+{
+    const float a = fabsf(x);
+    const bool swap = a > 0.25f;
+    const float y = swap ? 0.5f - a : a;
+    const float t = y * y;
+    float p = -0x1.2db8e8p-1f;
+    p = fmaf(p, t, 0x1.465f8ep+1f);
+    p = fmaf(p, t, -0x1.4abbbcp+2f);
+    p = fmaf(p, t, 0x1.921fb6p+1f);
+    const float sy = p * y;
+    float q = 0x1.d9cce2p-3f;
+    q = fmaf(q, t, -0x1.55c5b0p+0f);
+    q = fmaf(q, t, 0x1.03c1dcp+2f);
+    q = fmaf(q, t, -0x1.3bd3ccp+2f);
+    const float cy = fmaf(q, t, 1.0f);
+    s = copysignf(swap ? cy : sy, x);
+    c = swap ? sy : cy;
+}
+
 __device__ __forceinline__ Draw make_draw(uint64_t seed, uint32_t stream, uint64_t path, uint32_t k, uint32_t N,
       uint32_t maxDiff)
 {
@@ -78,7 +107,7 @@ __device__ __forceinline__ Draw make_draw(uint64_t seed, uint32_t stream, uint64
     d.left = 1u + __umulhi(r.y, N - d.diff - 1u);
     const int32_t odd = (int32_t)(((r.z >> 8) << 1) | 1u) - (int32_t)(1u << 24);
     const float x = (float)odd * 0x1p-25f;
-    sincospif(x, &d.sh, &d.ch);
+    sincospi_half(x, d.sh, d.ch);
     return d;
 }
 

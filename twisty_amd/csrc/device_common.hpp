@@ -97,6 +97,20 @@ __device__ __forceinline__ void sincospi_half(float x, float &s, float &c)
     c = swap ? sy : cy;
 }
 
+// sin(2 pi u) and cos(2 pi u) for u in [0, 1) a multiple of 2^-24 (a full
+// turn): x = 2u - 1 and, past |x| = 1/2, z = 1 - |x| are exact, so the error
+// is that of sincospi_half; (sin, cos)(2 pi u) = -(sin, cos)(pi x).
+__device__ __forceinline__ void sincospi_turn(float u, float &s, float &c)
+{
+    const float x = 2.0f * u - 1.0f;
+    const float a = fabsf(x);
+    const bool far = a > 0.5f;
+    float sz, cz;
+    sincospi_half(far ? 1.0f - a : a, sz, cz);
+    s = -copysignf(sz, x);
+    c = far ? cz : -cz;
+}
+
 __device__ __forceinline__ Draw make_draw(uint64_t seed, uint32_t stream, uint64_t path, uint32_t k, uint32_t N,
       uint32_t maxDiff)
 {

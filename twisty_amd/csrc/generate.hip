@@ -85,7 +85,7 @@ __device__ F3 resolve_two(F3 L, F3 R, float ds, const Words &g)
     if (fabsf(2.0f * ds - d) < 0.001f) return (L + R) * 0.5f;
     if (2.0f * ds < d) return nan3();
     float st, ct;
-    sincospif(2.0f * g.u1, &st, &ct);
+    sincospi_turn(g.u1, st, ct);
     if (d < 0.001f) {
         float c, s;
         phi_of(g.u0, 1.0f, c, s);
@@ -106,7 +106,7 @@ __device__ F3 resolve_three_first(F3 L, F3 R, float ds, const Words &g)
     if (fabsf(3.0f * ds - d) < 0.001f) return L + dv * (1.0f / 3.0f);
     if (3.0f * ds < d) return nan3();
     float st, ct;
-    sincospif(2.0f * g.u1, &st, &ct);
+    sincospi_turn(g.u1, st, ct);
     if (d < 0.001f) {
         float c, s;
         phi_of(g.u0, 1.0f, c, s);
@@ -143,7 +143,7 @@ __device__ F3 resolve_even_centre(uint32_t n, F3 L, F3 R, float ds, const Words 
     const float rps = ds * (float)(n / 2);
     if (2.0f * rps < d) return nan3();
     float st, ct;
-    sincospif(2.0f * g.u1, &st, &ct);
+    sincospi_turn(g.u1, st, ct);
     const F3 z = unit(dv);
     F3 x, y;
     frame(z, 0.999f, x, y);

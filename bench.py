@@ -59,6 +59,17 @@ def traffic_from_profiles():
         return json.load(f).get("bytes_per_launch")
 
 
+def valu_issue_from_profiles():
+    """VALU issue utilisation of the forward kernel (SQ_INSTS_VALU x 4 cycles /
+    (kernel cycles x SIMDs), rocprofv3 --pmc, profiles/), or None.  The kernel's
+    bound is instruction issue, so this is how close it runs to it."""
+    p = os.path.join(ROOT, "profiles", "forward_c2_valu_counters.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("valu_issue_utilisation")
+
+
 def cpu_baseline(table, pos, w, target_seconds):
     """Reference CPU path (oracle/, FP32 positions + FP64 weights) on the host."""
     from oracle import pyoracle
@@ -324,7 +335,8 @@ def main(argv=None):
                    "schedule": sched},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(),
-                     "kernel": "forward_kernel", "kernel_ms": fwd_ms, "flops_per_path": F},
+                     "kernel": "forward_kernel", "kernel_ms": fwd_ms, "flops_per_path": F,
+                     "valu_issue_utilisation": valu_issue_from_profiles()},
         "estimate": {"log10_avg_path_weight": mean_l, "rel_standard_error": rel_se,
                      "num_valid": res["num_valid"], "num_invalid": res["num_invalid"]},
         "fwd_adjoint_ms": None,

@@ -6,7 +6,7 @@
 
 `kernels` prints per-kernel dispatch count and mean/min/max duration.
 `counters` prints the mean per dispatch of every counter collected for the
-matching kernel, plus VALU and LDS utilisation derived from them
+matching kernel (optionally also as JSON), plus VALU and LDS utilisation derived from them
 (SQ_INSTS_VALU wave instructions x 4 cycles / (GRBM_GUI_ACTIVE per XCD x
 SIMDs), SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS).
 `traffic` prints FETCH_SIZE and WRITE_SIZE per dispatch of the matching kernel,
@@ -115,13 +115,17 @@ def main(argv=None):
     c = sub.add_parser("counters")
     c.add_argument("db")
     c.add_argument("--match", required=True)
+    c.add_argument("--json")
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         text, _ = kernels(a.db, a.skip_first, a.match)
         print(text)
     elif a.cmd == "counters":
-        text, _ = counters(a.db, a.match)
+        text, res = counters(a.db, a.match)
         print(text)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump({"kernel_match": a.match, **res}, f, indent=1)
     else:
         text, res = traffic(a.fetch, a.write, a.calib_fetch, a.calib_write, a.calib_bytes, a.match)
         print(text)

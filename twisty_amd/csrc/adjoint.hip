@@ -42,11 +42,16 @@ namespace twisty_hip {
 // redirection as the forward's perturb_one; the adjoint rows sit lamB bytes
 // after the segment rows).  Every slot of a lane past its count lands on the
 // same scratch address; they load the same values and store the same values.
+// The last slot (needed only by a lane holding WMAX segments of the sub-chain,
+// diff = 25 for G = 8) runs behind a wave-uniform branch taken when some lane
+// of the wave needs it, as in the forward; its terms are added after the
+// other slots', in the same order as an unconditional slot.
 template<int G, int BLOCK>
 __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], const DrawRec &rec)
 {
-    constexpr int WMAX = (25 + G - 1) / G;
+    constexpr int WALL = (25 + G - 1) / G;
+    constexpr int WMAX = WALL > 1 ? WALL - 1 : WALL;  // slots every perturbation runs
     constexpr uint32_t SLOTB = 12u * BLOCK;
     constexpr uint32_t COMPB = 4u * BLOCK;
     const uint32_t diff = rec.packed & 0xFFFFu;
@@ -74,6 +79,21 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
         ax += dx[w];
         ay += dy[w];
         az += dz[w];
+    }
+    // wave-uniform: some lane of the wave holds WALL segments of its sub-chain
+    const bool last = WMAX < WALL && __builtin_amdgcn_ballot_w64(cnt > (uint32_t)WMAX) != 0;
+    const uint32_t lastA = ((uint32_t)WMAX < cnt ? baseB : scrB[WALL - 1]) + (uint32_t)WMAX * SLOTB;
+    float ex = 0.f, ey = 0.f, ez = 0.f, mx = 0.f, my = 0.f, mz = 0.f;  // the last slot's d and lambda
+    if (last) {
+        ex = lds_ld(lastA);
+        ey = lds_ld(lastA + COMPB);
+        ez = lds_ld(lastA + 2 * COMPB);
+        mx = lds_ld(lastA + lamB);
+        my = lds_ld(lastA + lamB + COMPB);
+        mz = lds_ld(lastA + lamB + 2 * COMPB);
+        ax += ex;
+        ay += ey;
+        az += ez;
     }
     if constexpr (G > 1) {
         ax = group_sum<G>(ax);
@@ -105,6 +125,20 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
         by = fmaf(ly[w], pn, fmaf(py[w], ln, by));
         bz = fmaf(lz[w], pn, fmaf(pz[w], ln, bz));
     }
+    float qx_ = 0.f, qy_ = 0.f, qz_ = 0.f;  // the last slot's d_pre
+    if (last) {
+        qx_ = fmaf(R.m00, ex, fmaf(R.m10, ey, R.m20 * ez));
+        qy_ = fmaf(R.m01, ex, fmaf(R.m11, ey, R.m21 * ez));
+        qz_ = fmaf(R.m02, ex, fmaf(R.m12, ey, R.m22 * ez));
+        cx = fmaf(my, qz_, fmaf(-mz, qy_, cx));
+        cy = fmaf(mz, qx_, fmaf(-mx, qz_, cy));
+        cz = fmaf(mx, qy_, fmaf(-my, qx_, cz));
+        const float pn = fmaf(qx_, nx, fmaf(qy_, ny, qz_ * nz));
+        const float ln = fmaf(mx, nx, fmaf(my, ny, mz * nz));
+        bx = fmaf(mx, pn, fmaf(qx_, ln, bx));
+        by = fmaf(my, pn, fmaf(qy_, ln, by));
+        bz = fmaf(mz, pn, fmaf(qz_, ln, bz));
+    }
     float hx = fmaf(-rec.s, cx, rec.C * bx);
     float hy = fmaf(-rec.s, cy, rec.C * by);
     float hz = fmaf(-rec.s, cz, rec.C * bz);
@@ -133,6 +167,17 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
         lds_st(a + lamB, qx);
         lds_st(a + lamB + COMPB, qy);
         lds_st(a + lamB + 2 * COMPB, qz);
+    }
+    if (last) {
+        const float qx = fmaf(R.m00, mx, fmaf(R.m10, my, fmaf(R.m20, mz, dax)));
+        const float qy = fmaf(R.m01, mx, fmaf(R.m11, my, fmaf(R.m21, mz, day)));
+        const float qz = fmaf(R.m02, mx, fmaf(R.m12, my, fmaf(R.m22, mz, daz)));
+        lds_st(lastA, qx_);
+        lds_st(lastA + COMPB, qy_);
+        lds_st(lastA + 2 * COMPB, qz_);
+        lds_st(lastA + lamB, qx);
+        lds_st(lastA + lamB + COMPB, qy);
+        lds_st(lastA + lamB + 2 * COMPB, qz);
     }
 }
 

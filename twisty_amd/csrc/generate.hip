@@ -32,6 +32,14 @@ struct F3 {
     float x, y, z;
 };
 
+// Reciprocal, reciprocal square root, quotient and square root at the
+// hardware instructions' precision (about 1 ulp): the sphere-intersection
+// geometry tolerates it, and the IEEE-correct division and square root take
+// several times the instructions.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float hsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ F3 f3(float x, float y, float z) { return { x, y, z }; }
 __device__ __forceinline__ F3 operator+(F3 a, F3 b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
 __device__ __forceinline__ F3 operator-(F3 a, F3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
@@ -41,7 +49,7 @@ __device__ __forceinline__ F3 cross3(F3 a, F3 b)
 {
     return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
 }
-__device__ __forceinline__ F3 unit(F3 a) { return a * (1.0f / sqrtf(dot3(a, a))); }
+__device__ __forceinline__ F3 unit(F3 a) { return a * __builtin_amdgcn_rsqf(dot3(a, a)); }
 __device__ __forceinline__ F3 nan3() { return { __int_as_float(0x7fc00000), __int_as_float(0x7fc00000), __int_as_float(0x7fc00000) }; }
 
 struct Words {
@@ -70,7 +78,7 @@ __device__ __forceinline__ void phi_of(float u, float vmax, float &c, float &s)
 {
     const float v = u * vmax;
     c = fmaf(-2.0f, v, 1.0f);
-    s = 2.0f * sqrtf(fmaxf(0.0f, v * (1.0f - v)));
+    s = 2.0f * hsqrt(fmaxf(0.0f, v * (1.0f - v)));
 }
 
 __device__ __forceinline__ F3 spherical(F3 x, F3 y, F3 z, float c, float s, float ct, float st)
@@ -81,7 +89,7 @@ __device__ __forceinline__ F3 spherical(F3 x, F3 y, F3 z, float c, float s, floa
 __device__ F3 resolve_two(F3 L, F3 R, float ds, const Words &g)
 {
     const F3 dv = R - L;
-    const float d = sqrtf(dot3(dv, dv));
+    const float d = hsqrt(dot3(dv, dv));
     if (fabsf(2.0f * ds - d) < 0.001f) return (L + R) * 0.5f;
     if (2.0f * ds < d) return nan3();
     float st, ct;
@@ -91,18 +99,18 @@ __device__ F3 resolve_two(F3 L, F3 R, float ds, const Words &g)
         phi_of(g.u0, 1.0f, c, s);
         return L + f3(s * ct, s * st, c) * ds;
     }
-    const F3 n = dv * (1.0f / d);
+    const F3 n = dv * rcp(d);
     const F3 other = fabsf(n.x) >= 0.99f ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
     const F3 m = unit(cross3(n, other));
     const float hd = 0.5f * d;
-    const float h = ds > hd ? sqrtf((ds - hd) * (ds + hd)) : 0.0f;
+    const float h = ds > hd ? hsqrt((ds - hd) * (ds + hd)) : 0.0f;
     return (L + R) * 0.5f + (m * ct + cross3(n, m) * st) * h;
 }
 
 __device__ F3 resolve_three_first(F3 L, F3 R, float ds, const Words &g)
 {
     const F3 dv = R - L;
-    const float d = sqrtf(dot3(dv, dv));
+    const float d = hsqrt(dot3(dv, dv));
     if (fabsf(3.0f * ds - d) < 0.001f) return L + dv * (1.0f / 3.0f);
     if (3.0f * ds < d) return nan3();
     float st, ct;
@@ -112,7 +120,7 @@ __device__ F3 resolve_three_first(F3 L, F3 R, float ds, const Words &g)
         phi_of(g.u0, 1.0f, c, s);
         return L + f3(s * ct, s * st, c) * ds;
     }
-    const F3 z = dv * (1.0f / d);
+    const F3 z = dv * rcp(d);
     F3 x, y;
     frame(z, 0.999f, x, y);
     const float r1 = ds, r2 = 2.0f * ds;
@@ -120,15 +128,15 @@ __device__ F3 resolve_three_first(F3 L, F3 R, float ds, const Words &g)
     if (d + r1 < r2) {
         vmax = 1.0f;
     } else {
-        const float h = 0.5f + (r1 * r1 - r2 * r2) / (2.0f * d * d);
+        const float h = 0.5f + qdiv(r1 * r1 - r2 * r2, 2.0f * d * d);
         const float t = fmaf(-h * d, h * d, r1 * r1);
         float a;
         if (fabsf(t) < 0.001f) a = 0.0f;
         else if (t < 0.0f) return nan3();
-        else a = sqrtf(t);
-        const float sE = fminf(1.0f, a / r1);
-        const float cE = sqrtf((1.0f - sE) * (1.0f + sE));
-        vmax = (h * d < 0.0f) ? 0.5f + 0.5f * cE : 0.5f * sE * sE / (1.0f + cE);
+        else a = hsqrt(t);
+        const float sE = fminf(1.0f, qdiv(a, r1));
+        const float cE = hsqrt((1.0f - sE) * (1.0f + sE));
+        vmax = (h * d < 0.0f) ? 0.5f + 0.5f * cE : qdiv(0.5f * sE * sE, 1.0f + cE);
     }
     float c, s;
     phi_of(g.u0, vmax, c, s);
@@ -138,7 +146,7 @@ __device__ F3 resolve_three_first(F3 L, F3 R, float ds, const Words &g)
 __device__ F3 resolve_even_centre(uint32_t n, F3 L, F3 R, float ds, const Words &g)
 {
     const F3 dv = R - L;
-    const float d = sqrtf(dot3(dv, dv));
+    const float d = hsqrt(dot3(dv, dv));
     if (fabsf((float)n * ds - d) < 0.001f) return (L + R) * 0.5f;
     const float rps = ds * (float)(n / 2);
     if (2.0f * rps < d) return nan3();
@@ -152,10 +160,10 @@ __device__ F3 resolve_even_centre(uint32_t n, F3 L, F3 R, float ds, const Words 
         phi_of(g.u0, 1.0f, c, s);
         return L + spherical(x, y, z, c, s, ct, st) * (rps * cbrtf(g.u3));
     }
-    const float vmax = 0.5f * (1.0f - d / (2.0f * rps));
+    const float vmax = 0.5f * (1.0f - qdiv(d, 2.0f * rps));
     float c, s;
     phi_of(g.u0, vmax, c, s);
-    const float minPct = (0.5f * d / c) / rps;
+    const float minPct = qdiv(qdiv(0.5f * d, c), rps);
     const float radius = rps * cbrtf(fmaf(1.0f - minPct, g.u3, minPct));
     if (g.coin == 0u) return L + spherical(x, y, z, c, s, ct, st) * radius;
     return R + spherical(x, y, z, -c, s, ct, st) * radius;

@@ -75,6 +75,13 @@ float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegmen
 /* log10 of NormalizerStuff::Norm (PathWeightUtils.cpp:721-752). */
 double twisty_host_norm_log10(uint32_t numSegments, float ds, const float boundary[13]);
 
+/* The same with the node counts of the nested N = 5, 6 quadrature as arguments:
+ * outerSteps for the K(N) rule, innerSteps for each F(N, M) rule (the reference
+ * hard-codes 10000 and 1000, PathWeightUtils.cpp:694, 665; twisty_host_norm_log10
+ * uses those).  Either count 0 returns NaN. */
+double twisty_host_norm_log10_steps(uint32_t numSegments, float ds, const float boundary[13], uint32_t outerSteps,
+      uint32_t innerSteps);
+
 /* Path-batch export of numPaths paths ((numSegments+1)*3 floats each, and their
  * log10 weights, NaN = invalid and skipped) under <experimentDir>/GeneratedCurves:
  * BoundaryConditions.bcf, <prepend>Paths_Metadata.pmd, <prepend>Paths_Binary.pbd,

@@ -528,8 +528,10 @@ inline long double LogSumExp10(long double a, long double b)
 // macro calls F7, :703-705); everything else returns 1.0 (:745-750).
 // The nested form is restated with its step counts as parameters so tests can
 // run it small; the reference values are the defaults.
-// Pinned by: K4 closed form; the nested sums against a direct evaluation at
-// reduced step counts.  Boost big-float accumulation: parity unpinned.
+// Pinned by: the K4 closed form; for N = 5, 6, convergence at first order in
+// the node spacing to (4 pi)^5 p_5(z) / ds^3, p_5 the closed-form 5-step
+// random-flight density (each nesting level is a sphere convolution), with
+// K5 == K6.  Boost big-float accumulation (long double here): parity unpinned.
 // ---------------------------------------------------------------------------
 inline long double K4(float z, float ds)  // PathWeightUtils.cpp:643-651
 {

@@ -98,6 +98,36 @@ def test_norm(oracle):
     assert math.isclose(host.norm_log10(4, 0.4, bc), math.log10(ref), rel_tol=1e-9)
 
 
+def test_norm_nested_n5_n6_against_oracle(oracle):
+    """The host's N = 5, 6 nested quadrature (thread-split outer rule) equals the
+    oracle's at reduced node counts, and K5 = K6 as in the reference's K(N)
+    macro (PathWeightUtils.cpp:689-711)."""
+    ds = 0.4
+    for D in (1.0, 1.8, 2.6):
+        bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, D), (0, 0, 1), 2.0)
+        bc12 = np.array([0, 0, 0, 0, 0, 1, 0, 0, D, 0, 0, 1], dtype=np.float64)
+        for outer, inner in ((60, 60), (97, 41)):
+            ref = math.log10(oracle.lib().oracle_norm(5, ds, bc12, 2.0, outer, inner))
+            for N in (5, 6):
+                assert abs(host.norm_log10(N, ds, bc, outer, inner) - ref) < 1e-13, (D, N, outer, inner)
+    assert math.isnan(host.norm_log10(5, ds, bc, 0, 10))
+
+
+def test_norm_n5_reference_node_counts_near_closed_form():
+    """At the reference's 10000 x 1000 x 1000 nodes the N = 5 normaliser is
+    (4 pi)^5 p_5(z) / ds^3 (Treloar's 5-step random-flight density) up to the
+    rule's first-order error (0.46% here)."""
+    from tests.test_oracle import random_flight_density
+
+    ds, D = 0.4, 1.4
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, D), (0, 0, 1), 2.0)
+    f = np.float32
+    z = float(abs(f(f(D) * (f(1.0) / f(ds))) - f(2.0)))
+    closed = math.log10((4 * math.pi) ** 5 * random_flight_density(5, z) / ds ** 3)
+    v = host.norm_log10(5, ds, bc)
+    assert -0.01 < 10 ** (v - closed) - 1 < 0.0, (v, closed)
+
+
 def _config(**over):
     cfg = {"experiment": {
         "experimentParams": {

@@ -337,6 +337,70 @@ def test_norm_k4_and_default(oracle):
         assert oracle.lib().oracle_norm(N, 0.05, bc, 1.0, 2, 2) == 1.0
 
 
+def random_flight_density(n, r):
+    """3-D density of the sum of n independent uniform unit vectors at distance
+    r (Rayleigh 1919; Treloar 1946): a piecewise polynomial in r over r,
+    1 / (2^(n+1) pi (n-2)! r) sum_k (-1)^k C(n,k) (n - r - 2k)^(n-2), 0 < r < n."""
+    if r >= n:
+        return 0.0
+    s = sum((-1) ** k * math.comb(n, k) * (n - r - 2 * k) ** (n - 2) for k in range(int((n - r) // 2) + 1))
+    return s / (2 ** (n + 1) * math.pi * math.factorial(n - 2) * r)
+
+
+def test_random_flight_density_closed_form():
+    """The closed form used below, pinned by what it must satisfy: unit mass for
+    every n, the two-step density 1/(8 pi r) on (0, 2) (|s1 + s2|^2 = 2 + 2 cos t
+    with cos t uniform), and p_{n+1} = p_n convolved with the unit sphere's
+    uniform measure (checked numerically at n = 3 -> 4)."""
+    for n in (2, 3, 4, 5, 6):
+        r = np.linspace(1e-6, n, 400001)
+        mass = np.trapezoid([4 * math.pi * x * x * random_flight_density(n, x) for x in r[::50]], r[::50])
+        assert abs(mass - 1.0) < 2e-4, (n, mass)
+    for r in (0.1, 0.7, 1.3, 1.99):
+        assert math.isclose(random_flight_density(2, r), 1.0 / (8 * math.pi * r), rel_tol=1e-12)
+    # sphere convolution: (f * sigma)(w) / (4 pi) = 1/2 int_{-1}^{1} f(sqrt(w^2 + 1 - 2 w u)) du
+    u = np.linspace(-1.0, 1.0, 200001)
+    for w in (0.3, 1.2, 2.5, 3.4):
+        rr = np.sqrt(w * w + 1.0 - 2.0 * w * u)
+        conv = 0.5 * np.trapezoid([random_flight_density(3, x) for x in rr], u)
+        assert math.isclose(conv, random_flight_density(4, w), rel_tol=2e-4), (w, conv)
+
+
+def _bc_for_z(D):
+    """Boundary (0,0,0) -> (0,0,D), both tangents +z: the normaliser's
+    argument is z = |D/ds - 2| (PathWeightUtils.cpp:730-734), computed in float."""
+    return np.array([0, 0, 0, 0, 0, 1, 0, 0, D, 0, 0, 1], dtype=np.float64)
+
+
+def _z_of(D, ds):
+    f = np.float32
+    return float(abs(f(f(D) * (f(1.0) / f(ds))) - f(1.0) - f(1.0)))
+
+
+def test_norm_nested_quadrature_converges_to_random_flight(oracle):
+    """N = 5, 6 (PathWeightUtils.cpp:653-752).  K4 = 2 pi / (z ds^3) on z < 2 is
+    (4 pi)^2 p_2(z) / ds^3, and each F(N, M) / K(N) level is a left Riemann rule
+    for a convolution with the unit sphere (2 pi int f(|w - s|) d cos).  The
+    K(N) macro nests three levels over K4 for every N, so K5 = K6 ->
+    (4 pi)^5 p_5(z) / ds^3.  The oracle converges to that at first order in the
+    node spacing (the error halves as the node counts double) at every z, and
+    the Richardson combination 2 v(2n) - v(n) lands within 5e-3.  A missing or
+    extra level, a lost 2 pi or a wrong node spacing converges elsewhere."""
+    ds = 0.4
+    L = oracle.lib()
+    for D in (1.0, 1.4, 1.8, 2.2, 2.6):
+        z = _z_of(D, ds)
+        closed = (4 * math.pi) ** 5 * random_flight_density(5, z) / ds ** 3
+        v = {n: L.oracle_norm(5, ds, _bc_for_z(D), 5.0, n, n) for n in (100, 200, 400)}
+        e = {n: (v[n] - closed) / closed for n in v}
+        assert abs(e[400]) < 0.07, (z, e)
+        assert 1.7 < e[200] / e[400] < 2.3 and 1.7 < e[100] / e[200] < 2.3, (z, e)
+        assert abs((2 * v[400] - v[200]) - closed) / closed < 5e-3, (z, e)
+        assert L.oracle_norm(6, ds, _bc_for_z(D), 5.0, 100, 100) == v[100]  # K6 is K5
+    # beyond the reach of five unit steps the density, and the quadrature, vanish
+    assert L.oracle_norm(5, ds, _bc_for_z(3.0), 5.0, 60, 60) == 0.0
+
+
 # ---------------------------------------------------------------------------
 # End to end: an independent NumPy path (Rodrigues on positions, float32 lerp)
 # against the oracle's FP64 path for small N.

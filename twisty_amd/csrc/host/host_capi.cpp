@@ -1,6 +1,7 @@
 // host_capi.cpp -- C ABI of libtwisty_host.so (include/twisty_host.h).
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 
 #include "../../../include/twisty_host.h"
@@ -151,6 +152,13 @@ float twisty_host_minimum_arclength(const float boundary[13], uint32_t numSegmen
 double twisty_host_norm_log10(uint32_t numSegments, float ds, const float boundary[13])
 {
     return PathWeighting::NormLog10((int)numSegments, ds, from13(boundary));
+}
+
+double twisty_host_norm_log10_steps(uint32_t numSegments, float ds, const float boundary[13], uint32_t outerSteps,
+      uint32_t innerSteps)
+{
+    if (outerSteps == 0 || innerSteps == 0) return std::numeric_limits<double>::quiet_NaN();
+    return PathWeighting::NormLog10((int)numSegments, ds, from13(boundary), outerSteps, innerSteps);
 }
 
 int twisty_host_write_path_batches(const char *experimentDir, const char *prepend, const char *experimentName,

@@ -179,8 +179,11 @@ class SimpleWeightLookupTable : public BaseWeightLookupTable {
 };
 
 // NormalizerStuff::Norm (PathWeightUtils.cpp:721-752): K4 for N = 4, the nested
-// quadrature for N = 5, 6 and 1.0 otherwise.
-double NormLog10(int numberOfSegments, float ds, const PerturbUtils::BoundaryConditions &bc);
+// quadrature for N = 5, 6 and 1.0 otherwise.  outerSteps / innerSteps are the
+// node counts of the K(N) and F(N, M) rules; the defaults are the reference's
+// (PathWeightUtils.cpp:694, 665).
+double NormLog10(int numberOfSegments, float ds, const PerturbUtils::BoundaryConditions &bc,
+      uint32_t outerSteps = 10000, uint32_t innerSteps = 1000);
 
 }  // namespace PathWeighting
 

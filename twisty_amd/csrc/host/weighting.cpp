@@ -423,36 +423,39 @@ long double K4(float z, float ds)
 
 float rmag(float x, float z) { return std::sqrt((z * z) + 1.0 - (2.0 * z * x)); }
 
-// F(6,5): 1000-step rule over F5 = K4(rmag).
-long double F6(float x, float z, float ds)
+// F(6,5) (PathWeightUtils.cpp:659-687, numSteps 1000 there): left Riemann
+// rule over F5 = K4(rmag), i.e. K4 convolved with the unit sphere.
+long double F6(float x, float z, float ds, uint32_t steps)
 {
     const float r = rmag(x, z);
-    const float dx = (1.0f - -1.0f) / 1000;
-    long double acc = 0.0;
-    for (uint32_t i = 0; i < 1000; ++i) acc += K4(rmag(-1.0f + i * dx, r), ds);
-    return acc * ((2.0 * TwistyPi) * dx);
-}
-
-long double F7(float x, float z, float ds)
-{
-    const float r = rmag(x, z);
-    const float dx = (1.0f - -1.0f) / 1000;
-    long double acc = 0.0;
-    for (uint32_t i = 0; i < 1000; ++i) acc += F6(-1.0f + i * dx, r, ds);
-    return acc * ((2.0 * TwistyPi) * dx);
-}
-
-// K(N) macro: 10000-step rule over F7 for every order (PathWeightUtils.cpp:689-711).
-long double KMacro(float z, float ds)
-{
-    const uint32_t steps = 10000;
     const float dx = (1.0f - -1.0f) / steps;
-    const int threads = hardware_threads();
+    long double acc = 0.0;
+    for (uint32_t i = 0; i < steps; ++i) acc += K4(rmag(-1.0f + i * dx, r), ds);
+    return acc * ((2.0 * TwistyPi) * dx);
+}
+
+// F(7,6): the same rule over F6.
+long double F7(float x, float z, float ds, uint32_t steps)
+{
+    const float r = rmag(x, z);
+    const float dx = (1.0f - -1.0f) / steps;
+    long double acc = 0.0;
+    for (uint32_t i = 0; i < steps; ++i) acc += F6(-1.0f + i * dx, r, ds, steps);
+    return acc * ((2.0 * TwistyPi) * dx);
+}
+
+// K(N) macro (PathWeightUtils.cpp:689-711, numSteps 10000 there): the rule
+// over F7 for every order, so K5 == K6.  The outer nodes are dealt to the
+// host threads; their partial sums are added in thread order.
+long double KMacro(float z, float ds, uint32_t outerSteps, uint32_t innerSteps)
+{
+    const float dx = (1.0f - -1.0f) / outerSteps;
+    const int threads = std::max(1, std::min<int>(hardware_threads(), (int)outerSteps));
     std::vector<long double> part(threads, 0.0L);
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t)
         pool.emplace_back([&, t] {
-            for (uint32_t i = t; i < steps; i += threads) part[t] += F7(-1.0f + i * dx, z, ds);
+            for (uint32_t i = t; i < outerSteps; i += threads) part[t] += F7(-1.0f + i * dx, z, ds, innerSteps);
         });
     for (auto &th : pool) th.join();
     long double acc = 0.0;
@@ -461,13 +464,14 @@ long double KMacro(float z, float ds)
 }
 }  // namespace
 
-double NormLog10(int numberOfSegments, float ds, const PerturbUtils::BoundaryConditions &bc)
+double NormLog10(int numberOfSegments, float ds, const PerturbUtils::BoundaryConditions &bc, uint32_t outerSteps,
+      uint32_t innerSteps)
 {
     const Vec3 zVec = (bc.m_endPos - bc.m_startPos) * (1.0f / ds) - bc.m_endDir - bc.m_startDir;
     const float z = zVec.Magnitude();
     long double v = 1.0L;
     if (numberOfSegments == 4) v = K4(z, ds);
-    else if (numberOfSegments == 5 || numberOfSegments == 6) v = KMacro(z, ds);
+    else if (numberOfSegments == 5 || numberOfSegments == 6) v = KMacro(z, ds, outerSteps, innerSteps);
     return (double)std::log10(v);
 }
 

@@ -10,7 +10,7 @@ from . import capi
 
 LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
 EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
-            "twisty_host_norm_log10", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
+            "twisty_host_norm_log10", "twisty_host_norm_log10_steps", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
             "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc",
             "twisty_host_path_generation_mc_multi", "twisty_host_experiment_gradients"]
@@ -68,6 +68,8 @@ def lib():
         L.twisty_host_minimum_arclength.restype = f32
         L.twisty_host_norm_log10.argtypes = [u32, f32, vp]
         L.twisty_host_norm_log10.restype = f64
+        L.twisty_host_norm_log10_steps.argtypes = [u32, f32, vp, u32, u32]
+        L.twisty_host_norm_log10_steps.restype = f64
         L.twisty_host_run_config.argtypes = [C.c_char_p, C.POINTER(HostResult), C.c_char_p, u32]
         L.twisty_host_rt_table_eps_derivative.argtypes = [f32, f32, f32, f32, u32, u32, f32, f32, C.c_int, f32,
                                                           C.c_int, vp]
@@ -175,9 +177,14 @@ def minimum_arclength(boundary, N):
     return lib().twisty_host_minimum_arclength(_p(b), N)
 
 
-def norm_log10(N, ds, boundary):
+def norm_log10(N, ds, boundary, outer_steps=None, inner_steps=None):
+    """log10 of the path normaliser; outer_steps / inner_steps override the
+    node counts of the N = 5, 6 nested quadrature (reference: 10000 / 1000)."""
     b = np.ascontiguousarray(boundary, np.float32)
-    return lib().twisty_host_norm_log10(N, ds, _p(b))
+    if outer_steps is None and inner_steps is None:
+        return lib().twisty_host_norm_log10(N, ds, _p(b))
+    return lib().twisty_host_norm_log10_steps(N, ds, _p(b), 10000 if outer_steps is None else outer_steps,
+                                              1000 if inner_steps is None else inner_steps)
 
 
 def write_path_batches(experiment_dir, boundary, positions, log10w, norm_log10, prepend="",

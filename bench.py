@@ -242,20 +242,25 @@ def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     sched = b.get_schedule()
     b.close()
     t0 = time.perf_counter()
+    plain_dev_ms = 0.0
     for x in exps:
         m = wl["meta"][x["table"]]
         ctx.set_tables(wl["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]),
                        float(np.float32(x["arclength"]) / np.float32(x["N"])), float(m[4]))
         ctx.set_curve(x["positions"], x["boundary"])
         ctx.run(x["paths"], x["K"], seed=x["seed"])
+        plain_dev_ms += ctx.last_timing()[1]
     plain_s = time.perf_counter() - t0
     return {"experiments": len(exps), "paths": sum(x["paths"] for x in exps), "joint_weights": jw,
             "batched_ms": ms, "batched_joint_weights_per_s": jw / (ms / 1e3), "schedule": sched,
             "autotune_candidates": len(tried),
             "plain_loop_ms": plain_s * 1e3, "plain_joint_weights_per_s": jw / plain_s,
-            "batched_vs_plain": plain_s * 1e3 / ms, "setup_s": setup_s,
+            "plain_device_ms": plain_dev_ms,
+            "batched_vs_plain": plain_s * 1e3 / ms, "batched_vs_plain_device": plain_dev_ms / ms,
+            "tables": len(wl["tables"]), "setup_s": setup_s,
             "workload": "4096 experiments: N log-uniform [16,256], L/D ~ U[1.05,3], end direction uniform on "
-                        "the sphere, paths log-uniform [2^10,2^16], K=100, 8 shared tables sigma ~ U[0.03,0.2]"}
+                        "the sphere, paths log-uniform [2^10,2^16], K=100, weighting sigma ~ U[0.03,0.2] (8 values), "
+                        "each experiment's table built at its own ds"}
 
 
 def measure_batched_c2(ctx, capi, w, table, pos, bc):

@@ -41,6 +41,17 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
  * them; (numSegments+1)*3 floats.  Returns 1 when no arc exists. */
 int twisty_host_initial_arc(const float boundary[13], uint32_t numSegments, float *positionsOut);
 
+/* count tables as twisty_host_build_rt_table builds them, table k for its own
+ * (eps[k], ds[k]) and maxBound[k] (maxBound NULL: 10 / eps[k] in float, as
+ * ExperimentRunner.cpp:168-171 sets it), the other parameters shared.  What
+ * FullExperimentRunnerOptimalPerturb.cpp:55-65 does once per experiment, for a
+ * batch of experiments: each table is built on one thread, threads tables at a
+ * time (<= 0: every hardware thread).  tablesOut: count * numCurvatureSteps
+ * floats; metaOut: 6 floats per table. */
+int twisty_host_build_rt_tables(uint32_t count, const float *eps, const float *ds, const float *maxBound, float mu,
+      float scatter, float absorption, float bias, uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound,
+      int threads, float *tablesOut, float *metaOut);
+
 /* CachedMultiArclengthWeightLookupTable (PathWeightUtils.cpp:123-278):
  * numDsSteps tables for ds = minDs + i (maxDs - minDs)/(numDsSteps - 1), read
  * from <cacheRoot>/CachedWeightTables/<hash>.cwt when present, otherwise

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from tests import cases
+from tests.parity import assert_path_rule, path_rule
 from twisty_amd import capi, host, workloads
 
 pytestmark = pytest.mark.gpu
@@ -44,6 +45,37 @@ def plain_run(ctx, x, tables, meta):
 def hetero_batch():
     return workloads.batch_experiments(n_exp=14, seed=7, K=30, n_range=(5, 300), paths_range=(200, 6000), n_tables=3,
                                        bins=1024, numStepsInt=2000)
+
+
+class _Table:
+    def __init__(self, table, meta_row):
+        self.table = np.asarray(table, np.float32)
+        self.minK, self.maxK, self.step, self.ds = (float(v) for v in meta_row[:4])
+
+
+def check_experiment_against_oracle(ctx, oracle, x, tables, meta, batched_log10w, G, n_check, tag):
+    """One experiment of a batch against the FP64 oracle with the per-path rule
+    (tests/parity.py): its batched per-path weights are bit-identical to a plain
+    run at the same lanes per path, whose per-joint curvatures then meet the
+    rule against the oracle's on the same draws (first n_check paths)."""
+    m = meta[x["table"]]
+    ds = float(np.float32(x["arclength"]) / np.float32(x["N"]))
+    ctx.set_tables(tables[x["table"]], float(m[0]), float(m[1]), float(m[2]), ds, float(m[4]))
+    ctx.set_curve(x["positions"], x["boundary"])
+    ctx.set_schedule(G, 0, 0)
+    n = min(int(x["paths"]), n_check)
+    plain = ctx.run(n, x["K"], seed=x["seed"], per_path=True, per_joint=True)
+    ctx.set_schedule(0, 0, 0)
+    assert np.array_equal(plain["log10w"], np.asarray(batched_log10w[:n]), equal_nan=True), tag
+    t = _Table(tables[x["table"]], m)
+    ref = oracle.simulate(np.asarray(x["positions"], np.float32).astype(np.float64), x["N"], x["arclength"], x["K"], n,
+                          t, float(m[4]), seed=x["seed"], want_curv=True)
+    assert np.array_equal(np.isnan(plain["log10w"]), ~ref["valid"]), tag
+    valid = ref["valid"] & np.isfinite(ref["log10w"])
+    st = path_rule(plain["log10w"], ref["log10w"], valid, plain["curv"], ref["curv"], t, float(m[4]))
+    print(f"[{tag}] N={x['N']} ds={ds:.4g} G={G} paths={n}: {st}", flush=True)
+    assert_path_rule(st)
+    return st
 
 
 def test_batched_matches_plain_runs(hip_device):
@@ -158,26 +190,30 @@ def test_batched_lean_schedule_rejected(hip_device):
 
 
 def test_batched_experiment_vs_oracle(hip_device, oracle):
-    """One experiment of a batch, per path against the FP64 oracle (same-bin rule of the forward tests)."""
+    """Two experiments of one batch, every path against the FP64 oracle with the
+    per-path rule of the forward tests, chained through a plain run at the
+    batch's lanes per path (bit-identical per path)."""
     case = cases.MID
     table, pos32 = cases.setup(case, oracle)
     x = {"N": case.N, "paths": 4000, "K": case.K, "seed": case.seed, "arclength": case.L, "table": 1,
          "positions": pos32, "boundary": cases.bc13(case)}
-    other = dict(x, N=20, positions=host.initial_arc(host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2),
-                                                                 20), arclength=1.2, table=0, paths=1500, seed=5)
-    tables = np.stack([table.table, table.table])
-    meta = np.array([(table.minK, table.maxK, table.step, table.ds, case.absorption)] * 2, np.float32)
+    bc20 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 1.2)
+    other = dict(x, N=20, positions=host.initial_arc(bc20, 20), boundary=bc20, arclength=1.2, table=0, paths=1500,
+                 seed=5)
+    t20 = oracle.build_table(float(np.float32(1.2) / np.float32(20)), eps=float(np.float32(case.sigma) ** 2),
+                             numStepsInt=case.numStepsInt, numCurvatureSteps=case.bins)
+    tables = np.stack([t20.table, table.table])
+    meta = np.array([(t20.minK, t20.maxK, t20.step, t20.ds, case.absorption),
+                     (table.minK, table.maxK, table.step, table.ds, case.absorption)], np.float32)
     ctx = capi.Context(0)
     b = capi.Batch(ctx, [other, x], tables, meta)
+    spl = b.get_schedule()["segments_per_lane"]
     res, pp, _ = b.run(per_path=True)
-    ref = oracle.simulate(pos32.astype(np.float64), case.N, case.L, case.K, 4000, table, case.absorption, seed=case.seed)
-    g = pp[1].astype(np.float64)
-    ok = ref["valid"] & np.isfinite(ref["log10w"])
-    rel = np.abs(g[ok] - ref["log10w"][ok]) / np.abs(ref["log10w"][ok])
-    print(f"[batched vs oracle] median rel {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
-    assert np.median(rel) < 1e-6 and np.mean(rel < 1e-5) > 0.97
-    assert res[1]["num_valid"] == ref["n_valid"]
     b.close()
+    for e, y in enumerate((other, x)):
+        check_experiment_against_oracle(ctx, oracle, y, tables, meta, pp[e], batch_lanes(y["N"], spl), y["paths"],
+                                        f"batched vs oracle exp {e}")
+        assert res[e]["num_paths"] == y["paths"]
     ctx.close()
 
 

@@ -14,15 +14,17 @@ the other GPU tests):
   bit and the same histogram to the rounding of its float atomics, and the
   estimate of two halves (path offsets) merges into the full one.
 * C3 (4096 heterogeneous experiments, one graph-replayed launch): replays agree
-  bit for bit, every experiment reports its path count, and experiments spread
-  over the range of N match plain runs of the same draws.
+  bit for bit, every experiment reports its path count, every experiment's
+  table is the one built at its own ds, experiments spread over the range of N
+  match plain runs of the same draws, and the experiments with the largest and
+  smallest ds meet the per-path rule against the FP64 oracle.
 """
 import math
 
 import numpy as np
 import pytest
 
-from tests.test_gpu_batched import plain_run
+from tests.test_gpu_batched import batch_lanes, check_experiment_against_oracle, plain_run
 from twisty_amd import capi, host, workloads
 
 pytestmark = pytest.mark.gpu
@@ -87,17 +89,21 @@ def test_c5_full_size_properties(hip_device):
     assert abs(merged - a["log10_sum_w"]) <= 1e-6 * abs(a["log10_sum_w"])
 
 
-def test_c3_full_size_properties(hip_device):
+def test_c3_full_size_properties(hip_device, oracle):
     w = workloads.batch_experiments(n_exp=4096, seed=1, K=100)
     exps = w["experiments"]
     ctx = capi.Context(0)
     b = capi.Batch(ctx, exps, w["tables"], w["meta"])
-    first, _, ms = b.run()
+    spl = b.get_schedule()["segments_per_lane"]
+    first, pp, ms = b.run(per_path=True)
     again, _, _ = b.run()
     b.close()
     assert first == again
     assert [r["num_paths"] for r in first] == [x["paths"] for x in exps]
     assert all(r["num_valid"] + r["num_invalid"] == r["num_paths"] for r in first)
+    # every experiment is weighted with its own table, built at its own ds
+    for x in exps:
+        assert w["meta"][x["table"]][3] == np.float32(x["arclength"]) / np.float32(x["N"])
     order = np.argsort([x["N"] for x in exps])
     picks = [int(order[k]) for k in (0, len(order) // 3, 2 * len(order) // 3, len(order) - 1)]
     for e in picks:
@@ -107,5 +113,11 @@ def test_c3_full_size_properties(hip_device):
               f"plain {ref['log10_sum_w']:.9f}", flush=True)
         assert r["num_valid"] == ref["num_valid"]
         assert abs(r["log10_sum_w"] - ref["log10_sum_w"]) <= 1e-6 * abs(ref["log10_sum_w"]) + 1e-9
+    # the experiments with the largest and the smallest ds, path by path against the oracle
+    ds = np.array([np.float32(x["arclength"]) / np.float32(x["N"]) for x in exps])
+    for e in (int(np.argmax(ds)), int(np.argmin(ds))):
+        x = exps[e]
+        check_experiment_against_oracle(ctx, oracle, x, w["tables"], w["meta"], pp[e], batch_lanes(x["N"], spl),
+                                        4096, f"C3 4096 exp {e}")
     print(f"[C3 4096] replay {ms:.2f} ms", flush=True)
     ctx.close()

@@ -1,8 +1,11 @@
 // host_capi.cpp -- C ABI of libtwisty_host.so (include/twisty_host.h).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
+#include <thread>
+#include <vector>
 
 #include "../../../include/twisty_host.h"
 #include "twisty_host.hpp"
@@ -60,6 +63,50 @@ int twisty_host_build_rt_table(float mu, float eps, float scatter, float absorpt
         return 1;
     }
     return 0;
+}
+
+int twisty_host_build_rt_tables(uint32_t count, const float *eps, const float *ds, const float *maxBound, float mu,
+      float scatter, float absorption, float bias, uint32_t numStepsInt, uint32_t numCurvatureSteps, float minBound,
+      int threads, float *tablesOut, float *metaOut)
+{
+    if (!eps || !ds || !tablesOut || !metaOut || numCurvatureSteps < 2 || numStepsInt < 4 || numStepsInt % 2 != 0)
+        return 1;
+    for (uint32_t k = 0; k < count; ++k)
+        if (!(ds[k] > 0.0f) || !(eps[k] > 0.0f)) return 1;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = (int)std::min<uint32_t>((uint32_t)threads, std::max(1u, count));
+    std::atomic<uint32_t> next { 0 };
+    std::atomic<int> failed { 0 };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (uint32_t k = next++; k < count; k = next++) {
+                WeightingParameters wp;
+                wp.mu = mu;
+                wp.eps = eps[k];
+                wp.scatter = scatter;
+                wp.absorption = absorption;
+                wp.bias = bias;
+                wp.numStepsInt = numStepsInt;
+                wp.numCurvatureSteps = numCurvatureSteps;
+                wp.minBound = minBound;
+                // ExperimentRunner.cpp:168-171: maxBound = 10 / eps
+                wp.maxBound = maxBound ? maxBound[k] : 10.0f / eps[k];
+                try {
+                    PathWeighting::WeightLookupTableIntegral tb(wp, ds[k]);
+                    tb.Initialize(1);
+                    const auto &v = tb.AccessLookupTable();
+                    std::memcpy(tablesOut + (size_t)k * numCurvatureSteps, v.data(), v.size() * sizeof(float));
+                    const float meta[6] = { tb.GetDs(), tb.GetMinCurvature(), tb.GetMaxCurvature(),
+                        tb.GetCurvatureStepSize(), tb.GetMinSegmentWeight(), tb.GetMaxSegmentWeight() };
+                    std::memcpy(metaOut + (size_t)k * 6, meta, sizeof(meta));
+                } catch (const std::exception &) {
+                    failed = 1;
+                }
+            }
+        });
+    for (auto &th : pool) th.join();
+    return failed ? 1 : 0;
 }
 
 int twisty_host_cached_rt_tables(const char *cacheRoot, float mu, float eps, float scatter, float absorption,

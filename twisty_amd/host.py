@@ -9,7 +9,7 @@ import numpy as np
 from . import capi
 
 LIB_PATH = os.path.join(capi.HERE, "lib", "libtwisty_host.so")
-EXPORTED = ["twisty_host_build_rt_table", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
+EXPORTED = ["twisty_host_build_rt_table", "twisty_host_build_rt_tables", "twisty_host_initial_arc", "twisty_host_minimum_arclength",
             "twisty_host_norm_log10", "twisty_host_norm_log10_steps", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
             "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc",
@@ -63,6 +63,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         f32, u32, f64, vp = C.c_float, C.c_uint32, C.c_double, C.c_void_p
         L.twisty_host_build_rt_table.argtypes = [f32, f32, f32, f32, f32, u32, u32, f32, f32, f32, C.c_int, vp, vp]
+        L.twisty_host_build_rt_tables.argtypes = [u32, vp, vp, vp, f32, f32, f32, f32, u32, u32, f32, C.c_int, vp,
+                                                  vp]
         L.twisty_host_initial_arc.argtypes = [vp, u32, vp]
         L.twisty_host_minimum_arclength.argtypes = [vp, u32]
         L.twisty_host_minimum_arclength.restype = f32
@@ -114,6 +116,25 @@ def build_rt_table(ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000
     if rc != 0:
         raise ValueError("twisty_host_build_rt_table rejected its arguments")
     return WeightTable(t, meta)
+
+
+def build_rt_tables(ds, eps, mu, scatter, absorption, bias=1.0, numStepsInt=20000, numCurvatureSteps=4096,
+                    minBound=0.0, threads=0):
+    """One table per (ds[k], eps[k]) pair, each as build_rt_table builds it
+    (maxBound 10/eps[k]); tables built side by side, one per host thread.
+    Returns (tables [count, numCurvatureSteps], meta [count, 6])."""
+    ds = np.ascontiguousarray(ds, np.float32)
+    eps = np.ascontiguousarray(eps, np.float32)
+    if ds.shape != eps.shape or ds.ndim != 1:
+        raise ValueError("ds and eps must be 1-D arrays of one length")
+    n = len(ds)
+    t = np.zeros((n, numCurvatureSteps), np.float32)
+    meta = np.zeros((n, 6), np.float32)
+    rc = lib().twisty_host_build_rt_tables(n, _p(eps), _p(ds), None, mu, scatter, absorption, bias, numStepsInt,
+                                           numCurvatureSteps, minBound, threads, _p(t), _p(meta))
+    if rc != 0:
+        raise ValueError("twisty_host_build_rt_tables rejected its arguments")
+    return t, meta
 
 
 def cached_rt_tables(min_ds, max_ds, num_ds, mu, eps, scatter, absorption, bias=1.0, numStepsInt=20000,

@@ -7,9 +7,11 @@ N log-uniform in [16, 256]; paths log-uniform in [2^10, 2^16]; K
 perturbations per path.  A draw whose arclength is below the minimum for its
 boundary conditions (PathGeneration::CalculateMinimumArclength) or admits no
 initial arc is rejected and drawn again.  The experiments share a few
-radiative-transfer tables with Gaussian widths sigma ~ U[0.03, 0.2]; each
-experiment takes one at random.  The tables are built on the curvature grid of
-the batch's smallest ds, so every experiment's curvatures fall inside.
+radiative-transfer weighting parameter sets, Gaussian widths sigma ~ U[0.03, 0.2];
+each experiment takes one at random and, as the reference's runner does
+(FullExperimentRunnerOptimalPerturb.cpp:55-65), weighs its paths with the table
+of that sigma built at its own ds = L / N (experiments with the same sigma and
+the same float ds share one table).
 """
 import math
 
@@ -42,13 +44,21 @@ def batch_experiments(n_exp=4096, seed=1, K=100, n_range=(16, 256), paths_range=
                      "positions": pos, "end_dir": e})
     sigmas = rng.uniform(*sigma_range, size=n_tables)
     for x in exps:
-        x["table"] = int(rng.integers(n_tables))
-    ds_min = min(float(np.float32(x["arclength"]) / np.float32(x["N"])) for x in exps)
-    tables, meta = [], []
-    for sg in sigmas:
-        t = host.build_rt_table(ds_min, mu, float(np.float32(sg) ** 2), scatter, absorption, numStepsInt=numStepsInt,
-                                numCurvatureSteps=bins)
-        tables.append(t.table)
-        meta.append((t.minK, t.maxK, t.step, t.ds, absorption))
-    return {"experiments": exps, "tables": np.stack(tables), "meta": np.array(meta, np.float32),
-            "sigmas": sigmas, "ds_table": ds_min}
+        x["sigma_index"] = int(rng.integers(n_tables))
+    # one table per distinct (sigma, float ds)
+    key_of, keys = {}, []
+    for x in exps:
+        ds = np.float32(x["arclength"]) / np.float32(x["N"])
+        k = (x["sigma_index"], int(np.float32(ds).view(np.uint32)))
+        if k not in key_of:
+            key_of[k] = len(keys)
+            keys.append((x["sigma_index"], float(ds)))
+        x["table"] = key_of[k]
+    eps = np.array([np.float32(sigmas[si]) ** 2 for si, _ in keys], np.float32)
+    dss = np.array([d for _, d in keys], np.float32)
+    tables, m6 = host.build_rt_tables(dss, eps, mu, scatter, absorption, numStepsInt=numStepsInt,
+                                      numCurvatureSteps=bins)
+    # Batch table meta: minK, maxK, step, ds, absorption
+    meta = np.stack([m6[:, 1], m6[:, 2], m6[:, 3], m6[:, 0], np.full(len(keys), absorption, np.float32)], axis=1)
+    return {"experiments": exps, "tables": tables, "meta": meta.astype(np.float32), "sigmas": sigmas,
+            "table_sigma_index": np.array([si for si, _ in keys], np.int32)}

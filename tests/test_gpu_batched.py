@@ -33,13 +33,18 @@ def batch_lanes(N, spl):
     return G
 
 
-def plain_run(ctx, x, tables, meta):
-    """A plain run of one experiment: its table's grid, its own ds (absorption)."""
+def plain_run(ctx, x, tables, meta, G=0, per_joint=False):
+    """A plain run of one experiment: its table's grid, its own ds (absorption);
+    G lanes per path (0: the plain resolver's choice)."""
     m = meta[x["table"]]
     ds = float(np.float32(x["arclength"]) / np.float32(x["N"]))
     ctx.set_tables(tables[x["table"]], float(m[0]), float(m[1]), float(m[2]), ds, float(m[4]))
     ctx.set_curve(x["positions"], x["boundary"])
-    return ctx.run(x["paths"], x["K"], seed=x["seed"], per_path=True)
+    ctx.set_schedule(G, 0, 0)
+    try:
+        return ctx.run(x["paths"], x["K"], seed=x["seed"], per_path=True, per_joint=per_joint)
+    finally:
+        ctx.set_schedule(0, 0, 0)
 
 
 def hetero_batch():
@@ -79,34 +84,44 @@ def check_experiment_against_oracle(ctx, oracle, x, tables, meta, batched_log10w
 
 
 def test_batched_matches_plain_runs(hip_device):
+    """Every experiment of a heterogeneous batch is bit-identical per path to a
+    plain run at the batch's lanes per path; where the plain resolver picks
+    another lane count, the two plain runs differ only in the order of the
+    group sums and meet the per-path rule (tests/parity.py) against each other
+    on their own per-joint curvatures."""
     w = hetero_batch()
     ctx = capi.Context(0)
     b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
     sched = b.get_schedule()
     print(f"[batched hetero] schedule {sched}", flush=True)
     res, pp, ms = b.run(per_path=True)
-    exact = 0
+    other_g = 0
     for x, r, g in zip(w["experiments"], res, pp):
-        ref = plain_run(ctx, x, w["tables"], w["meta"])
-        same_g = batch_lanes(x["N"], sched["segments_per_lane"]) == plain_lanes(x["N"])
-        a, c = g.astype(np.float64), ref["log10w"].astype(np.float64)
-        assert np.array_equal(np.isnan(a), np.isnan(c))
-        ok = np.isfinite(c)
-        if same_g:
-            assert np.array_equal(a[ok], c[ok]), f"N={x['N']}"
-            exact += 1
-        else:
-            assert np.abs(a[ok] - c[ok]).max() <= 1e-5 * np.abs(c[ok]).max()
-        assert r["num_paths"] == x["paths"] and r["num_valid"] == ref["num_valid"]
-        assert abs(r["log10_sum_w"] - ref["log10_sum_w"]) <= 1e-6 * abs(ref["log10_sum_w"]) + 1e-9
-    print(f"  {exact} of {len(res)} experiments bit-identical to plain runs; replay {ms:.3f} ms", flush=True)
+        gb = batch_lanes(x["N"], sched["segments_per_lane"])
+        same = plain_run(ctx, x, w["tables"], w["meta"], G=gb, per_joint=gb != plain_lanes(x["N"]))
+        a = g.astype(np.float64)
+        c = same["log10w"].astype(np.float64)
+        assert np.array_equal(a, c, equal_nan=True), f"N={x['N']} G={gb}"
+        assert r["num_paths"] == x["paths"] and r["num_valid"] == same["num_valid"]
+        assert abs(r["log10_sum_w"] - same["log10_sum_w"]) <= 1e-6 * abs(same["log10_sum_w"]) + 1e-9
+        if gb != plain_lanes(x["N"]):
+            other_g += 1
+            dflt = plain_run(ctx, x, w["tables"], w["meta"], per_joint=True)
+            valid = np.isfinite(dflt["log10w"])
+            assert np.array_equal(valid, np.isfinite(c))
+            st = path_rule(c, dflt["log10w"], valid, same["curv"], dflt["curv"], _Table(w["tables"][x["table"]],
+                           w["meta"][x["table"]]), float(w["meta"][x["table"]][4]))
+            print(f"  N={x['N']} G {gb} vs {plain_lanes(x['N'])}: {st}", flush=True)
+            assert_path_rule(st)
+    print(f"  {len(res)} experiments bit-identical to plain runs at the batch's lanes per path "
+          f"({other_g} also checked against the plain default lanes); replay {ms:.3f} ms", flush=True)
     b.close()
     ctx.close()
 
 
 def test_batched_chains_and_burn_in_match_plain_runs(hip_device):
-    """Chains and burn-in: every experiment bit-identical to twisty_hip_run with
-    the same chain length and burn-in when both use the same lanes per path."""
+    """Chains and burn-in: every experiment bit-identical per path to
+    twisty_hip_run with the same chain length, burn-in and lanes per path."""
     w = workloads.batch_experiments(n_exp=6, seed=11, K=3, n_range=(5, 120), paths_range=(500, 3000), n_tables=2,
                                     bins=1024, numStepsInt=2000)
     shapes = [(1, 0), (16, 0), (16, 20), (1, 5), (250, 3), (7, 1)]
@@ -122,14 +137,12 @@ def test_batched_chains_and_burn_in_match_plain_runs(hip_device):
         ctx.set_tables(w["tables"][x["table"]], float(m[0]), float(m[1]), float(m[2]), ds, float(m[4]))
         ctx.set_curve(x["positions"], x["boundary"])
         ctx.set_burn_in(x["burn_in"])
+        ctx.set_schedule(batch_lanes(x["N"], sched["segments_per_lane"]), 0, 0)
         ref = ctx.run(x["paths"], x["K"], seed=x["seed"], chain_length=x["chain"], per_path=True)
+        ctx.set_schedule(0, 0, 0)
         a, c = g.astype(np.float64), ref["log10w"].astype(np.float64)
-        ok = np.isfinite(c)
-        assert np.array_equal(np.isnan(a), np.isnan(c)) and r["num_valid"] == ref["num_valid"]
-        if batch_lanes(x["N"], sched["segments_per_lane"]) == plain_lanes(x["N"]):
-            assert np.array_equal(a[ok], c[ok]), f"N={x['N']} chain={x['chain']} burn={x['burn_in']}"
-        else:
-            assert np.abs(a[ok] - c[ok]).max() <= 1e-5 * np.abs(c[ok]).max()
+        assert np.array_equal(a, c, equal_nan=True), f"N={x['N']} chain={x['chain']} burn={x['burn_in']}"
+        assert r["num_valid"] == ref["num_valid"]
         assert abs(r["log10_sum_w"] - ref["log10_sum_w"]) <= 1e-6 * abs(ref["log10_sum_w"]) + 1e-9
     ctx.set_burn_in(0)
     b.close()

@@ -102,6 +102,22 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
       uint32_t chainLength, uint64_t seed, twisty_hip_partials *out, float *perPathLog10,
       float *perJointCurvature);
 
+/* twisty_hip_run without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same draws, kernels,
+ * schedule and result as twisty_hip_run with perPathLog10 and
+ * perJointCurvature NULL: the per-block partials and the FP64 result live in a
+ * static device array of the library (about 128 KB per GPU, held until the
+ * process ends) instead of buffers the context allocates.  Limits: at most
+ * 4096 blocks per launch (numCUs * blocksPerCU; TWISTY_HIP_ERR_UNSUPPORTED
+ * beyond), the exit-direction and weight histograms and the kept final state
+ * must be off (TWISTY_HIP_ERR_UNSUPPORTED otherwise).  Tables and curve come
+ * from set_tables / set_curve as for twisty_hip_run.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two contexts on two host
+ * threads) race and must be serialised by the caller. */
+int twisty_hip_run_noalloc(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset,
+      uint32_t perturbationsPerPath, uint32_t chainLength, uint64_t seed, twisty_hip_partials *out);
+
 /* Launch schedule of twisty_hip_run: lanesPerPath in {1,2,4,8,16,32,64},
  * blockSize one of the compiled sizes for that lanesPerPath, blocksPerCU >= 1;
  * 0 for any value selects the built-in choice for the current curve.  For

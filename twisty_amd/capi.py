@@ -28,6 +28,7 @@ EXPORTED = [
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
     "twisty_hip_batch_get_schedule", "twisty_hip_batch_run", "twisty_hip_batch_autotune",
+    "twisty_hip_run_noalloc",
 ]
 
 
@@ -104,6 +105,7 @@ def _declare(L):
     L.twisty_hip_set_tables.argtypes = [vp, vp, u32, u32, f32, f32, f32, f32, f32]
     L.twisty_hip_set_curve.argtypes = [vp, vp, u32, vp]
     L.twisty_hip_run.argtypes = [vp, u64, u64, u32, u32, u64, C.POINTER(Partials), vp, vp]
+    L.twisty_hip_run_noalloc.argtypes = [vp, u64, u64, u32, u32, u64, C.POINTER(Partials)]
     L.twisty_hip_set_schedule.argtypes = [vp, u32, u32, u32]
     L.twisty_hip_get_schedule.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.twisty_hip_last_timing.argtypes = [vp, C.POINTER(f32), C.POINTER(f32)]
@@ -207,6 +209,14 @@ class Context:
         if per_joint:
             res["curv"] = pj.reshape(num_paths, self.N - 1)
         return res
+
+    def run_noalloc(self, num_paths, K, seed=1, path_offset=0, chain_length=1):
+        """twisty_hip_run_noalloc: the estimate of run() with no device
+        allocation (partials in the library's static device scratch)."""
+        out = Partials()
+        self._check(lib().twisty_hip_run_noalloc(self._h, num_paths, path_offset, K, chain_length, seed,
+                                                 C.byref(out)))
+        return out.as_dict()
 
     def set_adjoint_roulette(self, kappa):
         """Russian roulette over paths before the adjoint (0 = exact, every path)."""

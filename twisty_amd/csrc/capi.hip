@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,48 @@
 using namespace twisty_hip;
 
 namespace {
+
+// Static scratch of twisty_hip_run_noalloc, the entry point for callers that
+// cannot allocate: the FP64 result in the first kScratchFinalBytes, then at
+// most kNoAllocMaxBlocks per-block partials (every block of a launch writes
+// its own partial and the finalize kernel reads exactly those, so nothing has
+// to start at zero).  Every GPU the library runs on holds its own copy until
+// the process ends; calls that overlap on one GPU share it and race.
+constexpr uint32_t kNoAllocMaxBlocks = 4096;
+constexpr size_t kScratchFinalBytes = 256;
+constexpr size_t kScratchBytes = kScratchFinalBytes + (size_t)kNoAllocMaxBlocks * sizeof(BlockPartial);
+static_assert(sizeof(FinalOut) <= kScratchFinalBytes, "the FP64 result must fit its slot");
+constexpr int kScratchDevices = 64;
+
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in, not device memory: only
+// the per-device address from hipGetSymbolAddress (scratch_address) is passed
+// to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex gScratchMu;
+unsigned char *gScratchAddr[kScratchDevices] = {};
+
+// This device's address of `scratch`, looked up once per device; the device
+// must be current.
+hipError_t scratch_address(int device, unsigned char **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kScratchDevices) return hipSuccess;  // run_noalloc reports it
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    if (!gScratchAddr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratchAddr[device] = static_cast<unsigned char *>(p);
+    }
+    *out = gScratchAddr[device];
+    return hipSuccess;
+}
+
 
 // LDS words the forward kernel reserves for the weight-magnitude histogram.
 uint32_t hist_stride(const twisty_hip_ctx *ctx) { return ctx->whBins ? ((ctx->whBins + 63u) & ~63u) : 0u; }
@@ -117,6 +160,12 @@ struct FwdRequest {
     uint64_t numPaths = 0, pathOffset = 0, seed = 0;
     uint32_t K = 0, chainLength = 1, burnIn = 0;
     bool perPath = false, perJoint = false, pathLog2 = false, pathBin = false, finalSeg = false;
+    // When set, the per-block partials and the FP64 result go here (at most
+    // maxBlocks partials) instead of the context's own buffers, and no buffer
+    // is allocated: the request must ask for no per-path outputs.
+    BlockPartial *partials = nullptr;
+    FinalOut *final = nullptr;
+    uint32_t maxBlocks = 0;
 };
 
 // Validates a run, sizes the buffers it needs and enqueues forward + finalize
@@ -138,7 +187,15 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     if (rc) return fail(ctx, rc, w + ": " + why);
     const uint32_t grid = ctx->numCUs * bpc;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) return rc;
+    if (r.partials) {
+        if (grid > r.maxBlocks)
+            return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, w + ": the schedule launches " + std::to_string(grid) +
+                        " blocks, more than the " + std::to_string(r.maxBlocks) + " partials of the static scratch");
+    } else if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) {
+        return rc;
+    }
+    BlockPartial *partials = r.partials ? r.partials : ctx->dPartials;
+    FinalOut *final = r.final ? r.final : ctx->dFinal;
     if (r.perPath && (rc = ensure_dev(ctx, ctx->dPerPath, ctx->perPathCap, r.numPaths, "per-path buffer"))) return rc;
     const uint64_t nJoint = r.numPaths * (uint64_t)(N - 1);
     if (r.perJoint && (rc = ensure_dev(ctx, ctx->dPerJoint, ctx->perJointCap, nJoint, "per-joint buffer"))) return rc;
@@ -157,7 +214,7 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     FwdParams p {};
     p.initSeg = ctx->dInitSeg;
     p.table = ctx->dTable;
-    p.partials = ctx->dPartials;
+    p.partials = partials;
     p.perPathLog10 = r.perPath ? ctx->dPerPath : nullptr;
     p.perJointCurv = r.perJoint ? ctx->dPerJoint : nullptr;
     p.seed = r.seed;
@@ -202,7 +259,7 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "forward kernel launch");
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
+    e = launch_finalize(partials, grid, absorbLog2PerPath, final, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "finalize kernel launch");
     HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream));
     return TWISTY_HIP_OK;
@@ -315,6 +372,7 @@ int twisty_hip_create(int device, twisty_hip_ctx **out)
     if ((e = hipEventCreate(&ctx->ev2)) != hipSuccess) return bail(e, "event");
     if ((e = hipEventCreate(&ctx->ev3)) != hipSuccess) return bail(e, "event");
     if ((e = hipMalloc(&ctx->dFinal, sizeof(FinalOut))) != hipSuccess) return bail(e, "final buffer");
+    if ((e = scratch_address(device, &ctx->dScratch)) != hipSuccess) return bail(e, "static scratch address");
     *out = ctx;
     return TWISTY_HIP_OK;
 }
@@ -519,6 +577,44 @@ int twisty_hip_run(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, 
     ctx->lastHistPaths = hist ? numPaths : 0;
     ctx->lastHistBins = hist ? ctx->histTheta * ctx->histPhi : 0;
     ctx->lastFinalPaths = ctx->keepFinalState ? numPaths : 0;
+    copy_partials(fo, numPaths, out);
+    ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_run_noalloc(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset,
+      uint32_t perturbationsPerPath, uint32_t chainLength, uint64_t seed, twisty_hip_partials *out)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_noalloc: out is NULL");
+    if (!ctx->dScratch)
+        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "run_noalloc: no static scratch on device " +
+                    std::to_string(ctx->device) + " (devices 0.." + std::to_string(kScratchDevices - 1) + ")");
+    if (ctx->histTheta || ctx->whBins || ctx->keepFinalState)
+        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "run_noalloc: the exit-direction and weight histograms and "
+                    "the kept final state use buffers the library allocates; switch them off");
+    FwdRequest r;
+    r.numPaths = numPaths;
+    r.pathOffset = pathOffset;
+    r.seed = seed;
+    r.K = perturbationsPerPath;
+    r.chainLength = chainLength;
+    r.burnIn = ctx->burnIn;
+    r.final = reinterpret_cast<FinalOut *>(ctx->dScratch);
+    r.partials = reinterpret_cast<BlockPartial *>(ctx->dScratch + kScratchFinalBytes);
+    r.maxBlocks = kNoAllocMaxBlocks;
+    int rc = enqueue_forward(ctx, r, "run_noalloc");
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev3, ctx->stream));
+    FinalOut fo;
+    HIP_TRY(ctx, hipMemcpyAsync(&fo, r.final, sizeof(FinalOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev3));
+    ctx->lastAdjMs = 0.0f;
+    ctx->lastHistPaths = 0;
+    ctx->lastHistBins = 0;
+    ctx->lastFinalPaths = 0;
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
     return TWISTY_HIP_OK;

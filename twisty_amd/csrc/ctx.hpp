@@ -50,6 +50,9 @@ struct twisty_hip_ctx {
     BlockPartial *dPartials = nullptr;
     uint64_t partialsCap = 0;
     FinalOut *dFinal = nullptr;
+    // This device's address of the static scratch of twisty_hip_run_noalloc
+    // (looked up once per device by twisty_hip_create; never freed).
+    unsigned char *dScratch = nullptr;
 
     float *dPerPath = nullptr;
     uint64_t perPathCap = 0;

@@ -114,15 +114,24 @@ def cpu_baseline(table, pos, w, target_seconds):
     return out, ref_part
 
 
-def measure_adjoint(ctx, w, reps=3, roulette=True):
-    """Forward + adjoint device time at the same workload (HIP events): exact
-    (every path swept) and with Russian roulette (kappa = 1) in front of the
-    adjoint, with the roulette gradient's deviation from the exact one."""
+def measure_adjoint(ctx, w, bc=None, reps=3, roulette=True):
+    """Forward + adjoint device time at the same workload (HIP events): the
+    table gradient alone; the full gradient (table + reverse sweep) with the
+    default sweep (Russian roulette, kappa = 1) and with the exact sweep over
+    every path.  With the boundary conditions bc, the exit-angle gradient
+    (the sweep's dE/d(initial segments) along d(initial segments)/d(theta,
+    phi)) is reported with the roulette's added standard deviation and the
+    Monte Carlo standard error, for both sweeps."""
     import numpy as np
 
+    from twisty_amd import host
+
+    if bc is not None:
+        jt, jp = host.initial_arc_exit_jacobian(bc, w["N"])
+        ctx.set_adjoint_directions(np.stack([np.diff(jt, axis=0), np.diff(jp, axis=0)]))
     out = {}
     exact = None
-    legs = [(False, "table_only", 0.0), (True, "full", 0.0)] + ([(True, "full_roulette", 1.0)] if roulette else [])
+    legs = [(False, "table_only", 0.0), (True, "full_exact", 0.0)] + ([(True, "full", 1.0)] if roulette else [])
     for seg, key, kappa in legs:
         ctx.set_adjoint_roulette(kappa)
         r = ctx.run_adjoint(w["paths"], w["K"], seed=w["seed"], segments=seg)
@@ -132,17 +141,50 @@ def measure_adjoint(ctx, w, reps=3, roulette=True):
             tot.append(ctx.last_timing()[1])
             adj.append(ctx.last_adjoint_ms())
         out[key] = {"fwd_adjoint_ms": statistics.fmean(tot), "adjoint_ms": statistics.fmean(adj),
-                    "paths_swept": ctx.last_adjoint_paths()}
-        if key == "full":
+                    "paths_swept": ctx.last_adjoint_paths() if seg else 0, "roulette_kappa": kappa if seg else None}
+        if seg and bc is not None:
+            st = ctx.last_adjoint_direction_stats()
+            out[key]["exit_angle_gradient"] = {k: [float(v) for v in st[k]] for k in ("gradient", "roulette_std",
+                                                                                        "mc_std")}
+        if key == "full_exact":
             exact = r
-        if key == "full_roulette":
-            out[key]["kappa"] = kappa
+        if key == "full":
             out[key]["d_seg_rel_to_exact"] = float(np.linalg.norm(r["d_seg"] - exact["d_seg"]) /
                                                    np.linalg.norm(exact["d_seg"]))
-            out[key]["d_table_running_sum_rel_to_exact"] = float(
-                np.abs(np.cumsum(r["d_table"]) - np.cumsum(exact["d_table"])).max() / np.abs(exact["d_table"]).sum())
-    ctx.set_adjoint_roulette(0.0)
+            out[key]["d_table_rel_to_exact"] = float(np.linalg.norm(r["d_table"] - exact["d_table"]) /
+                                                     np.linalg.norm(exact["d_table"]))
+    ctx.set_adjoint_roulette(1.0)
+    ctx.set_adjoint_directions(None)
     return out
+
+
+def measure_c4(capi, host, reps=5):
+    """Differentiation workload: N=100, arclength 1.3, 2^20 paths, K=100,
+    sigma=0.08; the plain forward and forward + adjoint (table only, exact
+    full gradient, roulette) device times, HIP events."""
+    import numpy as np
+
+    w = dict(WORKLOAD, N=100, L=1.3, K=100, paths=1 << 20, sigma=0.08)
+    ds = float(np.float32(w["L"]) / np.float32(w["N"]))
+    table = host.build_rt_table(ds, w["mu"], float(np.float32(w["sigma"]) ** 2), w["scatter"], w["absorption"],
+                                numStepsInt=w["numStepsInt"], numCurvatureSteps=w["bins"])
+    bc = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), w["L"])
+    ctx = capi.Context(0)
+    ctx.set_tables(table.table, table.minK, table.maxK, table.step, table.ds, w["absorption"])
+    ctx.set_curve(host.initial_arc(bc, w["N"]), bc)
+    ctx.run(w["paths"], w["K"], seed=w["seed"])
+    fwd = []
+    for _ in range(reps):
+        ctx.run(w["paths"], w["K"], seed=w["seed"])
+        fwd.append(ctx.last_timing()[0])
+    fwd_ms = statistics.fmean(fwd)
+    adj = measure_adjoint(ctx, w, bc)
+    ctx.close()
+    return {"workload": "C4: N=100, arclength 1.3, K=100, 2^20 paths, 4096-bin RT table sigma=0.08",
+            "forward_ms": fwd_ms, **adj,
+            "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
+            "ratio_full_exact": adj["full_exact"]["fwd_adjoint_ms"] / fwd_ms,
+            "ratio_table_only": adj["table_only"]["fwd_adjoint_ms"] / fwd_ms}
 
 
 def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22, reps=5):
@@ -292,6 +334,7 @@ def main(argv=None):
     ap.add_argument("--no-generated", action="store_true")
     ap.add_argument("--no-roulette", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -348,17 +391,25 @@ def main(argv=None):
         "batched": None,
     }
     if not a.no_adjoint:
-        adj = measure_adjoint(ctx, w, roulette=not a.no_roulette)
-        progress(f"adjoint: full {adj['full']['fwd_adjoint_ms']:.3f} ms")
-        out["fwd_adjoint_ms"] = adj["full"]["fwd_adjoint_ms"]
+        adj = measure_adjoint(ctx, w, bc, roulette=not a.no_roulette)
+        dflt = adj["full"] if "full" in adj else adj["full_exact"]
+        progress(f"adjoint: full {dflt['fwd_adjoint_ms']:.3f} ms (exact sweep "
+                 f"{adj['full_exact']['fwd_adjoint_ms']:.3f} ms)")
+        out["fwd_adjoint_ms"] = dflt["fwd_adjoint_ms"]
         out["adjoint"] = {"forward_ms": fwd_ms, **adj,
-                          "ratio_full": adj["full"]["fwd_adjoint_ms"] / fwd_ms,
+                          "ratio_full": dflt["fwd_adjoint_ms"] / fwd_ms,
+                          "ratio_full_exact": adj["full_exact"]["fwd_adjoint_ms"] / fwd_ms,
                           "ratio_table_only": adj["table_only"]["fwd_adjoint_ms"] / fwd_ms,
-                          "ratio_full_roulette": (adj["full_roulette"]["fwd_adjoint_ms"] / fwd_ms
-                                                  if "full_roulette" in adj else None),
-                          "gradients": "full: dE/dT (4096) + dE/d(initial segments) -> sigma, exit angles; "
-                                       "table_only: dE/dT (-> sigma); full_roulette: full, paths kept with "
-                                       "probability min(1, omega n / kappa) and reweighted (unbiased)"}
+                          "gradients": "table_only: dE/dT (4096 entries, -> sigma), every path; full: dE/dT over "
+                                       "every path + dE/d(initial segments) (-> exit angles) from the default "
+                                       "reverse sweep, paths kept with probability min(1, omega n / kappa), "
+                                       "kappa = 1, and reweighted (unbiased; its added standard deviation is "
+                                       "reported next to the Monte Carlo error); full_exact: the same with the "
+                                       "sweep over every path"}
+    if not a.no_c4:
+        out["c4"] = measure_c4(capi, host)
+        progress(f"C4: forward {out['c4']['forward_ms']:.3f} ms, exact full gradient "
+                 f"{out['c4']['full']['fwd_adjoint_ms']:.3f} ms")
     if not a.no_batched:
         out["batched"] = measure_batched(ctx, capi)
         progress(f"batched: {out['batched']['batched_ms']:.3f} ms")

@@ -138,16 +138,20 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
  * FP64 finalize and, when enabled, the exit histogram or the adjoint). */
 int twisty_hip_last_timing(twisty_hip_ctx *ctx, float *forwardMs, float *totalMs);
 
-/* Russian roulette in front of the following twisty_hip_run_adjoint calls:
- * kappa > 0 keeps path p with probability q_p = min(1, omega_p * numPaths /
- * kappa) (omega_p = w_p / sum w from the forward pass; draw: Philox4x32-10,
- * counter (0xA0000000, 0, path lo, path hi), key = seed) and scales its
- * contribution by 1 / q_p, so the gradient keeps its expectation while about
- * numPaths / kappa + (paths with omega_p >= kappa / numPaths) paths are swept.
- * The added variance is at most kappa / numPaths times the omega-weighted mean
- * square of the per-path gradients.  kappa = 0 (the default) visits every path
- * and gives the exact gradient of the paths' estimate.  No counterpart in the
- * reference, which has no adjoint. */
+/* Russian roulette in front of the reverse sweep of the following
+ * twisty_hip_run_adjoint calls (the dE/d(initial segments) -> exit-direction
+ * gradient): kappa > 0 keeps path p with probability q_p = min(1, omega_p *
+ * numPaths / kappa) (omega_p = w_p / sum w from the forward pass; draw:
+ * Philox4x32-10, counter (0xA0000000, 0, path lo, path hi), key = seed) and
+ * scales its contribution by 1 / q_p, so the gradient keeps its expectation
+ * while about numPaths / kappa + (paths with omega_p >= kappa / numPaths)
+ * paths are swept.  The added variance is at most kappa / numPaths times the
+ * omega-weighted mean square of the per-path gradients, and is reported per
+ * direction by twisty_hip_last_adjoint_direction_stats next to the gradient's
+ * Monte Carlo error.  The default is kappa = 1; kappa = 0 sweeps every path
+ * and gives the exact gradient of the paths' estimate (the one to compare with
+ * a same-draw reference).  The table gradient always takes every path.  No
+ * counterpart in the reference, which has no adjoint. */
 int twisty_hip_set_adjoint_roulette(twisty_hip_ctx *ctx, float kappa);
 
 /* Paths the last twisty_hip_run_adjoint visited (numPaths without roulette). */
@@ -250,8 +254,27 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
       uint32_t perturbationsPerPath, uint64_t seed, twisty_hip_partials *out, double *dLog10EstimateDTable,
       double *dLog10EstimateDSegments);
 
+/* Directions of the initial segments along which the following
+ * twisty_hip_run_adjoint calls with a reverse sweep report per-result
+ * statistics: numDirections (<= 4) vectors of 3*N floats (N of the current
+ * curve), e.g. d(initial segments)/d(exit angle).  0 switches it off.  No
+ * counterpart in the reference. */
+int twisty_hip_set_adjoint_directions(twisty_hip_ctx *ctx, const float *directions, uint32_t numDirections);
+
+/* For the last twisty_hip_run_adjoint with directions, per direction v:
+ * gradient[v] = dE/dv (dLog10EstimateDSegments projected on v),
+ * rouletteStd[v] the standard deviation the roulette adds (0 for the exact
+ * sweep), monteCarloStd[v] the Monte Carlo standard error of dE/dv over
+ * independent path samples (delta method for the self-normalised estimator,
+ * estimated from the swept paths).  Arrays of numDirections doubles, each
+ * nullable. */
+int twisty_hip_last_adjoint_direction_stats(twisty_hip_ctx *ctx, double *gradient, double *rouletteStd,
+      double *monteCarloStd);
+
 /* Launch schedule of the adjoint kernel (same meaning as
- * twisty_hip_set_schedule; 0 = built-in choice). */
+ * twisty_hip_set_schedule; 0 = built-in choice).  It applies to the pass
+ * with the reverse sweep (to the table-only pass when there is no sweep); the
+ * table pass in front of a roulette sweep takes the built-in choice. */
 int twisty_hip_set_adjoint_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
       uint32_t blocksPerCU);
 

@@ -124,6 +124,11 @@ typedef struct twisty_host_gradients {
     double d_eps;           /* dE / d eps, the config's eps slot (maxBound = 10/eps moves with it) */
     double d_theta, d_phi;  /* dE / d angles of the end direction (sin t cos p, sin t sin p, cos t) */
     double kernel_ms;       /* forward + adjoint device time */
+    /* standard deviation of d_theta, d_phi added by the roulette of the
+     * reverse sweep (twisty_hip.adjointRoulette, default 1; 0 = exact sweep,
+     * and these are 0) and their Monte Carlo standard error */
+    double d_theta_roulette_std, d_theta_mc_std, d_phi_roulette_std, d_phi_mc_std;
+    uint64_t paths_swept;   /* paths the reverse sweep visited */
 } twisty_host_gradients;
 
 /* The adjoint of the experiment of a JSON config (text; experiment.

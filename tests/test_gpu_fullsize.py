@@ -47,7 +47,7 @@ def context(N, L, sigmas, paths_hint=None):
 def test_c4_full_size_adjoint_identities(hip_device):
     N, L, K, P = 100, 1.3, 100, 1 << 20
     ctx, tabs, pos = context(N, L, [0.08])
-    r = ctx.run_adjoint(P, K, seed=1, segments=True)
+    r = ctx.run_adjoint(P, K, seed=1, segments=True, kappa=0.0)
     ctx.close()
     assert r["num_paths"] == P and r["num_valid"] + r["num_invalid"] == P
     T = tabs[0].table.astype(np.float64)

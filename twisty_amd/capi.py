@@ -28,7 +28,7 @@ EXPORTED = [
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
     "twisty_hip_batch_get_schedule", "twisty_hip_batch_run", "twisty_hip_batch_autotune",
-    "twisty_hip_run_noalloc",
+    "twisty_hip_run_noalloc", "twisty_hip_set_adjoint_directions", "twisty_hip_last_adjoint_direction_stats",
 ]
 
 
@@ -123,6 +123,8 @@ def _declare(L):
     L.twisty_hip_get_exit_histogram.argtypes = [vp, vp, vp, u64]
     L.twisty_hip_run_adjoint.argtypes = [vp, u64, u64, u32, u64, C.POINTER(Partials), vp, vp]
     L.twisty_hip_set_adjoint_schedule.argtypes = [vp, u32, u32, u32]
+    L.twisty_hip_set_adjoint_directions.argtypes = [vp, vp, u32]
+    L.twisty_hip_last_adjoint_direction_stats.argtypes = [vp, vp, vp, vp]
     L.twisty_hip_last_adjoint_timing.argtypes = [vp, C.POINTER(f32)]
     L.twisty_hip_batch_create.argtypes = [vp, vp, u32, vp, u32, u32, vp, vp, u64, C.POINTER(vp)]
     L.twisty_hip_batch_destroy.argtypes = [vp]
@@ -303,8 +305,11 @@ class Context:
     def set_adjoint_schedule(self, lanes_per_path=0, block_size=0, blocks_per_cu=0):
         self._check(lib().twisty_hip_set_adjoint_schedule(self._h, lanes_per_path, block_size, blocks_per_cu))
 
-    def run_adjoint(self, num_paths, K, seed=1, path_offset=0, segments=True, table_bins=None):
-        """Forward + adjoint: partials, dE/dT (bins) and dE/d(initial segments) (N, 3) or None."""
+    def run_adjoint(self, num_paths, K, seed=1, path_offset=0, segments=True, table_bins=None, kappa=None):
+        """Forward + adjoint: partials, dE/dT (bins) and dE/d(initial segments) (N, 3) or None.
+        kappa, when given, sets the roulette of the reverse sweep first (0 = exact)."""
+        if kappa is not None:
+            self.set_adjoint_roulette(kappa)
         out = Partials()
         nb = table_bins if table_bins is not None else self.table_bins
         dT = np.empty(nb, np.float64)
@@ -315,6 +320,24 @@ class Context:
         res["d_table"] = dT
         res["d_seg"] = None if dS is None else dS.reshape(self.N, 3)
         return res
+
+    def set_adjoint_directions(self, directions):
+        """Directions (D, N, 3) of the initial segments for per-result statistics
+        of the reverse sweep's gradient (None or D = 0 switches it off)."""
+        if directions is None:
+            self._check(lib().twisty_hip_set_adjoint_directions(self._h, None, 0))
+            return
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3 * self.N)
+        self._check(lib().twisty_hip_set_adjoint_directions(self._h, _ptr(d), d.shape[0]))
+        self._num_dirs = d.shape[0]
+
+    def last_adjoint_direction_stats(self):
+        """dE/dv, the roulette's added standard deviation and the Monte Carlo
+        standard error of dE/dv, per direction v of set_adjoint_directions."""
+        D = getattr(self, "_num_dirs", 0)
+        g, r, m = (np.empty(D, np.float64) for _ in range(3))
+        self._check(lib().twisty_hip_last_adjoint_direction_stats(self._h, _ptr(g), _ptr(r), _ptr(m)))
+        return {"gradient": g, "roulette_std": r, "mc_std": m}
 
     def last_adjoint_ms(self):
         a = C.c_float()

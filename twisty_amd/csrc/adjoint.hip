@@ -250,6 +250,9 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
     const float M = (float)p.fin->maxLog2;
     const bool finiteM = M > -INFINITY && M < INFINITY;
     const uint64_t numVisits = p.pick ? (uint64_t)*p.pickCount : p.numPaths;
+    float st[kDirStatsMax];  // direction statistics of this group's paths (group leader)
+#pragma unroll
+    for (uint32_t k = 0; k < kDirStatsMax; ++k) st[k] = 0.0f;
 
     for (uint64_t visit = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; visit < numVisits;
           visit += (uint64_t)gridDim.x * groupsPerBlock) {
@@ -258,7 +261,9 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
         // Scaled weight 2^(log2 w - M) in (0, 1]; invalid (NaN) and zero-weight
         // paths, and those below FP32 range relative to the largest, add nothing.
         float wp = (finiteM && lw.x > -INFINITY && lw.x < INFINITY) ? exp2f((lw.x - M) + lw.y) : 0.0f;
-        if (p.pick) wp *= p.pick[visit].factor;
+        const float w0 = wp;                                   // before the roulette factor
+        const float factor = p.pick ? p.pick[visit].factor : 1.0f;
+        wp *= factor;
         if (!(wp > 0.0f)) continue;  // uniform across the path's group
         const uint64_t path = p.pathOffset + local;
         for (uint32_t s = 0; s < S; ++s) {
@@ -301,7 +306,9 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
                 const uint32_t qi = (q / G) * SLOT + groupBase + (q & (uint32_t)(G - 1));
                 float u0, u1, u2;
                 unit_tangent(smem[qi], smem[qi + BLOCK], smem[qi + 2 * BLOCK], u0, u1, u2);
-                const JointAdj ja = joint_adjoint<true>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT);
+                const JointAdj ja = p.tableGrad
+                      ? joint_adjoint<true>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT)
+                      : joint_adjoint<false>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT);
                 mx -= ja.vx;
                 my -= ja.vy;
                 mz -= ja.vz;
@@ -348,6 +355,41 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
                 }
             }
             wave_sync_lds();
+            if (p.numDirs) {
+                // this path's dE/d(initial segments) (scaled by w f) along the directions
+                float x[kMaxDirs];
+#pragma unroll
+                for (uint32_t d = 0; d < kMaxDirs; ++d) x[d] = 0.0f;
+                for (uint32_t s = 0; s < S; ++s) {
+                    const uint32_t i = s * G + j;
+                    if (i >= N) break;
+                    const float l0 = smem[lamOff + s * SLOT + lane];
+                    const float l1 = smem[lamOff + s * SLOT + BLOCK + lane];
+                    const float l2 = smem[lamOff + s * SLOT + 2 * BLOCK + lane];
+#pragma unroll
+                    for (uint32_t d = 0; d < kMaxDirs; ++d) {
+                        if (d < p.numDirs) {
+                            const float *v = p.dirs + (uint64_t)d * 3u * N + 3u * i;
+                            x[d] = fmaf(l0, v[0], fmaf(l1, v[1], fmaf(l2, v[2], x[d])));
+                        }
+                    }
+                }
+#pragma unroll
+                for (uint32_t d = 0; d < kMaxDirs; ++d) {
+                    if constexpr (G > 1) x[d] = group_sum<G>(x[d]);
+                }
+                if (j == 0) {
+                    const float q = 1.0f / factor;
+#pragma unroll
+                    for (uint32_t d = 0; d < kMaxDirs; ++d) {
+                        const float x2 = x[d] * x[d];
+                        st[3 * d] = fmaf(x2, 1.0f - q, st[3 * d]);
+                        st[3 * d + 1] = fmaf(x2, q, st[3 * d + 1]);
+                        st[3 * d + 2] = fmaf(x[d], w0, st[3 * d + 2]);
+                    }
+                    st[3 * kMaxDirs] = fmaf(w0 * w0, factor, st[3 * kMaxDirs]);
+                }
+            }
             for (uint32_t s = 0; s < S; ++s) {
                 const uint32_t i = s * G + j;
                 if (i < N) {
@@ -359,11 +401,33 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
         wave_sync_lds();  // neighbours' columns are read before the next path is loaded
     }
     __syncthreads();
-    float *gt = p.gradTablePartial + (uint64_t)blockIdx.x * p.tableSize;
-    for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) gt[i] = gradT[i];
+    if (p.tableGrad) {
+        float *gt = p.gradTablePartial + (uint64_t)blockIdx.x * p.tableSize;
+        for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) gt[i] = gradT[i];
+    }
     if constexpr (SWEEP) {
         float *gs = p.gradSegPartial + (uint64_t)blockIdx.x * (3 * N);
         for (uint32_t i = threadIdx.x; i < 3 * N; i += BLOCK) gs[i] = gradS[i];
+        if (p.numDirs) {
+            // block sum of the leaders' statistics: wave sums, then one row per
+            // wave in LDS (the state rows are free after the barrier above)
+#pragma unroll
+            for (uint32_t k = 0; k < kDirStatsMax; ++k) {
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) st[k] += __shfl_xor(st[k], o, 64);
+            }
+            const uint32_t wave = threadIdx.x >> 6;
+            if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+                for (uint32_t k = 0; k < kDirStatsMax; ++k) smem[wave * kDirStatsMax + k] = st[k];
+            }
+            __syncthreads();
+            if (threadIdx.x < kDirStatsMax) {
+                float v = 0.0f;
+                for (uint32_t w = 0; w < BLOCK / 64; ++w) v += smem[w * kDirStatsMax + threadIdx.x];
+                p.dirStats[(uint64_t)blockIdx.x * kDirStatsMax + threadIdx.x] = v;
+            }
+        }
     }
 }
 

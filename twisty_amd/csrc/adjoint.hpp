@@ -35,7 +35,22 @@ struct AdjParams {
     float minK, maxK, step, invDs, invStep;
     const AdjPick *pick;         // nullable: visit only these paths (roulette)
     const uint32_t *pickCount;   // their number (written by adjoint_pick_kernel)
+    uint32_t tableGrad;          // 1: scatter dE/dT (sweep instances may leave it to a table-only pass)
+    // Directions of the initial segments (sweep instances): numDirs vectors of
+    // 3N floats, dirs[d * 3N + 3i + c]; per visited path the kernel projects
+    // the path's dE/d(initial segments) onto them and accumulates the sums of
+    // adj_dir_stats into dirStats[block][kDirStatsMax].
+    const float *dirs;
+    uint32_t numDirs;
+    float *dirStats;
 };
+
+// Per-path sums of the direction statistics, for direction d (x = the path's
+// contribution w f dln w/d(dir) in scaled weight units, w its scaled weight,
+// f = 1/q its roulette factor):
+//   [3d] sum x^2 (1 - q), [3d + 1] sum x^2 q, [3d + 2] sum x w; [3 kMaxDirs] sum w^2 f.
+constexpr uint32_t kMaxDirs = 4;
+constexpr uint32_t kDirStatsMax = 3 * kMaxDirs + 1;
 
 // Dynamic LDS bytes: segment rows (and adjoint rows for the sweep), table,
 // table gradient, segment gradient and draw records (sweep, G > 1).

@@ -42,6 +42,9 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     const float *init = p.segs + e.segOffset;
     for (uint32_t q = lane / G; q < item.count; q += groupsPerBlock) {
         const uint64_t chain = item.begin + q;
+        smem[scr] = 0.0f;  // cleared per chain, as in forward_kernel
+        smem[scr + BLOCK] = 0.0f;
+        smem[scr + 2 * BLOCK] = 0.0f;
         for (uint32_t s = 0; s < e.S; ++s) {
             const uint32_t i = s * G + j;
             if (i < e.N) {

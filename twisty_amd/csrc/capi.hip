@@ -63,7 +63,6 @@ hipError_t scratch_address(int device, unsigned char **out)
     return hipSuccess;
 }
 
-
 // LDS words the forward kernel reserves for the weight-magnitude histogram.
 uint32_t hist_stride(const twisty_hip_ctx *ctx) { return ctx->whBins ? ((ctx->whBins + 63u) & ~63u) : 0u; }
 
@@ -404,6 +403,8 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dGradSPartial);
     (void)hipFree(ctx->dGradT);
     (void)hipFree(ctx->dGradS);
+    (void)hipFree(ctx->dDirs);
+    (void)hipFree(ctx->dDirStats);
     (void)hipFree(ctx->dWeightHist);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -908,6 +909,9 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (!out || !dLog10EstimateDTable) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_adjoint: out and dTable are required");
     const bool sweep = dLog10EstimateDSegments != nullptr;
+    // The roulette applies to the exit-direction sweep only; the table
+    // gradient then comes from a table-only pass over every path.
+    const bool roulette = sweep && ctx->adjKappa > 0.0f;
     FwdRequest r;
     r.numPaths = numPaths;
     r.pathOffset = pathOffset;
@@ -915,23 +919,28 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     r.K = perturbationsPerPath;
     r.chainLength = 1;
     r.pathLog2 = true;
-    r.finalSeg = true;
+    r.finalSeg = true;  // every adjoint pass starts from the paths' final segments
     const uint32_t N = ctx->N;
+    if (roulette && numPaths > 0xFFFFFFFFull)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run_adjoint: the roulette handles at most 2^32 - 1 paths");
+    // Schedules: the user's adjoint schedule applies to the pass that has the
+    // sweep (or to the table pass when there is no sweep); the table pass in
+    // front of a roulette sweep takes the built-in choice.
+    uint32_t G = ctx->adjG, B = ctx->adjBlock, bpc = ctx->adjBpc, lds = 0;
+    uint32_t tG = 0, tB = 0, tBpc = 0, tLds = 0;
     if (ctx->haveCurve && ctx->haveTables) {
-        uint32_t G = ctx->adjG, B = ctx->adjBlock, bpc = ctx->adjBpc, lds = 0;
         std::string why;
         int rc0 = resolve_adjoint_schedule(ctx, N, sweep, G, B, bpc, lds, why);
+        if (!rc0 && roulette) rc0 = resolve_adjoint_schedule(ctx, N, false, tG, tB, tBpc, tLds, why);
         if (rc0) return fail(ctx, rc0, "run_adjoint: " + why);
     }
+    const bool dirs = sweep && ctx->numDirs > 0 && ctx->dirsN == N;
     int rc = enqueue_forward(ctx, r, "run_adjoint");
     if (rc) return rc;
-    uint32_t G = ctx->adjG, B = ctx->adjBlock, bpc = ctx->adjBpc, lds = 0;
-    std::string why;
-    rc = resolve_adjoint_schedule(ctx, N, sweep, G, B, bpc, lds, why);
-    if (rc) return fail(ctx, rc, "run_adjoint: " + why);
     const uint32_t grid = ctx->numCUs * bpc;
+    const uint32_t tGrid = roulette ? ctx->numCUs * tBpc : grid;
     const uint32_t nb = ctx->tableSize;
-    if ((rc = ensure_dev(ctx, ctx->dGradTPartial, ctx->gradTPartialCap, (uint64_t)grid * nb, "table gradient partials")))
+    if ((rc = ensure_dev(ctx, ctx->dGradTPartial, ctx->gradTPartialCap, (uint64_t)tGrid * nb, "table gradient partials")))
         return rc;
     if ((rc = ensure_dev(ctx, ctx->dGradT, ctx->gradTCap, nb, "table gradient"))) return rc;
     if (sweep) {
@@ -940,26 +949,15 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
             return rc;
         if ((rc = ensure_dev(ctx, ctx->dGradS, ctx->gradSCap, 3ull * N, "segment gradient"))) return rc;
     }
-    const bool roulette = ctx->adjKappa > 0.0f;
-    if (roulette) {
-        if (numPaths > 0xFFFFFFFFull)
-            return fail(ctx, TWISTY_HIP_ERR_ARG, "run_adjoint: the roulette handles at most 2^32 - 1 paths");
-        if ((rc = ensure_dev(ctx, ctx->dPick, ctx->pickCap, numPaths, "roulette list"))) return rc;
-        if ((rc = ensure_dev(ctx, ctx->dPickCount, ctx->pickCountCap, 1, "roulette count"))) return rc;
-        HIP_TRY(ctx, hipMemsetAsync(ctx->dPickCount, 0, sizeof(uint32_t), ctx->stream));
-        const hipError_t pe = launch_adjoint_pick(ctx->dPathLog2, ctx->dFinal, numPaths, pathOffset, seed,
-              ctx->adjKappa, ctx->dPick, ctx->dPickCount, ctx->stream);
-        if (pe != hipSuccess) return hipfail(ctx, pe, "roulette kernel launch");
-    }
+    if (dirs && (rc = ensure_dev(ctx, ctx->dDirStats, ctx->dirStatsCap, (uint64_t)grid * kDirStatsMax,
+                       "direction statistics")))
+        return rc;
     AdjParams a {};
-    a.pick = roulette ? ctx->dPick : nullptr;
-    a.pickCount = roulette ? ctx->dPickCount : nullptr;
     a.finalSeg = ctx->dFinalSeg;
     a.pathLog2 = ctx->dPathLog2;
     a.fin = ctx->dFinal;
     a.table = ctx->dTable;
     a.gradTablePartial = ctx->dGradTPartial;
-    a.gradSegPartial = sweep ? ctx->dGradSPartial : nullptr;
     a.seed = seed;
     a.pathOffset = pathOffset;
     a.numPaths = numPaths;
@@ -967,8 +965,6 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     a.K = perturbationsPerPath;
     a.maxDiff = (N - 2 < 25u) ? (N - 2) : 25u;
     a.stream = 0;
-    a.S = (N + G - 1) / G;
-    a.scratchRow = forward_scratch_row(G, N);
     a.tableSize = (int32_t)nb;
     a.tableStride = table_stride(nb);
     a.segStride = (3u * N + 63u) & ~63u;
@@ -977,9 +973,34 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     a.step = ctx->step;
     a.invDs = (float)N / ctx->arclength;
     a.invStep = 1.0f / ctx->step;
-    hipError_t e = launch_adjoint(a, G, sweep, grid, B, lds, ctx->stream);
+    hipError_t e;
+    if (roulette) {
+        // exact table gradient over every path
+        AdjParams t = a;
+        t.S = (N + tG - 1) / tG;
+        t.scratchRow = forward_scratch_row(tG, N);
+        t.tableGrad = 1;
+        e = launch_adjoint(t, tG, false, tGrid, tB, tLds, ctx->stream);
+        if (e != hipSuccess) return hipfail(ctx, e, "adjoint table kernel launch");
+        if ((rc = ensure_dev(ctx, ctx->dPick, ctx->pickCap, numPaths, "roulette list"))) return rc;
+        if ((rc = ensure_dev(ctx, ctx->dPickCount, ctx->pickCountCap, 1, "roulette count"))) return rc;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->dPickCount, 0, sizeof(uint32_t), ctx->stream));
+        e = launch_adjoint_pick(ctx->dPathLog2, ctx->dFinal, numPaths, pathOffset, seed, ctx->adjKappa, ctx->dPick,
+              ctx->dPickCount, ctx->stream);
+        if (e != hipSuccess) return hipfail(ctx, e, "roulette kernel launch");
+    }
+    a.pick = roulette ? ctx->dPick : nullptr;
+    a.pickCount = roulette ? ctx->dPickCount : nullptr;
+    a.gradSegPartial = sweep ? ctx->dGradSPartial : nullptr;
+    a.S = (N + G - 1) / G;
+    a.scratchRow = forward_scratch_row(G, N);
+    a.tableGrad = roulette ? 0u : 1u;
+    a.dirs = dirs ? ctx->dDirs : nullptr;
+    a.numDirs = dirs ? ctx->numDirs : 0u;
+    a.dirStats = dirs ? ctx->dDirStats : nullptr;
+    e = launch_adjoint(a, G, sweep, grid, B, lds, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "adjoint kernel launch");
-    e = launch_grad_reduce(ctx->dGradTPartial, grid, nb, ctx->dFinal, ctx->dGradT, ctx->stream);
+    e = launch_grad_reduce(ctx->dGradTPartial, tGrid, nb, ctx->dFinal, ctx->dGradT, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "gradient reduce launch");
     if (sweep) {
         e = launch_grad_reduce(ctx->dGradSPartial, grid, 3u * N, ctx->dFinal, ctx->dGradS, ctx->stream);
@@ -993,11 +1014,36 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     if (sweep)
         HIP_TRY(ctx, hipMemcpyAsync(dLog10EstimateDSegments, ctx->dGradS, 3ull * N * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<float> stats(dirs ? (size_t)grid * kDirStatsMax : 0);
+    if (dirs)
+        HIP_TRY(ctx, hipMemcpyAsync(stats.data(), ctx->dDirStats, stats.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
     uint32_t picked = 0;
     if (roulette)
         HIP_TRY(ctx, hipMemcpyAsync(&picked, ctx->dPickCount, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->lastPicked = roulette ? picked : numPaths;
+    ctx->lastDirs = 0;
+    if (dirs) {
+        // FP64 sums over blocks; E = log10(sum w / P) with w scaled by 2^-M:
+        // dE/dv = G' / ln 10, G' = sum_p w_p dln w_p/dv / S.
+        double sum[kDirStatsMax] = {};
+        for (uint32_t blk = 0; blk < grid; ++blk)
+            for (uint32_t k = 0; k < kDirStatsMax; ++k) sum[k] += (double)stats[(size_t)blk * kDirStatsMax + k];
+        const double S = fo.sumScaled, ln10 = 2.30258509299404568402;
+        for (uint32_t d = 0; d < ctx->numDirs; ++d) {
+            double g = 0.0;
+            for (uint64_t q = 0; q < 3ull * N; ++q)
+                g += dLog10EstimateDSegments[q] * (double)ctx->dirsHost[(size_t)d * 3 * N + q];
+            const double Gp = g * ln10;  // in ln units
+            const double varR = sum[3 * d];  // a sum of x^2 (1 - q) >= 0: 0 for the exact sweep
+            const double varM = std::max(0.0, sum[3 * d + 1] - 2.0 * Gp * sum[3 * d + 2] + Gp * Gp * sum[3 * kMaxDirs]);
+            ctx->lastDirGrad[d] = g;
+            ctx->lastDirRouletteStd[d] = (S > 0.0) ? std::sqrt(varR) / (S * ln10) : 0.0;
+            ctx->lastDirMcStd[d] = (S > 0.0) ? std::sqrt(varM) / (S * ln10) : 0.0;
+        }
+        ctx->lastDirs = ctx->numDirs;
+    }
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev3));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastAdjMs, ctx->ev2, ctx->ev3));
@@ -1006,6 +1052,46 @@ int twisty_hip_run_adjoint(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
     ctx->lastFinalPaths = numPaths;  // the forward pass above kept the final state
     copy_partials(fo, numPaths, out);
     ctx->lastError.clear();
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_set_adjoint_directions(twisty_hip_ctx *ctx, const float *directions, uint32_t numDirections)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (numDirections == 0) {
+        ctx->numDirs = 0;
+        return TWISTY_HIP_OK;
+    }
+    if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "set_adjoint_directions: set_curve first");
+    if (!directions || numDirections > kMaxDirs)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_directions: need 1.." + std::to_string(kMaxDirs) +
+                    " directions of 3N floats");
+    const uint64_t n = (uint64_t)numDirections * 3ull * ctx->N;
+    for (uint64_t q = 0; q < n; ++q)
+        if (!std::isfinite(directions[q]))
+            return fail(ctx, TWISTY_HIP_ERR_ARG, "set_adjoint_directions: non-finite entry");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_dev(ctx, ctx->dDirs, ctx->dirsCap, n, "adjoint directions"))) return rc;
+    HIP_TRY(ctx, hipMemcpy(ctx->dDirs, directions, n * sizeof(float), hipMemcpyHostToDevice));
+    ctx->dirsHost.assign(directions, directions + n);
+    ctx->numDirs = numDirections;
+    ctx->dirsN = ctx->N;
+    return TWISTY_HIP_OK;
+}
+
+int twisty_hip_last_adjoint_direction_stats(twisty_hip_ctx *ctx, double *gradient, double *rouletteStd,
+      double *monteCarloStd)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (ctx->lastDirs == 0)
+        return fail(ctx, TWISTY_HIP_ERR_STATE, "last_adjoint_direction_stats: the last adjoint had no directions "
+                    "(set_adjoint_directions, then run_adjoint with dSegments)");
+    for (uint32_t d = 0; d < ctx->lastDirs; ++d) {
+        if (gradient) gradient[d] = ctx->lastDirGrad[d];
+        if (rouletteStd) rouletteStd[d] = ctx->lastDirRouletteStd[d];
+        if (monteCarloStd) monteCarloStd[d] = ctx->lastDirMcStd[d];
+    }
     return TWISTY_HIP_OK;
 }
 

@@ -86,7 +86,20 @@ struct twisty_hip_ctx {
     float *dMultiMeta = nullptr;
     uint64_t multiTablesCap = 0, multiMetaCap = 0;
     uint32_t multiBins = 0, multiCount = 0, multiSegments = 0;
-    float adjKappa = 0.0f;     // adjoint roulette: 0 = every path
+    // Adjoint roulette of the exit-direction sweep: 0 = every path; the
+    // default 1 keeps about a tenth of the paths (the table gradient always
+    // takes every path).
+    float adjKappa = 1.0f;
+    // Directions of the initial segments for per-result statistics of the
+    // sweep's gradient (twisty_hip_set_adjoint_directions).
+    float *dDirs = nullptr;
+    uint64_t dirsCap = 0;
+    uint32_t numDirs = 0, dirsN = 0;
+    std::vector<float> dirsHost;  // the same directions, for the host-side projection
+    float *dDirStats = nullptr;
+    uint64_t dirStatsCap = 0;
+    uint32_t lastDirs = 0;       // directions the last adjoint reported
+    double lastDirGrad[kMaxDirs] = {}, lastDirRouletteStd[kMaxDirs] = {}, lastDirMcStd[kMaxDirs] = {};
     AdjPick *dPick = nullptr;
     uint64_t pickCap = 0;
     uint32_t *dPickCount = nullptr;

@@ -108,6 +108,12 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
             if (local >= p.numPaths) break;  // the last chain may be shorter
             const uint64_t path = p.pathOffset + local;
             if (sIdx == 0) {
+                // A path whose rotation turned NaN stores NaN * 0 into the
+                // scratch row; clearing it per chain keeps that path's
+                // neighbours in the lane independent of it.
+                smem[scr] = 0.0f;
+                smem[scr + BLOCK] = 0.0f;
+                smem[scr + 2 * BLOCK] = 0.0f;
                 for (uint32_t s = 0; s < S; ++s) {
                     const uint32_t i = s * G + j;
                     if (i < N) {

@@ -55,6 +55,16 @@ std::optional<ExperimentGradients> ComputeExperimentGradients(const ExperimentRu
           table.GetMaxCurvature(), table.GetCurvatureStepSize(), table.GetDs(), ep.weightingParameters.absorption);
     const auto bc13 = PerturbUtils::ToCudaSafe(bc);
     if (rc == TWISTY_HIP_OK) rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(arc.data()), N, bc13.data());
+    // exit-angle directions of the initial segments: segment i = p_{i+1} - p_i
+    // moves by J_{i+1} - J_i
+    std::vector<float> dirs(2ull * 3 * N);
+    for (uint32_t i = 0; i < N; ++i)
+        for (int a = 0; a < 3; ++a) {
+            dirs[3 * i + a] = (float)(jTheta[3 * (i + 1) + a] - jTheta[3 * i + a]);
+            dirs[3ull * N + 3 * i + a] = (float)(jPhi[3 * (i + 1) + a] - jPhi[3 * i + a]);
+        }
+    if (rc == TWISTY_HIP_OK) rc = twisty_hip_set_adjoint_directions(ctx, dirs.data(), 2);
+    if (rc == TWISTY_HIP_OK) rc = twisty_hip_set_adjoint_roulette(ctx, ep.adjointRoulette);
     const uint64_t seed = ep.curvePurturbSeed ? ep.curvePurturbSeed : (uint64_t)std::time(nullptr);
     ExperimentGradients g;
     g.dTable.assign(tv.size(), 0.0);
@@ -66,6 +76,14 @@ std::optional<ExperimentGradients> ComputeExperimentGradients(const ExperimentRu
         error = twisty_hip_last_error(ctx);
         return std::nullopt;
     }
+    double dg[2], rs[2], ms[2];
+    if (twisty_hip_last_adjoint_direction_stats(ctx, dg, rs, ms) == TWISTY_HIP_OK) {
+        g.dThetaRouletteStd = rs[0];
+        g.dThetaMcStd = ms[0];
+        g.dPhiRouletteStd = rs[1];
+        g.dPhiMcStd = ms[1];
+    }
+    twisty_hip_last_adjoint_paths(ctx, &g.pathsSwept);
     float fwdMs = 0.0f, totMs = 0.0f;
     twisty_hip_last_timing(ctx, &fwdMs, &totMs);
     g.kernelMs = totMs;

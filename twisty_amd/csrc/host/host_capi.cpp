@@ -348,6 +348,11 @@ int twisty_host_experiment_gradients(const char *jsonText, twisty_host_gradients
         out->d_theta = g->dTheta;
         out->d_phi = g->dPhi;
         out->kernel_ms = g->kernelMs;
+        out->d_theta_roulette_std = g->dThetaRouletteStd;
+        out->d_theta_mc_std = g->dThetaMcStd;
+        out->d_phi_roulette_std = g->dPhiRouletteStd;
+        out->d_phi_mc_std = g->dPhiMcStd;
+        out->paths_swept = g->pathsSwept;
         if (dTableOut) std::copy(g->dTable.begin(), g->dTable.end(), dTableOut);
     } catch (const std::exception &e) {
         put_error(errorOut, errorLen, e.what());

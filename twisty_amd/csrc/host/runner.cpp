@@ -87,6 +87,7 @@ ExperimentRunner::ExperimentParameters ExperimentRunner::ParseExperimentParamsFr
         if (h.contains("device")) ep.device = (int)h.at("device").number();
         if (h.contains("writeFiles")) ep.writeFiles = h.at("writeFiles").boolean();
         if (h.contains("autotune")) ep.autotune = h.at("autotune").boolean();
+        if (h.contains("adjointRoulette")) ep.adjointRoulette = (float)h.at("adjointRoulette").number();
     }
     return ep;
 }

@@ -277,6 +277,9 @@ class ExperimentRunner {
         int device = 0;
         bool writeFiles = true;             // table CSV, Results-style files
         bool autotune = false;              // batched runs: time the schedules first
+        // Gradients: roulette of the reverse sweep (twisty_hip_set_adjoint_roulette;
+        // 0 = exact sweep over every path).
+        float adjointRoulette = 1.0f;
     };
 
     struct ExperimentResults {
@@ -375,6 +378,10 @@ struct ExperimentGradients {
     double dEps = 0.0, dTheta = 0.0, dPhi = 0.0;
     std::vector<double> dTable;
     double kernelMs = 0.0;
+    // the exit angles' standard deviation added by the roulette of the
+    // reverse sweep (0 for the exact sweep) and their Monte Carlo standard error
+    double dThetaRouletteStd = 0.0, dThetaMcStd = 0.0, dPhiRouletteStd = 0.0, dPhiMcStd = 0.0;
+    uint64_t pathsSwept = 0;
 };
 std::optional<ExperimentGradients> ComputeExperimentGradients(const ExperimentRunner::ExperimentParameters &ep,
       const PerturbUtils::BoundaryConditions &bc, std::string &error);

@@ -41,6 +41,11 @@ class Gradients(C.Structure):
         ("d_theta", C.c_double),
         ("d_phi", C.c_double),
         ("kernel_ms", C.c_double),
+        ("d_theta_roulette_std", C.c_double),
+        ("d_theta_mc_std", C.c_double),
+        ("d_phi_roulette_std", C.c_double),
+        ("d_phi_mc_std", C.c_double),
+        ("paths_swept", C.c_uint64),
     ]
 
 

@@ -81,8 +81,13 @@ def check_invariants(g, N, table, pos):
         assert abs(np.sum(d * lam)) <= 1e-4 * scale
 
 
-def test_adjoint_table_gradient_vs_oracle_fd(hip_device, oracle):
-    case, paths, h = cases.C4, 1 << 13, 1e-3
+@pytest.mark.parametrize("h", [1e-4, 1e-3])
+def test_adjoint_table_gradient_vs_oracle_fd(hip_device, oracle, h):
+    """dE/dT along a random direction v (entries scaled by the table) against
+    FP64 central differences of the oracle's estimator at T +- h v, h relative
+    (1e-4 is the differentiation workload's step).  The float tables are
+    rounded after the step, so the adjoint is applied to the rounded step."""
+    case, paths = cases.C4, 1 << 13
     ctx = cases.hip_context(case, oracle)
     g = ctx.run_adjoint(paths, case.K, seed=case.seed, segments=False)
     table, pos32 = cases.setup(case, oracle)
@@ -98,7 +103,7 @@ def test_adjoint_table_gradient_vs_oracle_fd(hip_device, oracle):
 
     fd = est(tp) - est(tm)
     an = g["d_table"] @ (tp.astype(np.float64) - tm.astype(np.float64))
-    print(f"[adjoint FD] fd={fd:.6e} adjoint={an:.6e}", flush=True)
+    print(f"[adjoint FD h={h:g}] fd={fd:.6e} adjoint={an:.6e}", flush=True)
     assert abs(fd - an) <= 3e-3 * abs(fd)
     ctx.close()
 

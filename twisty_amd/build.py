@@ -30,6 +30,11 @@ HIPCC_FLAGS = [
     "-Wno-unused-function",
     # packed FP32 (v_pk_*_f32) stays off: the compiler must not form it.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # No implicit FMA contraction: HIP's __fmul_rn / __fadd_rn are plain
+    # operators, so only this keeps the reference's separately rounded float
+    # steps (the lookup's lerp, the compensated sums); every fused operation
+    # in the kernels is an explicit fmaf.
+    "-ffp-contract=off",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 

@@ -34,6 +34,7 @@
 // position inside the bin (in-bin slope plus edge jumps per unit curvature).
 // For table entries no such choice arises: f is exactly linear in them.
 #include "adjoint.hpp"
+#include "path_kernels.hpp"
 
 namespace twisty_hip {
 
@@ -187,14 +188,14 @@ struct JointAdj {
     float vx, vy, vz;  // (d ln f / d kappa) * d kappa / d t_b  (= -d/d t_a)
 };
 
+// The joint between segment a (unit tangent ta) and segment b (raw, 1/|b| =
+// ib), with the forward pass's curvature (joint_diff).
 template<bool SCATTER>
 __device__ __forceinline__ JointAdj joint_adjoint(const TableView &tv, float invDs, float invStep, float ta0,
-      float ta1, float ta2, float tb0, float tb1, float tb2, float wp, float *gradT)
+      float ta1, float ta2, float b0, float b1, float b2, float ib, float wp, float *gradT)
 {
-    const float e0 = (tb0 - ta0) * invDs;
-    const float e1 = (tb1 - ta1) * invDs;
-    const float e2 = (tb2 - ta2) * invDs;
-    const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+    const JointDiff jd = joint_diff(ta0, ta1, ta2, b0, b1, b2, ib, invDs);
+    const float e0 = jd.e0, e1 = jd.e1, e2 = jd.e2, kappa = jd.kappa;
     const LookupTerms lt = lookup_terms(tv, kappa);
     if constexpr (SCATTER) {
         const float cw = wp / lt.value;
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
         if constexpr (SWEEP) smem[lamOff + scr + c * BLOCK] = 0.0f;
     }
     __syncthreads();
-    const TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    const TableView tv = make_table_view(smem + tabOff, p.tableSize, p.minK, p.maxK, p.step);
     const uint32_t ldsBase = __builtin_amdgcn_groupstaticsize();
     const uint32_t laneB = ldsBase + lane * 4u;
     const uint32_t lamB = lamOff * 4u;
@@ -266,15 +267,7 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
         wp *= factor;
         if (!(wp > 0.0f)) continue;  // uniform across the path's group
         const uint64_t path = p.pathOffset + local;
-        for (uint32_t s = 0; s < S; ++s) {
-            const uint32_t i = s * G + j;
-            if (i < N) {
-                const float *src = p.finalSeg + (uint64_t)(3 * i) * p.numPaths + local;
-                smem[s * SLOT + lane] = src[0];
-                smem[s * SLOT + BLOCK + lane] = src[p.numPaths];
-                smem[s * SLOT + 2 * BLOCK + lane] = src[2 * p.numPaths];
-            }
-        }
+        load_state<G, BLOCK>(smem, p.finalSeg + local, 3ull * p.numPaths, p.numPaths, S, N, lane, j);
         if constexpr (SWEEP) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) smem[lamOff + scr + c * BLOCK] = 0.0f;
@@ -288,15 +281,17 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
             const uint32_t i = s * G + j;
             if (i >= N) break;
             float t0, t1, t2;
-            const float ia = unit_tangent(smem[s * SLOT + lane], smem[s * SLOT + BLOCK + lane],
-                  smem[s * SLOT + 2 * BLOCK + lane], t0, t1, t2);
+            const float a0 = smem[s * SLOT + lane], a1 = smem[s * SLOT + BLOCK + lane];
+            const float a2 = smem[s * SLOT + 2 * BLOCK + lane];
+            const float ia = unit_tangent(a0, a1, a2, t0, t1, t2);
             float mx = 0.0f, my = 0.0f, mz = 0.0f;
             if (i >= 1) {
                 const uint32_t q = i - 1;
                 const uint32_t qi = (q / G) * SLOT + groupBase + (q & (uint32_t)(G - 1));
                 float u0, u1, u2;
                 unit_tangent(smem[qi], smem[qi + BLOCK], smem[qi + 2 * BLOCK], u0, u1, u2);
-                const JointAdj ja = joint_adjoint<false>(tv, p.invDs, p.invStep, u0, u1, u2, t0, t1, t2, wp, gradT);
+                const JointAdj ja = joint_adjoint<false>(tv, p.invDs, p.invStep, u0, u1, u2, a0, a1, a2, ia, wp,
+                      gradT);
                 mx += ja.vx;
                 my += ja.vy;
                 mz += ja.vz;
@@ -304,11 +299,11 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(const AdjParams p)
             if (i + 1 < N) {
                 const uint32_t q = i + 1;
                 const uint32_t qi = (q / G) * SLOT + groupBase + (q & (uint32_t)(G - 1));
-                float u0, u1, u2;
-                unit_tangent(smem[qi], smem[qi + BLOCK], smem[qi + 2 * BLOCK], u0, u1, u2);
+                const float b0 = smem[qi], b1 = smem[qi + BLOCK], b2 = smem[qi + 2 * BLOCK];
+                const float ib = inv_norm(b0, b1, b2);
                 const JointAdj ja = p.tableGrad
-                      ? joint_adjoint<true>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT)
-                      : joint_adjoint<false>(tv, p.invDs, p.invStep, t0, t1, t2, u0, u1, u2, wp, gradT);
+                      ? joint_adjoint<true>(tv, p.invDs, p.invStep, t0, t1, t2, b0, b1, b2, ib, wp, gradT)
+                      : joint_adjoint<false>(tv, p.invDs, p.invStep, t0, t1, t2, b0, b1, b2, ib, wp, gradT);
                 mx -= ja.vx;
                 my -= ja.vy;
                 mz -= ja.vz;

@@ -45,14 +45,7 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
         smem[scr] = 0.0f;  // cleared per chain, as in forward_kernel
         smem[scr + BLOCK] = 0.0f;
         smem[scr + 2 * BLOCK] = 0.0f;
-        for (uint32_t s = 0; s < e.S; ++s) {
-            const uint32_t i = s * G + j;
-            if (i < e.N) {
-                smem[s * SLOT + lane] = init[3 * i + 0];
-                smem[s * SLOT + BLOCK + lane] = init[3 * i + 1];
-                smem[s * SLOT + 2 * BLOCK + lane] = init[3 * i + 2];
-            }
-        }
+        load_state<G, BLOCK>(smem, init, 3u, 1u, e.S, e.N, lane, j);
         // burn-in: the draws of twisty_hip_run's burn-in at pathOffset 0
         for (uint32_t b = 0; b < e.burnIn; ++b)
             perturb_path<G, BLOCK, LEAN>(laneB, lane, j, groupBase, scrB, drawbuf, e.seed, 0u,
@@ -115,7 +108,8 @@ __global__ void __launch_bounds__(BLOCK) batched_kernel(const BatchParams p)
             __syncthreads();
             curTable = e.table;
         }
-        const TableView tv { LEAN ? p.tableData + bt.offset : smem + tabOff, bt.size, bt.minK, bt.maxK, bt.step };
+        const TableView tv = make_table_view(LEAN ? p.tableData + bt.offset : smem + tabOff, bt.size, bt.minK, bt.maxK,
+              bt.step);
         LseAcc acc;
         lse_init(acc);
         uint32_t clamped = 0;

@@ -158,36 +158,51 @@ __device__ __forceinline__ Mat3 rotation_from_axis(float nx, float ny, float nz,
 // bin (the reference would read one past the end there).
 // ---------------------------------------------------------------------------
 struct TableView {
-    const float *w;  // LDS
+    const float *w;  // LDS or global memory
     int32_t size;
     float minK, maxK, step;
+    float invStep;   // RN(1 / step), for the bin index (bin_floor)
 };
 
-__device__ __forceinline__ float lookup_weight(const TableView &t, float curvature, uint32_t &clamped)
+__device__ __forceinline__ TableView make_table_view(const float *w, int32_t size, float minK, float maxK, float step)
 {
-    if (curvature < t.minK) {
-        if (!(fabsf(curvature - t.minK) < 1e-3f)) ++clamped;
-        curvature = t.minK;
-    }
-    if (curvature > t.maxK) {
-        if (!(fabsf(t.maxK - curvature) < 1e-3f)) ++clamped;
-        curvature = t.maxK;
-    }
-    const float distanceFromMin = __fsub_rn(curvature, t.minK);
-    const float realIdx = __fdiv_rn(distanceFromMin, t.step);
-    int32_t leftIdx = (int32_t)floorf(realIdx);
-    leftIdx = min(max(leftIdx, 0), t.size - 1);
-    const int32_t rightIdx = (leftIdx == t.size - 1) ? leftIdx : leftIdx + 1;
-    const float interpDist = __fsub_rn(distanceFromMin, __fmul_rn((float)leftIdx, t.step));
-    const float a = __fmul_rn(t.w[leftIdx], __fsub_rn(1.0f, interpDist));
-    const float b = __fmul_rn(t.w[rightIdx], interpDist);
-    return __fadd_rn(a, b);
+    return { w, size, minK, maxK, step, __fdiv_rn(1.0f, step) };
 }
 
-// The same lookup, returning the interpolation terms the adjoint needs.  value
-// is bit-identical to lookup_weight's result for the same curvature; clamped is
-// set when the curvature was moved onto the table range (the weight is then
-// flat in the curvature).
+// floor(RN(d / step)) for 0 <= d (NaN allowed), the bin index of
+// PathWeighters.cpp:67-68, without a division per joint.  q = RN(d * RN(1/step))
+// is within (1 + 2^-24)^2 - 1 < 1.51 * 2^-23 of d/step relative, and RN(d/step)
+// within 2^-24, so |q - RN(d/step)| < q * 2^-22 for q in the normal range
+// (below it both are far under 1/2 and floor to 0).  Unless q is that close to
+// an integer (or d is NaN) no integer lies between the two, and floor(q) is
+// floor(RN(d/step)); d = 0 gives 0 either way.  The rare remaining lanes (about
+// 2e-5 of the curvatures) take the IEEE division.  tests/test_bin_floor.py
+// checks the bound and the floors for every float d of the workloads' table
+// ranges.
+__device__ __forceinline__ float bin_floor(const TableView &t, float d)
+{
+    float q = __fmul_rn(d, t.invStep);
+    const float k = rintf(q);
+    const bool fast = (d == 0.0f) | (fabsf(q - k) > q * 0x1p-22f);
+    if (!fast) q = __fdiv_rn(d, t.step);
+    return floorf(q);
+}
+
+// Curvature moved onto [minK, maxK] as PathWeighters.cpp:45-65 does it (a
+// first clamp to minK cannot trigger the second), without branches; *moved
+// is set when either bound applied and *counted when the reference would print
+// its out-of-range message (more than 1e-3 outside).
+__device__ __forceinline__ float clamp_curvature(const TableView &t, float k, bool &moved, bool &counted)
+{
+    const bool lo = k < t.minK;
+    const bool hi = k > t.maxK;
+    counted = (lo & !(fabsf(k - t.minK) < 1e-3f)) | (hi & !(fabsf(t.maxK - k) < 1e-3f));
+    moved = lo | hi;
+    return lo ? t.minK : (hi ? t.maxK : k);
+}
+
+// The interpolation terms of PathWeighters.cpp:67-82 for a curvature on the
+// table range: left and right bin, fraction in curvature units, lerped weight.
 struct LookupTerms {
     int32_t left, right;
     float frac;   // fraction in curvature units (PathWeighters.cpp:79)
@@ -195,21 +210,10 @@ struct LookupTerms {
     bool clamped;
 };
 
-__device__ __forceinline__ LookupTerms lookup_terms(const TableView &t, float curvature)
+__device__ __forceinline__ void lerp_terms(const TableView &t, float curvature, LookupTerms &r)
 {
-    LookupTerms r;
-    r.clamped = false;
-    if (curvature < t.minK) {
-        curvature = t.minK;
-        r.clamped = true;
-    }
-    if (curvature > t.maxK) {
-        curvature = t.maxK;
-        r.clamped = true;
-    }
     const float distanceFromMin = __fsub_rn(curvature, t.minK);
-    const float realIdx = __fdiv_rn(distanceFromMin, t.step);
-    int32_t leftIdx = (int32_t)floorf(realIdx);
+    int32_t leftIdx = (int32_t)bin_floor(t, distanceFromMin);
     leftIdx = min(max(leftIdx, 0), t.size - 1);
     r.left = leftIdx;
     r.right = (leftIdx == t.size - 1) ? leftIdx : leftIdx + 1;
@@ -217,6 +221,28 @@ __device__ __forceinline__ LookupTerms lookup_terms(const TableView &t, float cu
     const float a = __fmul_rn(t.w[r.left], __fsub_rn(1.0f, r.frac));
     const float b = __fmul_rn(t.w[r.right], r.frac);
     r.value = __fadd_rn(a, b);
+}
+
+__device__ __forceinline__ float lookup_weight(const TableView &t, float curvature, uint32_t &clamped)
+{
+    bool moved, counted;
+    curvature = clamp_curvature(t, curvature, moved, counted);
+    clamped += counted ? 1u : 0u;
+    LookupTerms r;
+    lerp_terms(t, curvature, r);
+    return r.value;
+}
+
+// The same lookup, returning the interpolation terms the adjoint needs.  value
+// is bit-identical to lookup_weight's result for the same curvature; clamped is
+// set when the curvature was moved onto the table range (the weight is then
+// flat in the curvature).
+__device__ __forceinline__ LookupTerms lookup_terms(const TableView &t, float curvature)
+{
+    LookupTerms r;
+    bool counted;
+    curvature = clamp_curvature(t, curvature, r.clamped, counted);
+    lerp_terms(t, curvature, r);
     return r;
 }
 
@@ -376,6 +402,51 @@ __device__ __forceinline__ float group_sum(float v)
     return v;
 }
 
+// Lane i ^ 16 within each 32-lane half (ds_swizzle in bit mode: and_mask 0x1F,
+// or_mask 0, xor_mask 0x10).  It goes through the LDS crossbar (no LDS
+// storage), so it issues no VALU instruction.
+__device__ __forceinline__ float swap16(float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+
+// group_sum of NV independent values, step by step across the values so that
+// the DPP steps of different values fill each other's hazard slots.  Every
+// lane gets the same totals as group_sum<G> would give (the same additions in
+// the same order; the 32-lane step adds its partner row's total, a commutative
+// addition of the same two values).
+template<int G, int NV>
+__device__ __forceinline__ void group_sum_n(float (&v)[NV])
+{
+    if constexpr (G >= 2) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += dpp_f<DPP_QUAD_XOR1>(v[k]);
+    }
+    if constexpr (G >= 4) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += dpp_f<DPP_QUAD_XOR2>(v[k]);
+    }
+    if constexpr (G >= 8) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += dpp_f<DPP_ROW_HALF_MIRROR>(v[k]);
+    }
+    if constexpr (G >= 16) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += dpp_f<DPP_ROW_MIRROR>(v[k]);
+    }
+    if constexpr (G >= 32) {
+        float o[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) o[k] = swap16(v[k]);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += o[k];
+    }
+    if constexpr (G >= 64) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], 32, 64);
+    }
+}
+
 template<int G>
 __device__ __forceinline__ uint32_t group_or(uint32_t v)
 {
@@ -435,14 +506,40 @@ typedef __attribute__((address_space(3))) float lds_f32;
 __device__ __forceinline__ float lds_ld(uint32_t byteAddr) { return *(lds_f32 *)(size_t)byteAddr; }
 __device__ __forceinline__ void lds_st(uint32_t byteAddr, float v) { *(lds_f32 *)(size_t)byteAddr = v; }
 
+// 1 / |a| of a segment vector as the forward pass forms it.
+__device__ __forceinline__ float inv_norm(float a0, float a1, float a2)
+{
+    return __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, __fmul_rn(a2, a2))));
+}
+
 // Unit tangent of a segment vector as the forward pass forms it.
 __device__ __forceinline__ float unit_tangent(float a0, float a1, float a2, float &t0, float &t1, float &t2)
 {
-    const float ia = __builtin_amdgcn_rsqf(fmaf(a0, a0, fmaf(a1, a1, a2 * a2)));
-    t0 = a0 * ia;
-    t1 = a1 * ia;
-    t2 = a2 * ia;
+    const float ia = inv_norm(a0, a1, a2);
+    t0 = __fmul_rn(a0, ia);
+    t1 = __fmul_rn(a1, ia);
+    t2 = __fmul_rn(a2, ia);
     return ia;
+}
+
+// The joint between segment a (unit tangent ta, rounded) and segment b (raw,
+// ib = 1/|b|): e = (t_b - t_a) N/L and the curvature kappa = |e|
+// (CurvePerturbUtils.cpp:219-243).  The difference is formed as
+// fma(b, ib, -ta), one rounding, then scaled: written out so that every kernel
+// forms the same bits whatever the compiler would contract.
+struct JointDiff {
+    float e0, e1, e2, kappa;
+};
+
+__device__ __forceinline__ JointDiff joint_diff(float ta0, float ta1, float ta2, float b0, float b1, float b2,
+      float ib, float invDs)
+{
+    JointDiff d;
+    d.e0 = __fmul_rn(fmaf(b0, ib, -ta0), invDs);
+    d.e1 = __fmul_rn(fmaf(b1, ib, -ta1), invDs);
+    d.e2 = __fmul_rn(fmaf(b2, ib, -ta2), invDs);
+    d.kappa = sqrtf(fmaf(d.e0, d.e0, fmaf(d.e1, d.e1, __fmul_rn(d.e2, d.e2))));
+    return d;
 }
 
 }  // namespace twisty_hip

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
     if (p.weightHist)
         for (uint32_t i = threadIdx.x; i < p.weightBins; i += BLOCK) whist[i] = 0u;
     __syncthreads();
-    const TableView tv { LEAN ? p.table : smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    const TableView tv = make_table_view(LEAN ? p.table : smem + tabOff, p.tableSize, p.minK, p.maxK, p.step);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
@@ -114,14 +114,7 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
                 smem[scr] = 0.0f;
                 smem[scr + BLOCK] = 0.0f;
                 smem[scr + 2 * BLOCK] = 0.0f;
-                for (uint32_t s = 0; s < S; ++s) {
-                    const uint32_t i = s * G + j;
-                    if (i < N) {
-                        smem[s * SLOT + lane] = p.initSeg[3 * i + 0];
-                        smem[s * SLOT + BLOCK + lane] = p.initSeg[3 * i + 1];
-                        smem[s * SLOT + 2 * BLOCK + lane] = p.initSeg[3 * i + 2];
-                    }
-                }
+                load_state<G, BLOCK>(smem, p.initSeg, 3u, 1u, S, N, lane, j);
                 // Burn-in (FullExperimentRunnerOptimalPerturb.cpp:559-561):
                 // states perturbed, never weighted; draws from path ids with
                 // the top bit set, which weighted paths never reach.
@@ -151,47 +144,38 @@ __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
             }
             FF lsum { 0.0f, 0.0f };
             uint32_t flags = 0;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
-            if (G > 1 && !p.perJointCurv && !p.pathBin) {
-                // No per-joint outputs: the shared weighting loop.
-                if constexpr (G > 1) {
-                    const PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, p.invDs, tv, clamped);
-                    lsum = pl.lsum;
-                    flags = pl.flags;
+            if constexpr (G > 1) {
+                // The shared weighting loop, with the per-joint outputs when
+                // asked for (curvatures; exit-direction bin from t_{N-2}).
+                PathLog pl;
+                if (!p.perJointCurv && !p.pathBin) {
+                    pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, p.invDs, tv, clamped);
+                } else {
+                    pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, p.invDs, tv, clamped,
+                          [&](uint32_t i, const JointEval &je) {
+                              if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = je.kappa;
+                              if (p.pathBin && i == N - 2) p.pathBin[local] = exit_bin(p, je.t0, je.t1, je.t2);
+                          });
                 }
+                lsum = pl.lsum;
+                flags = pl.flags;
             } else {
-                float ta0 = 0.0f, ta1 = 0.0f, ta2 = 0.0f;
-                if constexpr (G == 1) {
-                    unit_tangent(smem[lane], smem[BLOCK + lane], smem[2 * BLOCK + lane], ta0, ta1, ta2);
-                }
-                for (uint32_t i = j; i + 1 < N; i += G) {
-                    if constexpr (G > 1) {
-                        const uint32_t ai = (i / G) * SLOT + lane;
-                        unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
-                    }
-                    const uint32_t i1 = i + 1;
-                    const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
+                // One lane per path: each segment's tangent is formed once
+                // and carried to the next joint.
+                float ta0, ta1, ta2;
+                unit_tangent(smem[lane], smem[BLOCK + lane], smem[2 * BLOCK + lane], ta0, ta1, ta2);
+                for (uint32_t i = 0; i + 1 < N; ++i) {
+                    const uint32_t bi = (i + 1) * SLOT + lane;
+                    const float b0 = smem[bi], b1 = smem[bi + BLOCK], b2 = smem[bi + 2 * BLOCK];
                     float tb0, tb1, tb2;
-                    unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
-                    const float e0 = (tb0 - ta0) * p.invDs;
-                    const float e1 = (tb1 - ta1) * p.invDs;
-                    const float e2 = (tb2 - ta2) * p.invDs;
-                    const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
+                    const float ib = unit_tangent(b0, b1, b2, tb0, tb1, tb2);
+                    const float kappa = joint_diff(ta0, ta1, ta2, b0, b1, b2, ib, p.invDs).kappa;
                     if (p.perJointCurv) p.perJointCurv[local * (uint64_t)(N - 1) + i] = kappa;
                     if (p.pathBin && i == N - 2) p.pathBin[local] = exit_bin(p, ta0, ta1, ta2);
-                    const float l = log2f(lookup_weight(tv, kappa, clamped));
-                    if (isnan(l)) flags |= 1u;
-                    else if (l == -INFINITY) flags |= 2u;
-                    else if (l == INFINITY) flags |= 4u;
-                    else lsum = ff_add_f(lsum, l);
-                    if constexpr (G == 1) {
-                        ta0 = tb0;
-                        ta1 = tb1;
-                        ta2 = tb2;
-                    }
-                }
-                if constexpr (G > 1) {
-                    lsum = group_sum_ff<G>(lsum);
-                    flags = group_or<G>(flags);
+                    add_joint_log(lsum, flags, log2f(lookup_weight(tv, kappa, clamped)), true);
+                    ta0 = tb0;
+                    ta1 = tb1;
+                    ta2 = tb2;
                 }
             }
             if constexpr (G > 1) wave_sync_lds();  // joints read other lanes' columns before the next reset

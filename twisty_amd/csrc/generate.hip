@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
         for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
         __syncthreads();
     }
-    TableView tv { smem + tabOff, p.tableSize, p.minK, p.maxK, p.step };
+    TableView tv = make_table_view(smem + tabOff, p.tableSize, p.minK, p.maxK, p.step);
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
     const uint32_t groupBase = lane - j;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
             ds = fmaf(gen_words(p.seed, path, 0, 0).u0, p.dsHi - p.dsLo, p.dsLo);
             const uint32_t t = nearest_table(p, ds);
             const float *m = p.multiMeta + 5 * t;
-            tv = TableView { p.multiTables + (uint64_t)t * p.tableStride, p.tableSize, m[1], m[2], m[3] };
+            tv = make_table_view(p.multiTables + (uint64_t)t * p.tableStride, p.tableSize, m[1], m[2], m[3]);
             absorbLog2 = m[4];
             invDs = (float)N / (ds * (float)N);
         }

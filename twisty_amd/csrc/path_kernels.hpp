@@ -89,6 +89,99 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
 }
 
 
+// Whether the sub-chains [left, left + diff) of M records are pairwise
+// disjoint (uniform across a group: every lane holds the same records).
+template<int M>
+__device__ __forceinline__ bool disjoint_records(const DrawRec (&r)[M])
+{
+    bool ok = true;  // bitwise, not short-circuit: no branches
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+#pragma unroll
+        for (int b = a + 1; b < M; ++b) {
+            const uint32_t la = r[a].packed >> 16, lb = r[b].packed >> 16;
+            const uint32_t ra = la + (r[a].packed & 0xFFFFu), rb = lb + (r[b].packed & 0xFFFFu);
+            ok = ok & ((ra <= lb) | (rb <= la));
+        }
+    }
+    return ok;
+}
+
+// M perturbations of one path with pairwise disjoint sub-chains, applied at
+// once by a group of G >= 25 lanes (every lane holds at most one segment of a
+// sub-chain, so the window is one slot).  Disjoint perturbations read and
+// write disjoint segments and their draws do not depend on the state, so the
+// result is bit for bit that of perturb_one applied to them in order (the same
+// operations on the same values); the M independent chains of LDS loads, group
+// sums and rotations interleave and hide each other's latency.  A lane outside
+// a sub-chain reads and writes its zeroed scratch row at scrB0, as in
+// perturb_one: every redirected slot of the M loads reads zero (the loads all
+// precede the stores) and stores R * 0 = 0.
+template<int G, int BLOCK, int M>
+__device__ __forceinline__ void perturb_multi(uint32_t laneB, uint32_t j, uint32_t scrB0, const DrawRec (&rec)[M])
+{
+    static_assert(G >= 25, "one window slot per lane needs G >= 25");
+    constexpr uint32_t SLOTB = 12u * BLOCK;
+    constexpr uint32_t COMPB = 4u * BLOCK;
+    uint32_t addr[M];
+    float dx[M], dy[M], dz[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const uint32_t diff = rec[m].packed & 0xFFFFu;
+        const uint32_t left = rec[m].packed >> 16;
+        const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
+        addr[m] = (i0 < left + diff) ? (i0 / G) * SLOTB + laneB : scrB0;
+        dx[m] = lds_ld(addr[m]);
+        dy[m] = lds_ld(addr[m] + COMPB);
+        dz[m] = lds_ld(addr[m] + 2 * COMPB);
+    }
+    float a[3 * M];  // the M axes, reduced together
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        a[3 * m] = dx[m];
+        a[3 * m + 1] = dy[m];
+        a[3 * m + 2] = dz[m];
+    }
+    group_sum_n<G, 3 * M>(a);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const float ax = a[3 * m], ay = a[3 * m + 1], az = a[3 * m + 2];
+        const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+        const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec[m].c, rec[m].s, rec[m].C);
+        const float x = dx[m], y = dy[m], z = dz[m];
+        const uint32_t a = addr[m];
+        lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
+        lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
+        lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
+    }
+}
+
+// Perturbations kk0 .. kk0 + kcount - 1 of a batch for G >= 25, M at a time:
+// a chunk of M whose sub-chains are pairwise disjoint in every group of the
+// wave runs as one perturb_multi, any other chunk one perturbation at a time.
+// rec(t) returns the record of perturbation t of the batch (t < G).
+template<int G, int BLOCK, bool SPLIT, int M, class RecOf>
+__device__ __forceinline__ void perturb_batch_multi(uint32_t laneB, uint32_t j,
+      const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t kcount, RecOf rec)
+{
+    for (uint32_t kk = 0; kk < kcount; kk += M) {
+        DrawRec r[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) r[t] = rec(min(kk + (uint32_t)t, (uint32_t)G - 1u));
+        const bool ok = kcount - kk >= (uint32_t)M && disjoint_records<M>(r);
+        if (__builtin_amdgcn_ballot_w64(!ok) == 0) {
+            perturb_multi<G, BLOCK, M>(laneB, j, scrB[0], r);
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; ++t)
+                if (kk + (uint32_t)t < kcount) perturb_one<G, BLOCK, SPLIT>(laneB, j, scrB, r[t]);
+        }
+    }
+}
+
+// Perturbations applied together by perturb_batch_multi (groups of G >= 25).
+constexpr int kMultiPerturb = 4;
+
 // Draw record of the lane whose wave-lane byte index is srcB (ds_bpermute: a
 // cross-lane read through the LDS crossbar that uses no LDS storage).
 __device__ __forceinline__ DrawRec bperm_rec(int srcB, const DrawRec &r)
@@ -127,6 +220,9 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
                     if ((uint32_t)kk >= kcount) break;
                     perturb_one<G, BLOCK, true>(laneB, j, scrB, recs[kk]);
                 }
+            } else if constexpr (G >= 25) {
+                perturb_batch_multi<G, BLOCK, true, kMultiPerturb>(laneB, j, scrB, kcount,
+                      [&](uint32_t t) { return bperm_rec(firstB + 4 * (int)t, mine); });
             } else {
                 for (uint32_t kk = 0; kk < kcount; ++kk)
                     perturb_one<G, BLOCK, true>(laneB, j, scrB, bperm_rec(firstB + 4 * (int)kk, mine));
@@ -148,6 +244,10 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
                     if ((uint32_t)kk >= kcount) break;
                     perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
                 }
+            } else if constexpr (G >= 25) {
+                perturb_batch_multi<G, BLOCK, false, kMultiPerturb>(laneB, j, scrB, kcount,
+                      [&](uint32_t t) { return load_rec(drawbuf + (groupBase + t) * 4); });
+                wave_sync_lds();
             } else {
                 for (uint32_t kk = 0; kk < kcount; ++kk)
                     perturb_one<G, BLOCK>(laneB, j, scrB, load_rec(drawbuf + (groupBase + kk) * 4));
@@ -157,38 +257,118 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
     }
 }
 
+// Copies this lane's segments of a curve in global memory (component c of
+// segment i at src + i*segStride + c*compStride) into its state columns
+// (slot s, component c at s*3*BLOCK + c*BLOCK + lane) with direct global ->
+// LDS loads: no register round trip, so all of the path's 3*S loads are in
+// flight at once instead of one slot per memory latency.  Each wave fills its
+// own lanes' columns and waits for its loads before it goes on; the wave's
+// earlier reads of these rows have completed (their values were used).
+template<int G, int BLOCK>
+__device__ __forceinline__ void load_state(float *smem, const float *src, uint64_t segStride, uint64_t compStride,
+      uint32_t S, uint32_t N, uint32_t lane, uint32_t j)
+{
+    constexpr uint32_t SLOT = 3u * BLOCK;
+    float *rows = smem + (lane & ~63u);  // the wave's first lane: M0 of the LDS writes
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint32_t i = s * G + j;
+        if (i < N) {
+            const float *a = src + (uint64_t)i * segStride;
+            __builtin_amdgcn_global_load_lds(a, rows + s * SLOT, 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(a + compStride, rows + s * SLOT + BLOCK, 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(a + 2 * compStride, rows + s * SLOT + 2 * BLOCK, 4, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // every counter, vmcnt included: the LDS writes have landed
+    wave_sync_lds();
+}
+
 struct PathLog {
     FF lsum;         // sum of log2 segment weights (valid joints)
     uint32_t flags;  // 1 = NaN, 2 = -inf, 4 = +inf segment log
 };
 
-// Weighting of the state in LDS (G > 1): tangents (CurvePerturbUtils.cpp:147-154),
-// curvatures (:219-243), lookup and log (PathWeighters.cpp:41-96); lane j takes
-// joints i = j mod G; every lane of the group returns the group's totals.
+// log2 of the weight of joint i (segments i and i+1) of the state in LDS:
+// tangents (CurvePerturbUtils.cpp:147-154), curvature (:219-243), lookup
+// (PathWeighters.cpp:41-82); segment i sits in lane i mod G of the group.
+// One joint's weighting with its by-products: the curvature and the unit
+// tangent of its first segment (the exit-direction bin uses t_{N-2}).
+struct JointEval {
+    float l;            // log2 of the segment weight
+    float kappa;
+    float t0, t1, t2;   // unit tangent of segment i
+};
+
 template<int G, int BLOCK>
+__device__ __forceinline__ JointEval joint_eval(const float *smem, uint32_t lane, uint32_t groupBase, uint32_t i,
+      float invDs, const TableView &tv, uint32_t &clamped)
+{
+    constexpr uint32_t SLOT = 3u * BLOCK;
+    const uint32_t ai = (i / G) * SLOT + lane;
+    JointEval r;
+    unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], r.t0, r.t1, r.t2);
+    const uint32_t i1 = i + 1;
+    const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
+    const float b0 = smem[bi], b1 = smem[bi + BLOCK], b2 = smem[bi + 2 * BLOCK];
+    r.kappa = joint_diff(r.t0, r.t1, r.t2, b0, b1, b2, inv_norm(b0, b1, b2), invDs).kappa;
+    r.l = log2f(lookup_weight(tv, r.kappa, clamped));
+    return r;
+}
+
+// Per-joint by-products are not wanted.
+struct NoJointOutput {
+    __device__ __forceinline__ void operator()(uint32_t, const JointEval &) const {}
+};
+
+// Adds a joint's log2 weight l to the path's sum, or records it in the flags
+// when it is not finite (PathWeighters.cpp:88-96), without branches; a joint
+// with use = false changes nothing.
+__device__ __forceinline__ void add_joint_log(FF &lsum, uint32_t &flags, float l, bool use)
+{
+    const bool nan = !(l == l), ninf = l == -INFINITY, pinf = l == INFINITY;
+    const uint32_t f = nan ? 1u : (ninf ? 2u : (pinf ? 4u : 0u));
+    flags |= use ? f : 0u;
+    const FF t = ff_add_f(lsum, l);
+    const bool take = use & (f == 0u);
+    lsum.hi = take ? t.hi : lsum.hi;
+    lsum.lo = take ? t.lo : lsum.lo;
+}
+
+// Joints a lane weighs per pass of weigh_state.
+constexpr int kJointsPerPass = 4;
+
+// Weighting of the state in LDS (G > 1): lane j takes joints i = j mod G,
+// kJointsPerPass per pass (i, i + G, ...) so that their loads and lookups
+// overlap; the sum keeps the joint order.  out(i, joint) receives every
+// joint's by-products (per-joint outputs of the forward pass).  Every lane of
+// the group returns the group's totals.
+template<int G, int BLOCK, class Out = NoJointOutput>
 __device__ __forceinline__ PathLog weigh_state(const float *smem, uint32_t lane, uint32_t j, uint32_t groupBase,
-      uint32_t N, float invDs, const TableView &tv, uint32_t &clamped)
+      uint32_t N, float invDs, const TableView &tv, uint32_t &clamped, Out out = Out())
 {
     static_assert(G > 1, "weigh_state handles groups of two or more lanes");
-    constexpr uint32_t SLOT = 3u * BLOCK;
+    constexpr int U = kJointsPerPass;
     FF lsum { 0.0f, 0.0f };
     uint32_t flags = 0;
-    for (uint32_t i = j; i + 1 < N; i += G) {
-        const uint32_t ai = (i / G) * SLOT + lane;
-        float ta0, ta1, ta2, tb0, tb1, tb2;
-        unit_tangent(smem[ai], smem[ai + BLOCK], smem[ai + 2 * BLOCK], ta0, ta1, ta2);
-        const uint32_t i1 = i + 1;
-        const uint32_t bi = (i1 / G) * SLOT + groupBase + (i1 & (uint32_t)(G - 1));
-        unit_tangent(smem[bi], smem[bi + BLOCK], smem[bi + 2 * BLOCK], tb0, tb1, tb2);
-        const float e0 = (tb0 - ta0) * invDs;
-        const float e1 = (tb1 - ta1) * invDs;
-        const float e2 = (tb2 - ta2) * invDs;
-        const float kappa = sqrtf(fmaf(e0, e0, fmaf(e1, e1, e2 * e2)));
-        const float l = log2f(lookup_weight(tv, kappa, clamped));
-        if (isnan(l)) flags |= 1u;
-        else if (l == -INFINITY) flags |= 2u;
-        else if (l == INFINITY) flags |= 4u;
-        else lsum = ff_add_f(lsum, l);
+    for (uint32_t i = j; i + 1 < N; i += U * G) {
+        // A slot past the last joint recomputes joint i and its result is
+        // discarded.
+        JointEval e[U];
+        bool use[U];
+        uint32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t iu = i + (uint32_t)u * G;
+            use[u] = iu + 1 < N;
+            c[u] = 0;
+            e[u] = joint_eval<G, BLOCK>(smem, lane, groupBase, use[u] ? iu : i, invDs, tv, c[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            add_joint_log(lsum, flags, e[u].l, use[u]);
+            clamped += use[u] ? c[u] : 0u;
+            if (use[u]) out(i + (uint32_t)u * G, e[u]);
+        }
     }
     return { group_sum_ff<G>(lsum), group_or<G>(flags) };
 }

@@ -170,9 +170,11 @@ int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited);
  * meet) makes the path invalid.  out: the partials over paths
  * [pathOffset, pathOffset + numPaths) (the reference's estimate is
  * sum / num_valid); minLog10W (nullable): the smallest valid log10 weight;
- * perPathLog10 (nullable): every path's log10 weight, NaN when invalid. */
+ * perPathLog10 (nullable): every path's log10 weight, NaN when invalid;
+ * perJointCurvature (nullable): numPaths * (N - 1) joint curvatures, path-major
+ * (check output, as in twisty_hip_run). */
 int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      twisty_hip_partials *out, double *minLog10W, float *perPathLog10);
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10, float *perJointCurvature);
 
 /* Multi-ds tables for twisty_hip_run_generated_multi: numTables tables of
  * numBins floats, tableMeta 4 floats per table (ds, minCurvature,
@@ -190,7 +192,8 @@ int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32
  * absorption with the table's ds.  Otherwise as twisty_hip_run_generated (the
  * boundary comes from set_curve; set_tables is not needed). */
 int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10);
+      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10,
+      float *perJointCurvature);
 
 /* Burn-in of the following twisty_hip_run calls: every chain first applies
  * pathsPerChain unweighted paths of K perturbations (draws keyed by path id

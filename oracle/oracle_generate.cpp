@@ -46,6 +46,15 @@ namespace {
 using V = V3<double>;
 
 V cross(V a, V b) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+
+// Smallest distance of any branch decision of a path's recursion from its
+// threshold (the 0.001 tolerances, the sphere-meeting tests, the frame's
+// axis-choice limits).  Arithmetic of another precision can take the other
+// branch only where this is small.
+struct Margin {
+    double m = INFINITY;
+    void see(double q, double threshold) { m = std::min(m, std::fabs(q - threshold)); }
+};
 V mid(V a, V b) { return scale(add(a, b), 0.5); }
 V nanv() { return { NAN, NAN, NAN }; }
 
@@ -66,9 +75,10 @@ GenWords gen_words(uint64_t seed, uint64_t path, uint32_t node, uint32_t sub)
 
 // Frame around z (ExperimentBase.cpp:868-875): x = z x e, y = z x x, with
 // e = +x unless |z . x| > limit, then +y.
-void frame(V z, double limit, V &x, V &y)
+void frame(V z, double limit, V &x, V &y, Margin &mg)
 {
     V other { 1.0, 0.0, 0.0 };
+    mg.see(std::fabs(dot(z, other)), limit);
     if (std::fabs(dot(z, other)) > limit) other = { 0.0, 1.0, 0.0 };
     x = normalized(cross(z, other));
     y = normalized(cross(z, x));
@@ -89,9 +99,11 @@ void phi_of(double u, double vmax, double &c, double &s)
 
 // ResolveTwoSegments (ExperimentBase.cpp:617-698): the middle point of two
 // segments of length ds between L and R.
-V resolve_two(V L, V R, double ds, const GenWords &g)
+V resolve_two(V L, V R, double ds, const GenWords &g, Margin &mg)
 {
     const double d = norm(sub(R, L));
+    mg.see(std::fabs(2.0 * ds - d), 0.001);
+    mg.see(d, 0.001);
     if (std::fabs(2.0 * ds - d) < 0.001) return mid(L, R);
     if (2.0 * ds < d) return nanv();
     const double theta = 2.0 * kPi * g.u1;
@@ -102,6 +114,7 @@ V resolve_two(V L, V R, double ds, const GenWords &g)
     }
     const V n = scale(sub(R, L), 1.0 / d);
     V other { 1.0, 0.0, 0.0 };
+    mg.see(std::fabs(dot(n, other)), 0.99);
     if (std::fabs(dot(n, other)) >= 0.99) other = { 0.0, 1.0, 0.0 };
     const V m = normalized(cross(n, other));
     const double h = ds > 0.5 * d ? std::sqrt(ds * ds - 0.25 * d * d) : 0.0;
@@ -111,9 +124,11 @@ V resolve_two(V L, V R, double ds, const GenWords &g)
 
 // ResolveThreeSegments (ExperimentBase.cpp:702-804): the first of the two
 // points between L and R; the second follows from resolve_two.
-V resolve_three_first(V L, V R, double ds, const GenWords &g)
+V resolve_three_first(V L, V R, double ds, const GenWords &g, Margin &mg)
 {
     const double d = norm(sub(R, L));
+    mg.see(std::fabs(3.0 * ds - d), 0.001);
+    mg.see(d, 0.001);
     if (std::fabs(3.0 * ds - d) < 0.001) return add(L, scale(sub(R, L), 1.0 / 3.0));
     if (3.0 * ds < d) return nanv();
     const double theta = 2.0 * kPi * g.u1;
@@ -124,14 +139,16 @@ V resolve_three_first(V L, V R, double ds, const GenWords &g)
     }
     const V z = scale(sub(R, L), 1.0 / d);
     V x, y;
-    frame(z, 0.999, x, y);
+    frame(z, 0.999, x, y, mg);
     const double r1 = ds, r2 = 2.0 * ds;
     double vmax;
+    mg.see(d + r1, r2);
     if (d + r1 < r2) {
         vmax = 1.0;  // phiExtent = pi
     } else {
         const double h = 0.5 + (r1 * r1 - r2 * r2) / (2.0 * d * d);
         const double t = r1 * r1 - h * h * d * d;
+        mg.see(std::fabs(t), 0.001);
         double a;
         if (std::fabs(t) < 0.001) a = 0.0;
         else if (t < 0.0) return nanv();  // the reference's NaN phi extent
@@ -148,16 +165,18 @@ V resolve_three_first(V L, V R, double ds, const GenWords &g)
 
 // ResolveEvenNumberOfSegments (ExperimentBase.cpp:806-983): the centre point
 // of n segments between L and R.
-V resolve_even_centre(uint32_t n, V L, V R, double ds, const GenWords &g)
+V resolve_even_centre(uint32_t n, V L, V R, double ds, const GenWords &g, Margin &mg)
 {
     const double d = norm(sub(R, L));
+    mg.see(std::fabs((double)n * ds - d), 0.001);
+    mg.see(d, 0.001);
     if (std::fabs((double)n * ds - d) < 0.001) return mid(L, R);
     const double rps = ds * (double)(n / 2);
     if (2.0 * rps < d) return nanv();
     const double theta = 2.0 * kPi * g.u1;
     V x, y;
     const V z = normalized(sub(R, L));
-    frame(z, 0.999, x, y);
+    frame(z, 0.999, x, y, mg);
     if (d < 0.001) {
         double c, s;
         phi_of(g.u0, 1.0, c, s);
@@ -195,9 +214,11 @@ int oracle_generate_levels(uint32_t N, uint32_t *leafSpan)
 }
 
 // One path's points (N+1) by the recursion; false when a step fails.
+// margin (nullable): the smallest branch-decision margin of the path (Margin).
 bool GeneratePath(const double bc12[12], double ds, uint32_t N, int levels, uint32_t leafSpan, uint64_t seed,
-      uint64_t path, std::vector<V> &p)
+      uint64_t path, std::vector<V> &p, double *margin = nullptr)
 {
+    Margin mg;
     p.assign(N + 1, V { 0.0, 0.0, 0.0 });
     const V s0 { bc12[0], bc12[1], bc12[2] }, sd { bc12[3], bc12[4], bc12[5] };
     const V e0 { bc12[6], bc12[7], bc12[8] }, ed { bc12[9], bc12[10], bc12[11] };
@@ -211,19 +232,21 @@ bool GeneratePath(const double bc12[12], double ds, uint32_t N, int levels, uint
         const uint32_t nodes = 1u << lv, span = n0 >> lv;
         for (uint32_t q = 0; q < nodes; ++q) {
             const uint32_t l = 1 + q * span;
-            p[l + span / 2] = resolve_even_centre(span, p[l], p[l + span], ds, gen_words(seed, path, nodes + q, 0));
+            p[l + span / 2] =
+                  resolve_even_centre(span, p[l], p[l + span], ds, gen_words(seed, path, nodes + q, 0), mg);
         }
     }
     const uint32_t nodes = 1u << levels;
     for (uint32_t q = 0; q < nodes; ++q) {
         const uint32_t l = 1 + q * leafSpan;
         if (leafSpan == 2) {
-            p[l + 1] = resolve_two(p[l], p[l + 2], ds, gen_words(seed, path, nodes + q, 0));
+            p[l + 1] = resolve_two(p[l], p[l + 2], ds, gen_words(seed, path, nodes + q, 0), mg);
         } else {
-            p[l + 1] = resolve_three_first(p[l], p[l + 3], ds, gen_words(seed, path, nodes + q, 0));
-            p[l + 2] = resolve_two(p[l + 1], p[l + 3], ds, gen_words(seed, path, nodes + q, 1));
+            p[l + 1] = resolve_three_first(p[l], p[l + 3], ds, gen_words(seed, path, nodes + q, 0), mg);
+            p[l + 2] = resolve_two(p[l + 1], p[l + 3], ds, gen_words(seed, path, nodes + q, 1), mg);
         }
     }
+    if (margin) *margin = mg.m;
     for (const V &v : p)
         if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z)) return false;
     return true;
@@ -237,12 +260,15 @@ int oracle_generate_levels_c(uint32_t N, uint32_t *leafSpan) { return oracle_gen
 //   bc12: start, start dir, end, end dir (unit); arclength: the boundary's; ds =
 //   arclength / N in float (ExperimentBase.cpp:1253).
 // Outputs (nullable): log10w[n] (path log10 weight without normaliser; NaN when
-// invalid), valid[n], positions[n*(N+1)*3].
+// invalid), valid[n], positions[n*(N+1)*3], curv[n*(N-1)] (the float joint
+// curvatures the lookup uses; written for valid paths), margin[n] (the path's
+// smallest branch-decision margin, Margin).
 // result: [log10 sum_valid 10^w (long double LSE), nValid, nInvalid, min w,
 //   max w, seconds].
 int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t seed, uint64_t pathOffset,
       uint64_t nPaths, const float *table, int32_t tableSize, float ds, float minK, float maxK, float step,
-      float absorption, int threads, double *log10wOut, uint8_t *validOut, double *positionsOut, double *result)
+      float absorption, int threads, double *log10wOut, uint8_t *validOut, double *positionsOut, float *curvOut,
+      double *marginOut, double *result)
 {
     uint32_t leafSpan = 0;
     const int levels = oracle_generate_levels(N, &leafSpan);
@@ -278,7 +304,8 @@ int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t
             std::vector<float> curvF;
             Acc &a = accs[th];
             for (uint64_t li = b; li < e; ++li) {
-                const bool ok = GeneratePath(bc12, dsGen, N, levels, leafSpan, seed, pathOffset + li, p);
+                const bool ok = GeneratePath(bc12, dsGen, N, levels, leafSpan, seed, pathOffset + li, p,
+                      marginOut ? marginOut + li : nullptr);
                 if (positionsOut)
                     for (uint32_t i = 0; i <= N; ++i) {
                         positionsOut[(li * (N + 1) + i) * 3 + 0] = p[i].x;
@@ -286,7 +313,9 @@ int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t
                         positionsOut[(li * (N + 1) + i) * 3 + 2] = p[i].z;
                     }
                 PathWeightValue pw { false, NAN };
-                if (ok) pw = WeightState(p, ps, wt, absorption, tan, curvT, curvF);
+                if (ok)
+                    pw = WeightState(p, ps, wt, absorption, tan, curvT, curvF,
+                          curvOut ? curvOut + li * (uint64_t)(N - 1) : nullptr);
                 if (log10wOut) log10wOut[li] = pw.isValid ? pw.weight : NAN;
                 if (validOut) validOut[li] = pw.isValid ? 1 : 0;
                 if (!pw.isValid) {
@@ -322,10 +351,12 @@ int oracle_generate(const double bc12[12], float arclength, uint32_t N, uint64_t
 // u (maxDs - minDs) + minDs in float (u from node 0's words), the table whose
 // ds is nearest (first on ties, PathWeightUtils.cpp:223-236), curvature with
 // arclength ds N, absorption with the table's ds.  tables: T x bins floats;
-// meta: 4 per table (ds, minK, maxK, step).  Outputs as oracle_generate.
+// meta: 4 per table (ds, minK, maxK, step).  Outputs as oracle_generate, plus
+// tableOut[n] (nullable): the table each path is weighted with.
 int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint64_t pathOffset, uint64_t nPaths,
       const float *tables, int32_t bins, const float *meta, uint32_t numTables, float absorption, float minDs,
-      float maxDs, int threads, double *log10wOut, uint8_t *validOut, double *result)
+      float maxDs, int threads, double *log10wOut, uint8_t *validOut, float *curvOut, double *marginOut,
+      uint32_t *tableOut, double *result)
 {
     uint32_t leafSpan = 0;
     const int levels = oracle_generate_levels(N, &leafSpan);
@@ -374,9 +405,13 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
                 PathSpec ps;
                 ps.N = N;
                 ps.arclength = ds * (float)N;
-                const bool ok = GeneratePath(bc12, (double)ds, N, levels, leafSpan, seed, path, p);
+                if (tableOut) tableOut[li] = best;
+                const bool ok = GeneratePath(bc12, (double)ds, N, levels, leafSpan, seed, path, p,
+                      marginOut ? marginOut + li : nullptr);
                 PathWeightValue pw { false, NAN };
-                if (ok) pw = WeightState(p, ps, wts[best], absorption, tan, curvT, curvF);
+                if (ok)
+                    pw = WeightState(p, ps, wts[best], absorption, tan, curvT, curvF,
+                          curvOut ? curvOut + li * (uint64_t)(N - 1) : nullptr);
                 if (log10wOut) log10wOut[li] = pw.isValid ? pw.weight : NAN;
                 if (validOut) validOut[li] = pw.isValid ? 1 : 0;
                 if (!pw.isValid) {

@@ -80,10 +80,12 @@ def _declare(L):
     L.oracle_generate_levels_c.argtypes = [u32, C.POINTER(u32)]
     L.oracle_generate_levels_c.restype = C.c_int
     L.oracle_generate.argtypes = [_p(np.float64), f32, u32, u64, u64, u64, _p(np.float32), i32, f32, f32, f32, f32,
-                                  f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _p(np.float64)]
+                                  f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  _p(np.float64)]
     L.oracle_generate.restype = C.c_int
     L.oracle_generate_multi.argtypes = [_p(np.float64), u32, u64, u64, u64, _p(np.float32), i32, _p(np.float32), u32,
-                                        f32, f32, f32, C.c_int, C.c_void_p, C.c_void_p, _p(np.float64)]
+                                        f32, f32, f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, _p(np.float64)]
     L.oracle_generate_multi.restype = C.c_int
 
 
@@ -222,17 +224,22 @@ def generate_levels(N):
 
 
 def generate(bc, arclength, N, n_paths, table, absorption, seed=1, path_offset=0, threads=None, want_w=True,
-             want_pos=False):
-    """Independent-path estimator (oracle_generate.cpp): per-path log10 weights and the sums."""
+             want_pos=False, want_joints=False):
+    """Independent-path estimator (oracle_generate.cpp): per-path log10 weights and the sums;
+    want_joints adds the float joint curvatures ("curv", NaN rows for invalid paths) and each
+    path's smallest branch-decision margin ("margin")."""
     threads = threads or os.cpu_count() or 1
     b = np.ascontiguousarray(bc, np.float64)
     res = np.zeros(6, np.float64)
     w = np.zeros(n_paths, np.float64) if want_w else None
     v = np.zeros(n_paths, np.uint8) if want_w else None
     pos = np.zeros(n_paths * (N + 1) * 3, np.float64) if want_pos else None
+    curv = np.full(n_paths * (N - 1), np.nan, np.float32) if want_joints else None
+    margin = np.zeros(n_paths, np.float64) if want_joints else None
     ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
     rc = lib().oracle_generate(b, arclength, N, seed, path_offset, n_paths, table.table, len(table.table), table.ds,
-                               table.minK, table.maxK, table.step, absorption, threads, ptr(w), ptr(v), ptr(pos), res)
+                               table.minK, table.maxK, table.step, absorption, threads, ptr(w), ptr(v), ptr(pos),
+                               ptr(curv), ptr(margin), res)
     if rc != 0:
         raise ValueError("oracle_generate rejected its arguments")
     out = {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
@@ -242,24 +249,34 @@ def generate(bc, arclength, N, n_paths, table, absorption, seed=1, path_offset=0
         out["valid"] = v.astype(bool)
     if want_pos:
         out["positions"] = pos.reshape(n_paths, N + 1, 3)
+    if want_joints:
+        out["curv"] = curv.reshape(n_paths, N - 1)
+        out["margin"] = margin
     return out
 
 
 def generate_multi(bc, N, n_paths, tables, meta, absorption, min_ds, max_ds, seed=1, path_offset=0, threads=None):
-    """Multi-arclength independent-path estimator (oracle_generate.cpp): tables (T, bins), meta (T, 4)."""
+    """Multi-arclength independent-path estimator (oracle_generate.cpp): tables (T, bins), meta (T, 4).
+    Also returns the float joint curvatures ("curv"), each path's smallest branch-decision margin
+    ("margin") and the table each path is weighted with ("table")."""
     threads = threads or os.cpu_count() or 1
     t = np.ascontiguousarray(tables, np.float32)
     m = np.ascontiguousarray(meta, np.float32)
     res = np.zeros(6, np.float64)
     w = np.zeros(n_paths, np.float64)
     v = np.zeros(n_paths, np.uint8)
+    curv = np.full(n_paths * (N - 1), np.nan, np.float32)
+    margin = np.zeros(n_paths, np.float64)
+    tab = np.zeros(n_paths, np.uint32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
     rc = lib().oracle_generate_multi(np.ascontiguousarray(bc, np.float64), N, seed, path_offset, n_paths, t,
-                                     t.shape[1], m, t.shape[0], absorption, min_ds, max_ds, threads,
-                                     w.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), res)
+                                     t.shape[1], m, t.shape[0], absorption, min_ds, max_ds, threads, p(w), p(v),
+                                     p(curv), p(margin), p(tab), res)
     if rc != 0:
         raise ValueError("oracle_generate_multi rejected its arguments")
     return {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
-            "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool)}
+            "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool),
+            "curv": curv.reshape(n_paths, N - 1), "margin": margin, "table": tab}
 
 
 def exit_bins(tangents, e, n_theta, n_phi):

@@ -2,20 +2,77 @@
 
 ExperimentBase::MSegmentPathGenerationMC (Experiments/ExperimentBase.cpp:
 1246-1386): both sides build every path from the same Philox words by the
-recursion of :617-983 and weigh it with the same table.  FP32 and FP64 can take
-different branches where a span sits within the reference's 0.001 straight
-tolerance, and a joint can fall on either side of a table bin edge, so the
-per-path bar (1e-5 relative) is asserted on 99% of the paths and the median;
-the valid/invalid split, path offsets and the aggregate follow.
+recursion of :617-983 and weigh it with the same table.  Every path is held to
+the forward pass's rule (check_generated): 1e-5 relative, or, where a joint's
+FP32 and FP64 curvatures fall in different table bins, 1e-5 plus the lerp jumps
+there.  One exception is allowed, and only where it can occur: a path whose
+recursion made a branch decision (a 0.001 tolerance, a sphere-meeting test, a
+frame's axis limit) within MARGIN of its threshold in FP64 may take the other
+branch in FP32 (other geometry, or valid against invalid); such paths must
+stay under 0.1%.  The weighting stage is checked on the kernel's own
+curvatures, and path offsets and the aggregate follow.
 """
 import math
+import types
 
 import numpy as np
 import pytest
 
+from tests.lerp import joint_log10w
+from tests.parity import assert_path_rule, path_rule
 from twisty_amd import capi, host
 
 pytestmark = pytest.mark.gpu
+
+REL_STAGE = 2e-6     # log10 w against the sum of its joints' lerped log weights
+# A branch decision this close to its threshold in FP64 can go the other way in
+# FP32: the geometry's points carry errors of at most a few 1e-7 (hardware
+# reciprocal and square root, about 1 ulp each, over the recursion's levels).
+MARGIN = 1e-6
+# Largest FP32-vs-FP64 joint curvature difference on paths whose branches all
+# agree (hardware-precision geometry: tangent errors of ~1e-6 over ds ~ 0.02).
+KAPPA_TOL = 2e-2
+
+
+def check_generated(tag, gpu, ref, tables, tab_idx, absorption):
+    """The per-path rule (tests/parity.py) on every path whose branch decisions
+    are not marginal; tables[tab_idx[p]] weighs path p."""
+    g, r = gpu["log10w"].astype(np.float64), ref["log10w"]
+    n = len(r)
+    gv, rv = ~np.isnan(g), ref["valid"]
+    marginal = ref["margin"] < MARGIN
+    # valid against invalid, and zero weight against nonzero, only where a decision is marginal
+    assert np.all((gv == rv) | marginal), np.flatnonzero((gv != rv) & ~marginal)[:10]
+    both = gv & rv
+    assert np.all((np.isneginf(g) == np.isneginf(r)) | ~both | marginal)
+    fin = both & np.isfinite(r) & np.isfinite(g)
+    dk = np.where(fin[:, None], np.abs(gpu["curv"].astype(np.float64) - ref["curv"].astype(np.float64)), 0.0).max(1)
+    apart = fin & (dk > KAPPA_TOL)
+    # geometry apart only where a decision is marginal
+    assert not np.any(apart & ~marginal), np.flatnonzero(apart & ~marginal)[:10]
+    held = fin & ~apart
+    worst = {"straddle_paths": 0, "max_rel_clean": 0.0, "max_excess": -1.0}
+    stage = 0.0
+    for t in np.unique(tab_idx):
+        sel = tab_idx == t
+        st = path_rule(g[sel], r[sel], held[sel], gpu["curv"][sel], ref["curv"][sel], tables[t], absorption)
+        assert_path_rule(st)
+        worst["straddle_paths"] += st["straddle_paths"]
+        worst["max_rel_clean"] = max(worst["max_rel_clean"], st["max_rel_clean"])
+        worst["max_excess"] = max(worst["max_excess"], st["max_excess"])
+        # weighting stage: the kernel's log10 w against its own curvatures' joint weights
+        gf = (gv & np.isfinite(g))[sel]
+        if gf.any():
+            lg, _ = joint_log10w(gpu["curv"][sel][gf], tables[t], absorption)
+            stage = max(stage, float((np.abs(g[sel][gf] - lg.sum(1)) / np.abs(g[sel][gf])).max()))
+    # the exceptions taken: validity, zero weight or geometry apart
+    taken = (gv != rv) | (both & (np.isneginf(g) != np.isneginf(r))) | apart
+    print(f"[{tag}] paths {n}, valid gpu {int(gv.sum())} oracle {int(rv.sum())}, marginal {int(marginal.sum())}, "
+          f"exceptions {int(taken.sum())} (margins {np.sort(ref['margin'][taken])[:5]}), {worst}, "
+          f"max dkappa held {dk[held].max() if held.any() else 0.0:.2e}, stage {stage:.2e}", flush=True)
+    assert stage <= REL_STAGE
+    assert taken.sum() <= max(2, n // 1000)
+
 
 CASES = [(6, 1.4, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))),
          (10, 2.5, ((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1))),
@@ -42,15 +99,10 @@ def _setup(oracle, N, L, geom):
 def test_generated_paths_match_oracle(hip_device, oracle, N, L, geom):
     n = 4096
     ctx, t, bc12 = _setup(oracle, N, L, geom)
-    g = ctx.run_generated(n, seed=7, per_path=True)
-    ref = oracle.generate(bc12, L, N, n, t, 0.004, seed=7)
-    gw, rw = g["log10w"].astype(np.float64), ref["log10w"]
-    both = ~np.isnan(gw) & ref["valid"]
-    rel = np.abs(gw[both] - rw[both]) / np.abs(rw[both])
-    print(f"[generated N={N}] valid gpu {g['num_valid']} oracle {ref['n_valid']}, median rel {np.median(rel):.2e}, "
-          f"99% {np.quantile(rel, 0.99):.2e}, max {rel.max():.2e}", flush=True)
-    assert abs(g["num_valid"] - ref["n_valid"]) <= max(2, n // 1000)
-    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.99) <= 1e-5
+    g = ctx.run_generated(n, seed=7, per_path=True, per_joint=True)
+    ref = oracle.generate(bc12, L, N, n, t, 0.004, seed=7, want_joints=True)
+    gw = g["log10w"].astype(np.float64)
+    check_generated(f"generated N={N}", g, ref, [t], np.zeros(n, np.int64), 0.004)
     assert g["num_paths"] == n and g["num_valid"] + g["num_invalid"] == n
     # aggregates over the same valid paths
     lse = lambda v: v.max() + math.log10(np.sum(10.0 ** (v - v.max())))
@@ -150,15 +202,11 @@ def test_generated_multi_ds_matches_oracle(hip_device, oracle):
     ctx.set_multi_tables(tables, meta, 0.004, N)
     lo, hi = 0.045, 0.075
     n = 4096
-    g = ctx.run_generated_multi(n, lo, hi, seed=5, per_path=True)
+    g = ctx.run_generated_multi(n, lo, hi, seed=5, per_path=True, per_joint=True)
     ref = oracle.generate_multi(bc13[:12].astype(np.float64), N, n, tables, meta, 0.004, lo, hi, seed=5)
-    gw, rw = g["log10w"].astype(np.float64), ref["log10w"]
-    both = ~np.isnan(gw) & ref["valid"]
-    rel = np.abs(gw[both] - rw[both]) / np.abs(rw[both])
-    print(f"[generated multi-ds N={N}] valid gpu {g['num_valid']} oracle {ref['n_valid']}, median rel "
-          f"{np.median(rel):.2e}, 99% {np.quantile(rel, 0.99):.2e}", flush=True)
-    assert abs(g["num_valid"] - ref["n_valid"]) <= max(2, n // 1000)
-    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.99) <= 1e-5
+    views = [types.SimpleNamespace(table=tables[i], ds=float(meta[i, 0]), minK=float(meta[i, 1]),
+                                   maxK=float(meta[i, 2]), step=float(meta[i, 3])) for i in range(len(tables))]
+    check_generated(f"generated multi-ds N={N}", g, ref, views, ref["table"].astype(np.int64), 0.004)
     assert abs(g["log10_sum_w"] - ref["log10_sum_w"]) < 1e-4 * abs(ref["log10_sum_w"])
     ctx.close()
 

@@ -555,6 +555,44 @@ def test_generated_paths_structure(oracle, N, L, s, sd, e, ed):
     assert exact_leaves > 0
 
 
+@pytest.mark.parametrize("N,L,s,sd,e,ed", GEN_CASES[:4])
+def test_generated_joint_outputs(oracle, N, L, s, sd, e, ed):
+    """The curvatures and branch margins the GPU parity rule reads: curv is the
+    float of |t_{i+1} - t_i| N / L over the returned points' unit tangents, and
+    margin is at most the distance of each leaf's and centre's 0.001 tolerance
+    test (|span ds - d| against 0.001, d against 0.001), recomputed from the
+    returned points, from its threshold; a path whose margin sits on one of
+    these distances equals it."""
+    bc = oracle.bc12(s, np.array(sd) / np.linalg.norm(sd), e, np.array(ed) / np.linalg.norm(ed))
+    ds, t = _gen_setup(oracle, N, L, bc)
+    n = 300
+    r = oracle.generate(bc, L, N, n, t, 0.004, seed=5, want_pos=True, want_joints=True)
+    P = r["positions"]
+    seg = np.diff(P, axis=1)
+    tan = seg / np.linalg.norm(seg, axis=2, keepdims=True)
+    kap = np.linalg.norm(np.diff(tan, axis=1), axis=2) * float(np.float32(N) / np.float32(L))
+    v = r["valid"]
+    assert v.all()
+    assert np.abs(r["curv"][v].astype(np.float64) - kap[v]).max() <= 1e-6 * max(1.0, kap.max())
+    k, leaf = oracle.generate_levels(N)
+    n0 = N - 2
+    dist = []
+    for lv in range(k):
+        span = n0 >> lv
+        for q in range(1 << lv):
+            lft = 1 + q * span
+            d = np.linalg.norm(P[:, lft + span] - P[:, lft], axis=1)
+            dist += [np.abs(np.abs(span * ds - d) - 0.001), np.abs(d - 0.001)]
+    for q in range(1 << k):
+        lft = 1 + q * leaf
+        d = np.linalg.norm(P[:, lft + leaf] - P[:, lft], axis=1)
+        dist += [np.abs(np.abs(leaf * ds - d) - 0.001), np.abs(d - 0.001)]
+    lowest = np.min(np.stack(dist), axis=0)
+    m = r["margin"]
+    assert (m > 0).all() and (m <= lowest + 1e-12).all()
+    assert np.isclose(m, lowest, rtol=0, atol=1e-12).any()
+
+
 def test_generated_two_segment_angle_is_the_drawn_theta(oracle):
     """N = 6: the leaf point sits on the circle of radius sqrt(ds^2 - d^2/4)
     around its chord at the angle 2 pi u1 of its node's Philox words, measured

@@ -115,8 +115,8 @@ def _declare(L):
     L.twisty_hip_set_adjoint_roulette.argtypes = [vp, f32]
     L.twisty_hip_last_adjoint_paths.argtypes = [vp, C.POINTER(u64)]
     L.twisty_hip_set_multi_tables.argtypes = [vp, vp, u32, u32, vp, f32, u32]
-    L.twisty_hip_run_generated_multi.argtypes = [vp, u64, u64, u64, f32, f32, vp, C.POINTER(C.c_double), vp]
-    L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp]
+    L.twisty_hip_run_generated_multi.argtypes = [vp, u64, u64, u64, f32, f32, vp, C.POINTER(C.c_double), vp, vp]
+    L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp, vp]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
     L.twisty_hip_set_exit_histogram.argtypes = [vp, u32, u32]
@@ -229,17 +229,20 @@ class Context:
         self._check(lib().twisty_hip_last_adjoint_paths(self._h, C.byref(v)))
         return v.value
 
-    def run_generated(self, num_paths, seed=1, path_offset=0, per_path=False):
+    def run_generated(self, num_paths, seed=1, path_offset=0, per_path=False, per_joint=False):
         """Independent-path estimator (paths built by recursive sphere intersection)."""
         out = Partials()
         mn = C.c_double()
         pp = np.empty(num_paths, np.float32) if per_path else None
+        pj = np.empty(num_paths * (self.N - 1), np.float32) if per_joint else None
         self._check(lib().twisty_hip_run_generated(self._h, num_paths, path_offset, seed, C.byref(out), C.byref(mn),
-                                                   _ptr(pp)))
+                                                   _ptr(pp), _ptr(pj)))
         res = out.as_dict()
         res["min_log10_w"] = mn.value
         if per_path:
             res["log10w"] = pp
+        if per_joint:
+            res["curv"] = pj.reshape(num_paths, self.N - 1)
         return res
 
     def set_multi_tables(self, tables, meta, absorption, num_segments):
@@ -249,17 +252,21 @@ class Context:
         self._check(lib().twisty_hip_set_multi_tables(self._h, _ptr(t), t.shape[1], t.shape[0], _ptr(m), absorption,
                                                       num_segments))
 
-    def run_generated_multi(self, num_paths, min_ds, max_ds, seed=1, path_offset=0, per_path=False):
+    def run_generated_multi(self, num_paths, min_ds, max_ds, seed=1, path_offset=0, per_path=False,
+                            per_joint=False):
         """Independent paths with ds drawn per path and the nearest cached table."""
         out = Partials()
         mn = C.c_double()
         pp = np.empty(num_paths, np.float32) if per_path else None
+        pj = np.empty(num_paths * (self.N - 1), np.float32) if per_joint else None
         self._check(lib().twisty_hip_run_generated_multi(self._h, num_paths, path_offset, seed, min_ds, max_ds,
-                                                         C.byref(out), C.byref(mn), _ptr(pp)))
+                                                         C.byref(out), C.byref(mn), _ptr(pp), _ptr(pj)))
         res = out.as_dict()
         res["min_log10_w"] = mn.value
         if per_path:
             res["log10w"] = pp
+        if per_joint:
+            res["curv"] = pj.reshape(num_paths, self.N - 1)
         return res
 
     def last_timing(self):

@@ -638,7 +638,8 @@ int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited)
 }
 
 static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      bool multi, float dsLo, float dsHi, twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+      bool multi, float dsLo, float dsHi, twisty_hip_partials *out, double *minLog10W, float *perPathLog10,
+      float *perJointCurvature)
 {
     if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: out is NULL");
     if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_curve first");
@@ -674,6 +675,9 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
     int rc;
     if ((rc = ensure_dev(ctx, ctx->dPartials, ctx->partialsCap, grid, "partials"))) return rc;
     if (perPathLog10 && (rc = ensure_dev(ctx, ctx->dPerPath, ctx->perPathCap, numPaths, "per-path buffer"))) return rc;
+    if (perJointCurvature &&
+          (rc = ensure_dev(ctx, ctx->dPerJoint, ctx->perJointCap, numPaths * (uint64_t)(N - 1), "per-joint buffer")))
+        return rc;
     if ((rc = ensure_dev(ctx, ctx->dGenMin, ctx->genMinCap, 1, "min buffer"))) return rc;
     // the multi-ds form adds each path's own table absorption in the kernel
     const double absorbLog2PerPath =
@@ -682,6 +686,7 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
     p.table = ctx->dTable;
     p.partials = ctx->dPartials;
     p.perPathLog10 = perPathLog10 ? ctx->dPerPath : nullptr;
+    p.perJointCurv = perJointCurvature ? ctx->dPerJoint : nullptr;
     p.minLog10 = ctx->dGenMin;
     p.seed = seed;
     p.pathOffset = pathOffset;
@@ -728,6 +733,9 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
     if (perPathLog10)
         HIP_TRY(ctx, hipMemcpyAsync(perPathLog10, ctx->dPerPath, numPaths * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream));
+    if (perJointCurvature)
+        HIP_TRY(ctx, hipMemcpyAsync(perJointCurvature, ctx->dPerJoint, numPaths * (uint64_t)(N - 1) * sizeof(float),
+                           hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev2));
@@ -743,10 +751,11 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
 }
 
 int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10, float *perJointCurvature)
 {
     if (!ctx) return TWISTY_HIP_ERR_ARG;
-    return run_generated_impl(ctx, numPaths, pathOffset, seed, false, 0.0f, 0.0f, out, minLog10W, perPathLog10);
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, false, 0.0f, 0.0f, out, minLog10W, perPathLog10,
+          perJointCurvature);
 }
 
 int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
@@ -787,12 +796,14 @@ int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32
 }
 
 int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10)
+      float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10,
+      float *perJointCurvature)
 {
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (ctx->multiCount && ctx->haveCurve && ctx->multiSegments != ctx->N)
         return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_multi: set_multi_tables was given another numSegments");
-    return run_generated_impl(ctx, numPaths, pathOffset, seed, true, minDs, maxDs, out, minLog10W, perPathLog10);
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, true, minDs, maxDs, out, minLog10W, perPathLog10,
+          perJointCurvature);
 }
 
 int twisty_hip_set_burn_in(twisty_hip_ctx *ctx, uint32_t pathsPerChain)

@@ -311,7 +311,13 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
             }
         }
         wave_sync_lds();
-        PathLog pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, invDs, tv, clamped);
+        PathLog pl;
+        if (!p.perJointCurv) {
+            pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, invDs, tv, clamped);
+        } else {
+            pl = weigh_state<G, BLOCK>(smem, lane, j, groupBase, N, invDs, tv, clamped,
+                  [&](uint32_t i, const JointEval &je) { p.perJointCurv[local * (uint64_t)(N - 1) + i] = je.kappa; });
+        }
         if constexpr (MULTI) pl.lsum = ff_add_f(pl.lsum, absorbLog2);  // the table's own absorption
         wave_sync_lds();  // joints read other lanes' columns before the next path's points
         if (j == 0) {

@@ -13,6 +13,7 @@ struct GenParams {
     const float *table;     // tableSize floats (device)
     BlockPartial *partials;  // one per block
     float *perPathLog10;     // nullable
+    float *perJointCurv;     // nullable: numPaths x (N - 1) joint curvatures
     float *minLog10;         // one float, +inf before the launch
     uint64_t seed;
     uint64_t pathOffset;

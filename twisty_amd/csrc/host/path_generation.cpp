@@ -57,7 +57,7 @@ std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperi
     if (rc == TWISTY_HIP_OK) rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(arc.data()), N, bc13.data());
     twisty_hip_partials part {};
     double mn = INFINITY;
-    if (rc == TWISTY_HIP_OK) rc = twisty_hip_run_generated(ctx, numExperimentPaths, 0, seed, &part, &mn, nullptr);
+    if (rc == TWISTY_HIP_OK) rc = twisty_hip_run_generated(ctx, numExperimentPaths, 0, seed, &part, &mn, nullptr, nullptr);
     if (rc != TWISTY_HIP_OK) {
         error = twisty_hip_last_error(ctx);
         return std::nullopt;
@@ -137,7 +137,7 @@ std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperi
     twisty_hip_partials part {};
     double mn = INFINITY;
     if (rc == TWISTY_HIP_OK)
-        rc = twisty_hip_run_generated_multi(ctx, numExperimentPaths, 0, seed, minDs, maxDs, &part, &mn, nullptr);
+        rc = twisty_hip_run_generated_multi(ctx, numExperimentPaths, 0, seed, minDs, maxDs, &part, &mn, nullptr, nullptr);
     if (rc != TWISTY_HIP_OK) {
         error = twisty_hip_last_error(ctx);
         return std::nullopt;

@@ -334,12 +334,15 @@ __device__ __forceinline__ void add_joint_log(FF &lsum, uint32_t &flags, float l
     lsum.lo = take ? t.lo : lsum.lo;
 }
 
-// Joints a lane weighs per pass of weigh_state.
-constexpr int kJointsPerPass = 4;
+// Joints a lane weighs per pass of weigh_state.  One is fastest as measured
+// (weighting of 2^24 paths of 1024 segments at 32 lanes per path: 118 ms at
+// one joint per pass, 142 ms at two, 127 ms at four; 64 segments at 4 lanes
+// also favour one): the compiler already overlaps consecutive joints.
+constexpr int kJointsPerPass = 1;
 
 // Weighting of the state in LDS (G > 1): lane j takes joints i = j mod G,
-// kJointsPerPass per pass (i, i + G, ...) so that their loads and lookups
-// overlap; the sum keeps the joint order.  out(i, joint) receives every
+// kJointsPerPass per pass (i, i + G, ...); the sum keeps the joint order, so
+// every kJointsPerPass gives the same bits.  out(i, joint) receives every
 // joint's by-products (per-joint outputs of the forward pass).  Every lane of
 // the group returns the group's totals.
 template<int G, int BLOCK, class Out = NoJointOutput>

@@ -234,12 +234,13 @@ __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts
 // per G (lean or not).  For N = 64 (G = 4) a 640-lane block with the table in
 // LDS measured slower than 512; the lean 768-lane block runs 12 waves per CU.
 // 320 lanes (G = 32, 64) fit five waves where 384 no longer fits: N = 1024
-// 486 -> 469 ms against 256 lanes.
+// 486 -> 469 ms against 256 lanes; 352 lanes (G = 32: eleven paths, the last
+// wave half full) fit too, 433 -> 405 ms at N = 1024, K = 200.
 // ---------------------------------------------------------------------------
 #define TWISTY_FWD_INSTANCES(X)                                                                                   \
     X(1, 128, false) X(1, 192, false) X(2, 192, false) X(2, 256, false) X(2, 320, false) X(4, 256, false)        \
     X(4, 384, false) X(4, 512, false) X(4, 768, true) X(8, 512, false) X(8, 768, false) X(8, 1024, false)         \
-    X(16, 512, false) X(16, 1024, false) X(32, 256, false) X(32, 320, false) X(32, 384, false)                   \
+    X(16, 512, false) X(16, 1024, false) X(32, 256, false) X(32, 320, false) X(32, 352, false) X(32, 384, false)                   \
     X(32, 512, false) X(32, 1024, false) X(64, 256, false) X(64, 320, false) X(64, 512, false)                   \
     X(64, 640, false) X(64, 704, false) X(64, 1024, false)
 

@@ -86,7 +86,7 @@ inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32
     const uint64_t rows = (uint64_t)forward_scratch_row(G, N) + 1;
     const uint64_t floats = rows * 3ull * block + (lean ? 0ull : (uint64_t)tableStride) + histStride +
           (G > 1 && !lean ? 4ull * block : 0ull);
-    const uint64_t reduceBytes = (uint64_t)(block / 64) * sizeof(LseAcc);
+    const uint64_t reduceBytes = (uint64_t)((block + 63) / 64) * sizeof(LseAcc);
     const uint64_t bytes = floats * 4ull;
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);
 }

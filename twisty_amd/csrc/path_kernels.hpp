@@ -70,6 +70,16 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
     }
     // The axis is the chord of >= 2 segments; it vanishes only for a sub-chain
     // that closes exactly, and then the path turns NaN and is counted invalid.
+    if constexpr (WMAX == 1) {
+        // one slot per lane: Rodrigues' form (the one perturb_multi uses)
+        const AxisRotation R = axis_rotation(ax, ay, az, rec.c, rec.s, rec.C);
+        float x = dx[0], y = dy[0], z = dz[0];
+        rotate_rodrigues(R, x, y, z);
+        lds_st(addr[0], x);
+        lds_st(addr[0] + COMPB, y);
+        lds_st(addr[0] + 2 * COMPB, z);
+        return;
+    }
     const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
     const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec.c, rec.s, rec.C);
 #pragma unroll
@@ -145,14 +155,12 @@ __device__ __forceinline__ void perturb_multi(uint32_t laneB, uint32_t j, uint32
     group_sum_n<G, 3 * M>(a);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const float ax = a[3 * m], ay = a[3 * m + 1], az = a[3 * m + 2];
-        const float inv = __builtin_amdgcn_rsqf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
-        const Mat3 R = rotation_from_axis(ax * inv, ay * inv, az * inv, rec[m].c, rec[m].s, rec[m].C);
-        const float x = dx[m], y = dy[m], z = dz[m];
-        const uint32_t a = addr[m];
-        lds_st(a, fmaf(R.m00, x, fmaf(R.m01, y, R.m02 * z)));
-        lds_st(a + COMPB, fmaf(R.m10, x, fmaf(R.m11, y, R.m12 * z)));
-        lds_st(a + 2 * COMPB, fmaf(R.m20, x, fmaf(R.m21, y, R.m22 * z)));
+        const AxisRotation R = axis_rotation(a[3 * m], a[3 * m + 1], a[3 * m + 2], rec[m].c, rec[m].s, rec[m].C);
+        float x = dx[m], y = dy[m], z = dz[m];
+        rotate_rodrigues(R, x, y, z);
+        lds_st(addr[m], x);
+        lds_st(addr[m] + COMPB, y);
+        lds_st(addr[m] + 2 * COMPB, z);
     }
 }
 
@@ -460,8 +468,11 @@ __device__ __forceinline__ void finalize_partials(const BlockPartial *parts, con
 // has read the scratch (the caller's next barrier).
 __device__ __forceinline__ void block_reduce_lse(LseAcc acc, float *scratch, BlockPartial *out)
 {
+    // A block of a size that is not a multiple of 64 ends in a partial wave:
+    // lanes past the block hold nothing and are skipped.
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
+        const bool partner = (threadIdx.x ^ (unsigned)o) < blockDim.x;
         LseAcc b;
         b.m = __shfl_xor(acc.m, o, 64);
         b.s.hi = __shfl_xor(acc.s.hi, o, 64);
@@ -471,10 +482,10 @@ __device__ __forceinline__ void block_reduce_lse(LseAcc acc, float *scratch, Blo
         b.nValid = (uint32_t)__shfl_xor((int)acc.nValid, o, 64);
         b.nInvalid = (uint32_t)__shfl_xor((int)acc.nInvalid, o, 64);
         b.nClamped = (uint32_t)__shfl_xor((int)acc.nClamped, o, 64);
-        lse_merge(acc, b);
+        if (partner) lse_merge(acc, b);
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6, nWaves = (blockDim.x + 63) >> 6;
     LseAcc *sh = reinterpret_cast<LseAcc *>(scratch);
     if ((threadIdx.x & 63) == 0) sh[wave] = acc;
     __syncthreads();

@@ -123,7 +123,10 @@ int twisty_hip_run_noalloc(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t path
  * 0 for any value selects the built-in choice for the current curve.  For
  * lanesPerPath 4, blockSize 768 is the lean kernel (table read from global
  * memory, draw records exchanged between lanes without LDS), the built-in
- * choice whenever it keeps more lanes per CU resident.  Replaces the
+ * choice whenever it keeps more lanes per CU resident; for lanesPerPath 32,
+ * blockSize 352 keeps eleven paths per CU (the last wave half full) and 384
+ * reads the table from global memory where the table-in-LDS instance does not
+ * fit (N = 1024).  Replaces the
  * compile-time PerturbGridSize / PerturbBlockSize (...Optimized_GPU.cu:45-47). */
 int twisty_hip_set_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t blockSize,
       uint32_t blocksPerCU);

@@ -168,6 +168,31 @@ def test_large_n_1024(hip_device, oracle):
     check_against_oracle(case, oracle, gpu, ref, ref32, table, "C5 256 paths")
 
 
+def test_large_n_instances_agree(hip_device, oracle):
+    """N = 1024 on every G = 32 instance that fits (blocks 256, 320, 352 with
+    the table in LDS; 384 with the table in global memory): per-path weights,
+    per-joint curvatures and exit bins bit-identical, with chains."""
+    case = cases.C5
+    n, L = 2048, 4
+    out = {}
+    for block in (256, 320, 352, 384):
+        with cases.hip_context(case, oracle, schedule=(32, block, 0)) as ctx:
+            ctx.set_exit_histogram(64, 64)
+            r = ctx.run(n, 20, seed=case.seed, chain_length=L, per_path=True, per_joint=True)
+            r["hist"], r["bins"] = ctx.exit_histogram(n)
+            r["schedule"] = ctx.get_schedule()
+            out[block] = r
+        _report(f"C5 block={block}", **out[block]["schedule"])
+    a = out[256]
+    for block in (320, 352, 384):
+        b = out[block]
+        assert np.array_equal(a["log10w"].view(np.uint32), b["log10w"].view(np.uint32)), block
+        assert np.array_equal(a["curv"].view(np.uint32), b["curv"].view(np.uint32)), block
+        assert np.array_equal(a["bins"], b["bins"]), block
+        assert np.allclose(a["hist"], b["hist"], rtol=1e-6, atol=0), block
+        assert a["num_valid"] == b["num_valid"] and abs(a["log10_sum_w"] - b["log10_sum_w"]) <= 1e-6
+
+
 def test_path_offset(hip_device, oracle):
     case = cases.C2
     table, _ = cases.setup(case, oracle)

@@ -68,11 +68,13 @@ uint32_t hist_stride(const twisty_hip_ctx *ctx) { return ctx->whBins ? ((ctx->wh
 
 // Resolves a schedule for segment count N.  Zero fields are filled in: G from N
 // (about 16 segments per lane), then the largest compiled block size with the
-// table in LDS whose LDS fits one block (a lean instance instead when it keeps
-// more lanes resident per CU), then as many blocks per CU as the per-CU LDS
-// holds (at most 2048 lanes per CU).
+// table in LDS whose LDS fits one block (an instance that reads the table from
+// global memory instead when it keeps more lanes resident per CU), then as
+// many blocks per CU as the per-CU LDS holds (at most 2048 lanes per CU).  A
+// given block size takes the first of its instances whose LDS fits (the table
+// in LDS first).  mode receives the chosen instance's FwdMode.
 int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_t &block, uint32_t &bpc,
-      uint32_t &lds, std::string &why)
+      uint32_t &lds, std::string &why, int &mode)
 {
     const uint32_t ts = table_stride(ctx->tableSize ? ctx->tableSize : 4096);
     if (G == 0) {
@@ -88,33 +90,39 @@ int resolve_schedule(const twisty_hip_ctx *ctx, uint32_t N, uint32_t &G, uint32_
     }
     FwdInstance inst[32];
     const int ns = forward_instances(G, inst, 32);
-    auto ldsOf = [&](const FwdInstance &x) { return forward_lds_bytes(G, x.block, N, ts, hist_stride(ctx), x.lean); };
+    auto ldsOf = [&](const FwdInstance &x) { return forward_lds_bytes(G, x.block, N, ts, hist_stride(ctx), x.mode); };
     auto lanesPerCU = [&](const FwdInstance &x) {
         uint32_t b = ctx->ldsPerCU / ldsOf(x);
         if (b * x.block > 2048) b = 2048 / x.block;
         return (b < 1 ? 1u : b) * x.block;
     };
+    int chosen = -1;
     if (block == 0) {
-        int pick = -1;
-        for (int k = ns - 1; k >= 0 && pick < 0; --k)
-            if (!inst[k].lean && ldsOf(inst[k]) <= ctx->ldsPerBlock) pick = k;
+        for (int k = ns - 1; k >= 0 && chosen < 0; --k)
+            if (inst[k].mode == kFwdLds && ldsOf(inst[k]) <= ctx->ldsPerBlock) chosen = k;
         for (int k = 0; k < ns; ++k)
-            if (inst[k].lean && ldsOf(inst[k]) <= ctx->ldsPerBlock &&
-                  (pick < 0 || lanesPerCU(inst[k]) > lanesPerCU(inst[pick])))
-                pick = k;
-        if (pick < 0) {
+            if (inst[k].mode != kFwdLds && ldsOf(inst[k]) <= ctx->ldsPerBlock &&
+                  (chosen < 0 || lanesPerCU(inst[k]) > lanesPerCU(inst[chosen])))
+                chosen = k;
+        if (chosen < 0) {
             why = "no compiled block size fits the LDS of one block for N=" + std::to_string(N);
             return TWISTY_HIP_ERR_UNSUPPORTED;
         }
-        block = inst[pick].block;
+        block = inst[chosen].block;
+    } else {
+        int any = -1;
+        for (int k = 0; k < ns; ++k) {
+            if (inst[k].block != block) continue;
+            if (any < 0) any = k;
+            if (chosen < 0 && ldsOf(inst[k]) <= ctx->ldsPerBlock) chosen = k;
+        }
+        if (any < 0) {
+            why = "no kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
+            return TWISTY_HIP_ERR_UNSUPPORTED;
+        }
+        if (chosen < 0) chosen = any;  // reported below as an LDS overflow
     }
-    int chosen = -1;
-    for (int k = 0; k < ns; ++k)
-        if (inst[k].block == block) chosen = k;
-    if (chosen < 0) {
-        why = "no kernel instance for lanesPerPath=" + std::to_string(G) + " blockSize=" + std::to_string(block);
-        return TWISTY_HIP_ERR_UNSUPPORTED;
-    }
+    mode = inst[chosen].mode;
     lds = ldsOf(inst[chosen]);
     if (lds > ctx->ldsPerBlock) {
         why = "schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
@@ -181,8 +189,9 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     if (r.numPaths > (1ull << 40)) return fail(ctx, TWISTY_HIP_ERR_ARG, w + ": numPaths > 2^40");
     const uint32_t N = ctx->N;
     uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
+    int mode = kFwdLds;
     std::string why;
-    int rc = resolve_schedule(ctx, N, G, B, bpc, lds, why);
+    int rc = resolve_schedule(ctx, N, G, B, bpc, lds, why, mode);
     if (rc) return fail(ctx, rc, w + ": " + why);
     const uint32_t grid = ctx->numCUs * bpc;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -255,7 +264,7 @@ int enqueue_forward(twisty_hip_ctx *ctx, const FwdRequest &r, const char *who)
     ctx->lastWeightBins = ctx->whBins;
 
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipError_t e = launch_forward(p, G, grid, B, lds, ctx->stream);
+    hipError_t e = launch_forward(p, G, grid, B, mode, lds, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "forward kernel launch");
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     e = launch_finalize(partials, grid, absorbLog2PerPath, final, ctx->stream);
@@ -506,8 +515,8 @@ int twisty_hip_set_schedule(twisty_hip_ctx *ctx, uint32_t lanesPerPath, uint32_t
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (lanesPerPath && (!is_pow2(lanesPerPath) || lanesPerPath > 64))
         return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: lanesPerPath must be a power of two <= 64");
-    if (blockSize && (blockSize % 64 != 0 || blockSize > 1024))
-        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: blockSize must be a multiple of 64 <= 1024");
+    if (blockSize && (blockSize % 32 != 0 || blockSize > 1024))
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: blockSize must be a multiple of 32 <= 1024");
     if (blocksPerCU > 32) return fail(ctx, TWISTY_HIP_ERR_ARG, "set_schedule: blocksPerCU <= 32");
     ctx->schedG = lanesPerPath;
     ctx->schedBlock = blockSize;
@@ -521,8 +530,9 @@ int twisty_hip_get_schedule(twisty_hip_ctx *ctx, uint32_t *lanesPerPath, uint32_
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "get_schedule: set_curve first");
     uint32_t G = ctx->schedG, B = ctx->schedBlock, bpc = ctx->schedBpc, lds = 0;
+    int mode = kFwdLds;
     std::string why;
-    const int rc = resolve_schedule(ctx, ctx->N, G, B, bpc, lds, why);
+    const int rc = resolve_schedule(ctx, ctx->N, G, B, bpc, lds, why, mode);
     if (rc) return fail(ctx, rc, "get_schedule: " + why);
     if (lanesPerPath) *lanesPerPath = G;
     if (blockSize) *blockSize = B;

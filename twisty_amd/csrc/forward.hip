@@ -57,30 +57,33 @@ __device__ __forceinline__ uint32_t weight_bin(const FwdParams &p, uint32_t flag
     return bi < p.weightBins ? bi : p.weightBins - 1u;
 }
 
-// LEAN: the table is read from global memory (through the caches; it is only
-// touched by the weighting, once per path) and the draw records go through
-// ds_bpermute, so the LDS holds only path state and histogram bins and more
-// paths stay resident per CU.
-template<int G, int BLOCK, bool LEAN>
+// MODE (FwdMode): kFwdLds keeps the table and the draw records in LDS.  The
+// other two read the table from global memory (through the caches; it is only
+// touched by the weighting, once per path) so that more paths stay resident
+// per CU; kFwdLean also passes the draw records with ds_bpermute instead of an
+// LDS buffer, so the LDS holds only path state and histogram bins.
+template<int G, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) forward_kernel(const FwdParams p)
 {
+    constexpr bool TABLE_GLOBAL = MODE != kFwdLds;
+    constexpr bool LEAN = MODE == kFwdLean;
     constexpr uint32_t SLOT = 3u * BLOCK;  // floats per slot row
     extern __shared__ float smem[];
     const uint32_t N = p.N, S = p.S, K = p.K;
     // Rows 0..S-1 hold segments; row p.scratchRow (>= S and >= WMAX-1, so that
     // scr - w*SLOT never underflows) is the lane's zero row.
     const uint32_t tabOff = (p.scratchRow + 1) * SLOT;
-    const uint32_t histOff = tabOff + (LEAN ? 0u : p.tableStride);
+    const uint32_t histOff = tabOff + (TABLE_GLOBAL ? 0u : p.tableStride);
     const uint32_t drawOff = histOff + p.histStride;
     uint32_t *drawbuf = LEAN ? nullptr : reinterpret_cast<uint32_t *>(smem + drawOff);
     uint32_t *whist = reinterpret_cast<uint32_t *>(smem + histOff);
 
-    if constexpr (!LEAN)
+    if constexpr (!TABLE_GLOBAL)
         for (int i = threadIdx.x; i < p.tableSize; i += BLOCK) smem[tabOff + i] = p.table[i];
     if (p.weightHist)
         for (uint32_t i = threadIdx.x; i < p.weightBins; i += BLOCK) whist[i] = 0u;
     __syncthreads();
-    const TableView tv = make_table_view(LEAN ? p.table : smem + tabOff, p.tableSize, p.minK, p.maxK, p.step);
+    const TableView tv = make_table_view(TABLE_GLOBAL ? p.table : smem + tabOff, p.tableSize, p.minK, p.maxK, p.step);
 
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
@@ -228,26 +231,27 @@ __global__ void __launch_bounds__(256) finalize_kernel(const BlockPartial *parts
 }
 
 // ---------------------------------------------------------------------------
-// Instances: (lanes per path, block size, lean).  Sub-chain slot offsets are
-// immediates, so BLOCK is a template parameter; the set below covers the
-// shapes the schedule resolver chooses from.  A block size names one instance
-// per G (lean or not).  For N = 64 (G = 4) a 640-lane block with the table in
-// LDS measured slower than 512; the lean 768-lane block runs 12 waves per CU.
-// 320 lanes (G = 32, 64) fit five waves where 384 no longer fits: N = 1024
-// 486 -> 469 ms against 256 lanes; 352 lanes (G = 32: eleven paths, the last
-// wave half full) fit too, 433 -> 405 ms at N = 1024, K = 200.
+// Instances: (lanes per path, block size, FwdMode).  Sub-chain slot offsets
+// are immediates, so BLOCK is a template parameter; the set below covers the
+// shapes the schedule resolver chooses from.  A block size can have one
+// instance per mode; the resolver tries the table-in-LDS one first.  For
+// N = 64 (G = 4) a 640-lane block with the table in LDS measured slower than
+// 512; the lean 768-lane block runs 12 waves per CU.  320 lanes (G = 32, 64)
+// fit five waves where 384 no longer fits: N = 1024 486 -> 469 ms against 256
+// lanes; 352 lanes (G = 32: eleven paths, the last wave half full) fit too,
+// 433 -> 405 ms at N = 1024, K = 200.  With the table in global memory 384
+// lanes (twelve paths) fit at N = 1024.
 // ---------------------------------------------------------------------------
 #define TWISTY_FWD_INSTANCES(X)                                                                                   \
-    X(1, 128, false) X(1, 192, false) X(2, 192, false) X(2, 256, false) X(2, 320, false) X(4, 256, false)        \
-    X(4, 384, false) X(4, 512, false) X(4, 768, true) X(8, 512, false) X(8, 768, false) X(8, 1024, false)         \
-    X(16, 512, false) X(16, 1024, false) X(32, 256, false) X(32, 320, false) X(32, 352, false) X(32, 384, false)                   \
-    X(32, 512, false) X(32, 1024, false) X(64, 256, false) X(64, 320, false) X(64, 512, false)                   \
-    X(64, 640, false) X(64, 704, false) X(64, 1024, false)
+    X(1, 128, 0) X(1, 192, 0) X(2, 192, 0) X(2, 256, 0) X(2, 320, 0) X(4, 256, 0) X(4, 384, 0) X(4, 512, 0)       \
+    X(4, 768, 1) X(8, 512, 0) X(8, 768, 0) X(8, 1024, 0) X(16, 512, 0) X(16, 1024, 0) X(32, 256, 0)              \
+    X(32, 320, 0) X(32, 352, 0) X(32, 384, 0) X(32, 384, 2) X(32, 512, 0) X(32, 1024, 0) X(64, 256, 0)           \
+    X(64, 320, 0) X(64, 512, 0) X(64, 640, 0) X(64, 704, 0) X(64, 1024, 0)
 
-template<int G, int B, bool L>
+template<int G, int B, int M>
 static hipError_t launch_gb(const FwdParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    auto *fn = forward_kernel<G, B, L>;
+    auto *fn = forward_kernel<G, B, M>;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, s, p);
@@ -257,18 +261,18 @@ static hipError_t launch_gb(const FwdParams &p, uint32_t grid, uint32_t lds, hip
 int forward_instances(uint32_t G, FwdInstance *out, int cap)
 {
     int n = 0;
-#define X(g, b, l)                                                                                                \
-    if (G == g && n < cap) out[n++] = FwdInstance { b, l };
+#define X(g, b, m)                                                                                                \
+    if (G == g && n < cap) out[n++] = FwdInstance { b, m };
     TWISTY_FWD_INSTANCES(X)
 #undef X
     return n;
 }
 
-hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, int mode, uint32_t lds,
       hipStream_t s)
 {
-#define X(g, b, l)                                                                                                \
-    if (G == g && block == b) return launch_gb<g, b, l>(p, grid, lds, s);
+#define X(g, b, m)                                                                                                \
+    if (G == g && block == b && mode == m) return launch_gb<g, b, m>(p, grid, lds, s);
     TWISTY_FWD_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;

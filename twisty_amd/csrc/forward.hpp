@@ -55,12 +55,16 @@ struct FinalOut {
     double sumScaled;   // S = sum over valid paths of 2^(log2 w - M)
 };
 
-// A compiled forward kernel for some G: its block size and whether it is lean
-// (table read from global memory, draw records exchanged with ds_bpermute, so
-// neither takes LDS).
+// Where a forward instance keeps the weight table and the draw records:
+// both in LDS; lean (the table read from global memory through the caches,
+// the records exchanged with ds_bpermute, so neither takes LDS); or the table
+// in global memory and the records in LDS.
+enum FwdMode : int { kFwdLds = 0, kFwdLean = 1, kFwdTableGlobal = 2 };
+
+// A compiled forward kernel for some G: its block size and mode.
 struct FwdInstance {
     uint32_t block;
-    bool lean;
+    int mode;
 };
 // Instances compiled for G lanes per path (ascending block size); returns the
 // count.
@@ -81,17 +85,17 @@ inline uint32_t forward_scratch_row(uint32_t G, uint32_t N)
 // (not lean), weight-histogram bins, draw records (G > 1, not lean); at least
 // the block reduction's scratch.
 inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride,
-      uint32_t histStride = 0, bool lean = false)
+      uint32_t histStride = 0, int mode = kFwdLds)
 {
     const uint64_t rows = (uint64_t)forward_scratch_row(G, N) + 1;
-    const uint64_t floats = rows * 3ull * block + (lean ? 0ull : (uint64_t)tableStride) + histStride +
-          (G > 1 && !lean ? 4ull * block : 0ull);
+    const uint64_t floats = rows * 3ull * block + (mode == kFwdLds ? (uint64_t)tableStride : 0ull) + histStride +
+          (G > 1 && mode != kFwdLean ? 4ull * block : 0ull);
     const uint64_t reduceBytes = (uint64_t)((block + 63) / 64) * sizeof(LseAcc);
     const uint64_t bytes = floats * 4ull;
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);
 }
 
-hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_forward(const FwdParams &p, uint32_t G, uint32_t grid, uint32_t block, int mode, uint32_t lds,
       hipStream_t s);
 hipError_t launch_finalize(const BlockPartial *parts, uint32_t n, double absorbLog2PerPath, FinalOut *out,
       hipStream_t s);

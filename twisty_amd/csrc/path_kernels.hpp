@@ -164,11 +164,34 @@ __device__ __forceinline__ void perturb_multi(uint32_t laneB, uint32_t j, uint32
     }
 }
 
+// Bit of a chunk's first draw record set by mark_disjoint_chunks when the
+// chunk's M sub-chains are pairwise disjoint (diff <= 25 uses bits 0-4).
+constexpr uint32_t kChunkDisjoint = 0x8000u;
+
+// Once per batch, before perturb_batch_multi<..., FLAGGED = true>: lane j of a
+// group with j mod M == 0 checks the chunk of records j .. j + M - 1 (all
+// drawn: j + M <= kcount) and marks its own record when they are pairwise
+// disjoint, so the group reads one bit per chunk instead of each lane
+// repeating the M (M - 1) / 2 comparisons.  The records sit in the group's
+// slice of drawbuf; the caller syncs the wave before and after.
+template<int G, int M>
+__device__ __forceinline__ void mark_disjoint_chunks(uint32_t *drawbuf, uint32_t groupBase, uint32_t j,
+      uint32_t kcount)
+{
+    if ((j % (uint32_t)M) == 0 && j + (uint32_t)M <= kcount) {
+        DrawRec r[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) r[t].packed = drawbuf[(groupBase + j + (uint32_t)t) * 4];
+        if (disjoint_records<M>(r)) drawbuf[(groupBase + j) * 4] = r[0].packed | kChunkDisjoint;
+    }
+}
+
 // Perturbations kk0 .. kk0 + kcount - 1 of a batch for G >= 25, M at a time:
 // a chunk of M whose sub-chains are pairwise disjoint in every group of the
 // wave runs as one perturb_multi, any other chunk one perturbation at a time.
-// rec(t) returns the record of perturbation t of the batch (t < G).
-template<int G, int BLOCK, bool SPLIT, int M, class RecOf>
+// rec(t) returns the record of perturbation t of the batch (t < G).  FLAGGED:
+// the chunks were marked by mark_disjoint_chunks.
+template<int G, int BLOCK, bool SPLIT, int M, bool FLAGGED, class RecOf>
 __device__ __forceinline__ void perturb_batch_multi(uint32_t laneB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t kcount, RecOf rec)
 {
@@ -176,7 +199,13 @@ __device__ __forceinline__ void perturb_batch_multi(uint32_t laneB, uint32_t j,
         DrawRec r[M];
 #pragma unroll
         for (int t = 0; t < M; ++t) r[t] = rec(min(kk + (uint32_t)t, (uint32_t)G - 1u));
-        const bool ok = kcount - kk >= (uint32_t)M && disjoint_records<M>(r);
+        bool ok;
+        if constexpr (FLAGGED) {
+            ok = (r[0].packed & kChunkDisjoint) != 0u;
+            r[0].packed &= ~kChunkDisjoint;
+        } else {
+            ok = kcount - kk >= (uint32_t)M && disjoint_records<M>(r);
+        }
         if (__builtin_amdgcn_ballot_w64(!ok) == 0) {
             perturb_multi<G, BLOCK, M>(laneB, j, scrB[0], r);
         } else {
@@ -229,7 +258,7 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
                     perturb_one<G, BLOCK, true>(laneB, j, scrB, recs[kk]);
                 }
             } else if constexpr (G >= 25) {
-                perturb_batch_multi<G, BLOCK, true, kMultiPerturb>(laneB, j, scrB, kcount,
+                perturb_batch_multi<G, BLOCK, true, kMultiPerturb, false>(laneB, j, scrB, kcount,
                       [&](uint32_t t) { return bperm_rec(firstB + 4 * (int)t, mine); });
             } else {
                 for (uint32_t kk = 0; kk < kcount; ++kk)
@@ -253,7 +282,9 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
                     perturb_one<G, BLOCK>(laneB, j, scrB, recs[kk]);
                 }
             } else if constexpr (G >= 25) {
-                perturb_batch_multi<G, BLOCK, false, kMultiPerturb>(laneB, j, scrB, kcount,
+                mark_disjoint_chunks<G, kMultiPerturb>(drawbuf, groupBase, j, kcount);
+                wave_sync_lds();
+                perturb_batch_multi<G, BLOCK, false, kMultiPerturb, true>(laneB, j, scrB, kcount,
                       [&](uint32_t t) { return load_rec(drawbuf + (groupBase + t) * 4); });
                 wave_sync_lds();
             } else {

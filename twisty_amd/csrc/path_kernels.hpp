@@ -164,50 +164,97 @@ __device__ __forceinline__ void perturb_multi(uint32_t laneB, uint32_t j, uint32
     }
 }
 
-// Bit of a chunk's first draw record set by mark_disjoint_chunks when the
-// chunk's M sub-chains are pairwise disjoint (diff <= 25 uses bits 0-4).
+// Bits of the draw records set by mark_disjoint_chunks (diff <= 25 uses bits
+// 0-4): on a chunk's first record when the chunk's M sub-chains are pairwise
+// disjoint, and on the first record of each half chunk when that half's are.
 constexpr uint32_t kChunkDisjoint = 0x8000u;
+constexpr uint32_t kHalfDisjoint = 0x4000u;
+
+// Whether the sub-chains of the packed records pk[lo..hi) are pairwise
+// disjoint (bitwise, no branches).
+template<int LO, int HI, int N>
+__device__ __forceinline__ bool packed_disjoint(const uint32_t (&pk)[N])
+{
+    bool ok = true;
+#pragma unroll
+    for (int a = LO; a < HI; ++a) {
+#pragma unroll
+        for (int b = a + 1; b < HI; ++b) {
+            const uint32_t la = pk[a] >> 16, lb = pk[b] >> 16;
+            const uint32_t ra = la + (pk[a] & 0xFFFFu), rb = lb + (pk[b] & 0xFFFFu);
+            ok = ok & ((ra <= lb) | (rb <= la));
+        }
+    }
+    return ok;
+}
 
 // Once per batch, before perturb_batch_multi<..., FLAGGED = true>: lane j of a
-// group with j mod M == 0 checks the chunk of records j .. j + M - 1 (all
-// drawn: j + M <= kcount) and marks its own record when they are pairwise
-// disjoint, so the group reads one bit per chunk instead of each lane
-// repeating the M (M - 1) / 2 comparisons.  The records sit in the group's
-// slice of drawbuf; the caller syncs the wave before and after.
+// group with j mod M == 0 checks the chunk of records j .. j + M - 1 and marks
+// the chunk (all M drawn: j + M <= kcount) and each half of it (its records
+// drawn) that is pairwise disjoint, so the group reads bits instead of each
+// lane repeating the comparisons.  The records sit in the group's slice of
+// drawbuf; the caller syncs the wave before and after.
 template<int G, int M>
 __device__ __forceinline__ void mark_disjoint_chunks(uint32_t *drawbuf, uint32_t groupBase, uint32_t j,
       uint32_t kcount)
 {
-    if ((j % (uint32_t)M) == 0 && j + (uint32_t)M <= kcount) {
-        DrawRec r[M];
+    constexpr int H = M / 2;
+    if ((j % (uint32_t)M) == 0 && j + (uint32_t)H <= kcount) {
+        uint32_t pk[M];
 #pragma unroll
-        for (int t = 0; t < M; ++t) r[t].packed = drawbuf[(groupBase + j + (uint32_t)t) * 4];
-        if (disjoint_records<M>(r)) drawbuf[(groupBase + j) * 4] = r[0].packed | kChunkDisjoint;
+        for (int t = 0; t < M; ++t) pk[t] = drawbuf[(groupBase + j + (uint32_t)t) * 4];  // past kcount: unused
+        const bool full = j + (uint32_t)M <= kcount;
+        const bool half0 = packed_disjoint<0, H>(pk);
+        const bool half1 = full && packed_disjoint<H, M>(pk);
+        const bool all = full && packed_disjoint<0, M>(pk);
+        drawbuf[(groupBase + j) * 4] = pk[0] | (all ? kChunkDisjoint : 0u) | (half0 ? kHalfDisjoint : 0u);
+        if (half1) drawbuf[(groupBase + j + (uint32_t)H) * 4] = pk[H] | kHalfDisjoint;
     }
 }
 
 // Perturbations kk0 .. kk0 + kcount - 1 of a batch for G >= 25, M at a time:
 // a chunk of M whose sub-chains are pairwise disjoint in every group of the
-// wave runs as one perturb_multi, any other chunk one perturbation at a time.
-// rec(t) returns the record of perturbation t of the batch (t < G).  FLAGGED:
-// the chunks were marked by mark_disjoint_chunks.
+// wave runs as one perturb_multi, any other chunk one perturbation at a time
+// or, FLAGGED (the chunks marked by mark_disjoint_chunks), one half chunk at a
+// time where a half is disjoint in every group of the wave.  rec(t) returns
+// the record of perturbation t of the batch (t < G).
 template<int G, int BLOCK, bool SPLIT, int M, bool FLAGGED, class RecOf>
 __device__ __forceinline__ void perturb_batch_multi(uint32_t laneB, uint32_t j,
       const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t kcount, RecOf rec)
 {
+    constexpr int H = M / 2;
     for (uint32_t kk = 0; kk < kcount; kk += M) {
         DrawRec r[M];
 #pragma unroll
         for (int t = 0; t < M; ++t) r[t] = rec(min(kk + (uint32_t)t, (uint32_t)G - 1u));
         bool ok;
+        uint32_t halves = 0;  // FLAGGED: bit h = half h marked disjoint
         if constexpr (FLAGGED) {
             ok = (r[0].packed & kChunkDisjoint) != 0u;
-            r[0].packed &= ~kChunkDisjoint;
+            halves = ((r[0].packed & kHalfDisjoint) ? 1u : 0u) | ((r[H].packed & kHalfDisjoint) ? 2u : 0u);
+            r[0].packed &= ~(kChunkDisjoint | kHalfDisjoint);
+            r[H].packed &= ~kHalfDisjoint;
         } else {
             ok = kcount - kk >= (uint32_t)M && disjoint_records<M>(r);
         }
         if (__builtin_amdgcn_ballot_w64(!ok) == 0) {
             perturb_multi<G, BLOCK, M>(laneB, j, scrB[0], r);
+        } else if constexpr (FLAGGED && H >= 2) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (kk + (uint32_t)(h * H) >= kcount) break;
+                if (__builtin_amdgcn_ballot_w64(((halves >> h) & 1u) == 0u) == 0) {
+                    DrawRec rh[H];
+#pragma unroll
+                    for (int t = 0; t < H; ++t) rh[t] = r[h * H + t];
+                    perturb_multi<G, BLOCK, H>(laneB, j, scrB[0], rh);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < H; ++t)
+                        if (kk + (uint32_t)(h * H + t) < kcount)
+                            perturb_one<G, BLOCK, SPLIT>(laneB, j, scrB, r[h * H + t]);
+                }
+            }
         } else {
 #pragma unroll
             for (int t = 0; t < M; ++t)

@@ -207,15 +207,20 @@ __device__ __forceinline__ TableView make_table_view(const float *w, int32_t siz
 // (below it both are far under 1/2 and floor to 0).  Unless q is that close to
 // an integer (or d is NaN) no integer lies between the two, and floor(q) is
 // floor(RN(d/step)); d = 0 gives 0 either way.  The rare remaining lanes (about
-// 2e-5 of the curvatures) take the IEEE division.  tests/test_bin_floor.py
-// checks the bound and the floors for every float d of the workloads' table
-// ranges.
+// 2e-5 of the curvatures) take the IEEE division, behind a wave-uniform branch
+// so that the wave skips it unless one of its lanes needs it (a per-lane
+// condition is if-converted: the division's dozen instructions would run for
+// every joint).  tests/test_bin_floor.py checks the bound and the floors for
+// every float d of the workloads' table ranges.
 __device__ __forceinline__ float bin_floor(const TableView &t, float d)
 {
     float q = __fmul_rn(d, t.invStep);
     const float k = rintf(q);
-    const bool fast = (d == 0.0f) | (fabsf(q - k) > q * 0x1p-22f);
-    if (!fast) q = __fdiv_rn(d, t.step);
+    const bool slow = !((d == 0.0f) | (fabsf(q - k) > q * 0x1p-22f));
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(slow) != 0, 0)) {
+        const float r = __fdiv_rn(d, t.step);
+        q = slow ? r : q;
+    }
     return floorf(q);
 }
 

@@ -407,17 +407,21 @@ struct NoJointOutput {
 };
 
 // Adds a joint's log2 weight l to the path's sum, or records it in the flags
-// when it is not finite (PathWeighters.cpp:88-96), without branches; a joint
-// with use = false changes nothing.
+// when it is not finite (PathWeighters.cpp:88-96); a joint with use = false
+// changes nothing.  The kind of a non-finite log (rare: a zero or non-finite
+// weight) is worked out behind a wave-uniform branch.
 __device__ __forceinline__ void add_joint_log(FF &lsum, uint32_t &flags, float l, bool use)
 {
-    const bool nan = !(l == l), ninf = l == -INFINITY, pinf = l == INFINITY;
-    const uint32_t f = nan ? 1u : (ninf ? 2u : (pinf ? 4u : 0u));
-    flags |= use ? f : 0u;
+    const bool finite = __builtin_isfinite(l);
     const FF t = ff_add_f(lsum, l);
-    const bool take = use & (f == 0u);
+    const bool take = use & finite;
     lsum.hi = take ? t.hi : lsum.hi;
     lsum.lo = take ? t.lo : lsum.lo;
+    const bool odd = use & !finite;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(odd) != 0, 0)) {
+        const uint32_t f = !(l == l) ? 1u : (l < 0.0f ? 2u : 4u);
+        flags |= odd ? f : 0u;
+    }
 }
 
 // Joints a lane weighs per pass of weigh_state.  One is fastest as measured

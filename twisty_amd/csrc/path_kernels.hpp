@@ -32,12 +32,15 @@ __device__ __forceinline__ void perturb_one(uint32_t laneB, uint32_t j,
     constexpr uint32_t COMPB = 4u * BLOCK;       // bytes between components
     const uint32_t diff = rec.packed & 0xFFFFu;
     const uint32_t left = rec.packed >> 16;
-    const uint32_t right = left + diff;
-    // First segment index >= left owned by this lane, and how many of this
-    // lane's segments lie in [left, right).
-    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
-    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
-    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
+    // This lane owns segments s G + j.  Its first slot at or past left is
+    // floor((left - 1 - j) / G) + 1 and the number of its slots in
+    // [left, left + diff) is floor((left + diff - 1 - j) / G) - floor((left -
+    // 1 - j) / G) (arithmetic shifts: left - 1 - j >= -G).
+    constexpr int LOG2G = __builtin_ctz((unsigned)G);
+    const int32_t a = (int32_t)left - (int32_t)(j + 1u);
+    const int32_t sa = a >> LOG2G;
+    const uint32_t cnt = (uint32_t)(((a + (int32_t)diff) >> LOG2G) - sa);
+    const uint32_t baseB = (uint32_t)(sa + 1) * SLOTB + laneB;
     uint32_t addr[WMAIN];
     float dx[WMAIN], dy[WMAIN], dz[WMAIN];
 #pragma unroll

@@ -21,6 +21,13 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ k in one v_bitop3_b32 (truth table 0x96, the three-input XOR of
+// gfx950); written as two XORs the round takes two instructions per word.
+__device__ __forceinline__ uint32_t xor3_key(uint32_t a, uint32_t b, uint32_t k)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, k, 0x96);
+}
+
 __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
       uint32_t k1)
 {
@@ -28,8 +35,8 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3_key((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3_key((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;

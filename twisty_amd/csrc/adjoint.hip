@@ -57,10 +57,13 @@ __device__ __forceinline__ void unperturb_one(uint32_t laneB, uint32_t lamB, uin
     constexpr uint32_t COMPB = 4u * BLOCK;
     const uint32_t diff = rec.packed & 0xFFFFu;
     const uint32_t left = rec.packed >> 16;
-    const uint32_t right = left + diff;
-    const uint32_t i0 = left + ((j - left) & (uint32_t)(G - 1));
-    const uint32_t cnt = (i0 < right) ? ((right - 1u - i0) / G + 1u) : 0u;
-    const uint32_t baseB = (i0 / G) * SLOTB + laneB;
+    // first slot and slot count of this lane in [left, left + diff), as in
+    // perturb_one
+    constexpr int LOG2G = __builtin_ctz((unsigned)G);
+    const int32_t a = (int32_t)left - (int32_t)(j + 1u);
+    const int32_t sa = a >> LOG2G;
+    const uint32_t cnt = (uint32_t)(((a + (int32_t)diff) >> LOG2G) - sa);
+    const uint32_t baseB = (uint32_t)(sa + 1) * SLOTB + laneB;
     uint32_t addr[WMAX];
     float dx[WMAX], dy[WMAX], dz[WMAX], lx[WMAX], ly[WMAX], lz[WMAX];
 #pragma unroll

@@ -82,8 +82,8 @@ inline uint32_t forward_scratch_row(uint32_t G, uint32_t N)
 }
 
 // Dynamic LDS bytes of the forward kernel: segment rows + zero row, table
-// (not lean), weight-histogram bins, draw records (G > 1, not lean); at least
-// the block reduction's scratch.
+// (kFwdLds only), weight-histogram bins, draw records (G > 1, not kFwdLean);
+// at least the block reduction's scratch.
 inline uint32_t forward_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint32_t tableStride,
       uint32_t histStride = 0, int mode = kFwdLds)
 {

@@ -160,12 +160,12 @@ def test_other_shapes(hip_device, oracle, case):
 def test_large_n_1024(hip_device, oracle):
     case = cases.C5
     table, _ = cases.setup(case, oracle)
-    n = 256
+    n = 4096
     with cases.hip_context(case, oracle) as ctx:
         gpu = ctx.run(n, case.K, seed=case.seed, per_path=True, per_joint=True)
         _report("C5 schedule", **ctx.get_schedule())
     ref, ref32 = both_refs(case, oracle, n)
-    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C5 256 paths")
+    check_against_oracle(case, oracle, gpu, ref, ref32, table, "C5 4096 paths")
 
 
 def test_large_n_instances_agree(hip_device, oracle):

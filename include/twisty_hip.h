@@ -311,12 +311,14 @@ typedef struct twisty_hip_experiment {
     uint32_t burn_in;                 /* unweighted paths opening every chain (<= 2^24) */
 } twisty_hip_experiment;
 
-/* Launch schedule of a batch: blockSize 256 or 512; blocksPerCU 0 = as many as
- * LDS allows; segmentsPerLane picks each experiment's lanes per path (the
- * smallest power of two >= 2 with ceil(N / lanes) <= segmentsPerLane);
- * tilePaths = paths per work item; lean must be 0 (tables and draw records in
- * LDS): lean 1 (tables read from global memory, draw records exchanged without
- * LDS) is reserved and currently returns TWISTY_HIP_ERR_UNSUPPORTED. */
+/* Launch schedule of a batch: blocksPerCU 0 = as many as LDS and registers
+ * allow; segmentsPerLane picks each experiment's lanes per path (the smallest
+ * power of two >= 2 with ceil(N / lanes) <= segmentsPerLane); tilePaths = paths
+ * per work item; lean places the tables and the draw records: 0 both in LDS
+ * (blockSize 256 or 512), 1 tables read from global memory and draw records
+ * exchanged without LDS (blockSize 256 or 512), 2 tables read from global
+ * memory and draw records in LDS (blockSize 256, 320 or 384).  Every placement
+ * gives the same bits per path. */
 typedef struct twisty_hip_batch_schedule {
     uint32_t block_size;
     uint32_t blocks_per_cu;

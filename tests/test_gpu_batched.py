@@ -161,14 +161,10 @@ def test_batched_replays_are_deterministic(hip_device):
     ctx.close()
 
 
-LEAN_OFF = pytest.mark.skip(reason="lean batched schedules are switched off: a lean batch run did not complete "
-                                   "and its cause has not been shown")
-
-
 @pytest.mark.parametrize("sched", [(256, 0, 8, 1024), (512, 1, 16, 333), (256, 2, 16, 4096), (256, 0, 32, 100000),
-                                   pytest.param((256, 0, 16, 4096, 1), marks=LEAN_OFF),
-                                   pytest.param((256, 3, 16, 333, 1), marks=LEAN_OFF),
-                                   pytest.param((512, 0, 8, 1024, 1), marks=LEAN_OFF)])
+                                   (256, 0, 16, 4096, 1), (256, 3, 16, 333, 1), (512, 0, 8, 1024, 1),
+                                   (256, 0, 16, 4096, 2), (320, 0, 16, 1024, 2), (384, 2, 8, 333, 2),
+                                   (320, 3, 32, 4096, 2)])
 def test_batched_schedules_agree(hip_device, sched):
     w = hetero_batch()
     ctx = capi.Context(0)
@@ -187,17 +183,20 @@ def test_batched_schedules_agree(hip_device, sched):
     ctx.close()
 
 
-def test_batched_lean_schedule_rejected(hip_device):
-    """lean = 1 is switched off: set_schedule fails before any launch and the
-    batch keeps its previous schedule."""
+def test_batched_unsupported_schedule_rejected(hip_device):
+    """A block size the placement is not compiled for, or an unknown placement,
+    fails before any launch and the batch keeps its previous schedule."""
     w = hetero_batch()
     ctx = capi.Context(0)
     b = capi.Batch(ctx, w["experiments"], w["tables"], w["meta"])
     before = b.get_schedule()
     with pytest.raises(capi.TwistyHipError) as e:
-        b.set_schedule(256, 0, 16, 4096, 1)
+        b.set_schedule(512, 0, 16, 4096, 2)
     assert e.value.code == capi.ERR_UNSUPPORTED
-    assert b.get_schedule() == before and before["lean"] == 0
+    with pytest.raises(capi.TwistyHipError) as e:
+        b.set_schedule(256, 0, 16, 4096, 3)
+    assert e.value.code == capi.ERR_ARG
+    assert b.get_schedule() == before
     b.close()
     ctx.close()
 
@@ -240,6 +239,7 @@ def test_batched_autotune(hip_device):
     print(f"[batched autotune] best {best}; {len(tried)} candidates, "
           f"{min(t for _, t in tried):.3f}..{max(t for _, t in tried):.3f} ms", flush=True)
     assert len(tried) >= 8 and b.get_schedule()["block_size"] == best["block_size"]
+    assert {t["lean"] for t, _ in tried} == {0, 1, 2}  # every placement was timed
     after, _, ms = b.run()
     assert abs(ms - min(t for _, t in tried)) < 0.5 * max(ms, 1e-3) + 1.0
     for r0, r1 in zip(base, after):

@@ -48,6 +48,10 @@ struct twisty_hip_batch {
 namespace {
 
 const twisty_hip_batch_schedule kDefaultSchedule { 256, 0, 16, 4096, 0 };
+// Lanes of the batched kernel resident per CU by registers: three waves of 64
+// per SIMD at 168 VGPRs (lean 0 and 2), two for the lean form (about 190).
+constexpr uint32_t kBatchedLanesPerCU = 768;
+constexpr uint32_t kBatchedLeanLanesPerCU = 512;
 
 uint32_t lanes_for(uint32_t N, uint32_t spl)
 {
@@ -70,15 +74,11 @@ int plan(twisty_hip_batch *b)
 {
     twisty_hip_ctx *ctx = b->ctx;
     const twisty_hip_batch_schedule &sc = b->sched;
-    if (!batched_block_supported(sc.block_size))
-        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "batch: blockSize must be 256 or 512");
-    if (sc.segments_per_lane < 1 || sc.segments_per_lane > 64 || sc.tile_paths < 1 || sc.lean > 1)
-        return fail(ctx, TWISTY_HIP_ERR_ARG, "batch: segmentsPerLane in [1, 64], tilePaths >= 1, lean 0 or 1");
-    // The lean batched kernel is switched off: a lean batch run (block 256,
-    // three blocks per CU by LDS, two by registers, G = 2..32) did not
-    // complete, and no cause has been shown yet.
-    if (sc.lean)
-        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED, "batch: lean schedules are not enabled");
+    if (sc.segments_per_lane < 1 || sc.segments_per_lane > 64 || sc.tile_paths < 1 || sc.lean > 2)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "batch: segmentsPerLane in [1, 64], tilePaths >= 1, lean 0, 1 or 2");
+    if (!batched_block_supported(sc.block_size, (int)sc.lean))
+        return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
+              "batch: blockSize must be 256 or 512 (lean 0 or 1), 256, 320 or 384 (lean 2)");
     const uint32_t nE = (uint32_t)b->exps.size();
     std::vector<BatchExp> de(nE);
     uint32_t rows = 0;
@@ -111,7 +111,7 @@ int plan(twisty_hip_batch *b)
         pathOut += x.num_paths;
     }
     const uint32_t ts = table_stride(b->numBins);
-    const uint32_t lds = batched_lds_bytes(sc.block_size, rows, ts, sc.lean != 0);
+    const uint32_t lds = batched_lds_bytes(sc.block_size, rows, ts, (int)sc.lean);
     if (lds > ctx->ldsPerBlock)
         return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
               "batch: schedule needs " + std::to_string(lds) + " LDS bytes per block, limit " +
@@ -119,7 +119,10 @@ int plan(twisty_hip_batch *b)
     uint32_t bpc = sc.blocks_per_cu;
     if (bpc == 0) {
         bpc = ctx->ldsPerCU / lds;
-        if (bpc * sc.block_size > 2048) bpc = 2048 / sc.block_size;
+        // registers: three waves per SIMD (168 VGPRs), two for lean (it is
+        // not capped: it would spill)
+        const uint32_t lanes = sc.lean == 1 ? kBatchedLeanLanesPerCU : kBatchedLanesPerCU;
+        if (bpc * sc.block_size > lanes) bpc = lanes / sc.block_size;
         if (bpc < 1) bpc = 1;
     }
     // work items of whole chains (about tilePaths paths each), longest first
@@ -164,7 +167,7 @@ int plan(twisty_hip_batch *b)
     HIP_TRY(ctx, hipMemcpy(b->dExpItems, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(b->dExpItemStart, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(b->dExps, de.data(), de.size() * sizeof(BatchExp), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, batched_prepare(sc.block_size, sc.lean != 0, lds));
+    HIP_TRY(ctx, batched_prepare(sc.block_size, (int)sc.lean, lds));
     b->grid = ctx->numCUs * bpc;
     b->lds = lds;
     b->rows = rows;
@@ -194,7 +197,7 @@ int capture(twisty_hip_batch *b, bool perPath)
     p.rows = b->rows;
     p.tableStride = table_stride(b->numBins);
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t le = launch_batched(p, b->grid, b->sched.block_size, b->sched.lean != 0, b->lds,
+    const hipError_t le = launch_batched(p, b->grid, b->sched.block_size, (int)b->sched.lean, b->lds,
           b->dExpItemStart, b->dExpItems, b->dAbsorb, (uint32_t)b->exps.size(), b->dOuts, ctx->stream);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
@@ -432,14 +435,18 @@ int twisty_hip_batch_autotune(twisty_hip_batch *b, uint32_t reps, twisty_hip_bat
     if (!b->ctx) return TWISTY_HIP_ERR_STATE;  // its context was destroyed
     twisty_hip_ctx *ctx = b->ctx;
     if (reps == 0) reps = 1;
-    // Candidates: block size x lanes policy x blocks per CU x tile size.  A
-    // candidate whose LDS does not fit is skipped.
+    // Candidates: block size x lanes policy x blocks per CU x tile size with
+    // the tables in LDS, and the two other placements at block 256 with as
+    // many blocks per CU as fit.  A candidate whose LDS does not fit is skipped.
     std::vector<twisty_hip_batch_schedule> cand;
     for (uint32_t block : { 256u, 512u })
         for (uint32_t spl : { 8u, 16u })
             for (uint32_t bpc : { 1u, 2u })
                 for (uint32_t tile : { 1024u, 4096u, 16384u })
                     cand.push_back({ block, bpc, spl, tile, 0u });
+    for (uint32_t lean : { 1u, 2u })
+        for (uint32_t spl : { 8u, 16u })
+            cand.push_back({ 256u, 0u, spl, 4096u, lean });
     twisty_hip_batch_schedule bestS = b->sched;
     float bestMs = INFINITY;
     uint32_t n = 0;

@@ -9,17 +9,19 @@
 // share the machine without a tail of idle CUs.  Each item runs the same
 // per-path code as the plain kernel (perturb_path / weigh_state) with the
 // lanes-per-path G of its experiment, chosen from its segment count; the
-// block keeps the experiment's table in LDS and reloads it only when the next
-// item belongs to an experiment with another table (lean: tables are read from
-// global memory and draw records go through ds_bpermute, so the LDS holds path
-// state only).  Each item leaves one LSE partial; a second kernel merges an
-// experiment's partials in FP64.
+// MODE (FwdMode) places the tables and the draw records as in forward_kernel:
+// kFwdLds keeps the experiment's table in LDS and reloads it only when the next
+// item belongs to an experiment with another table; kFwdTableGlobal reads the
+// tables from global memory (no reload, no barrier per switch) and keeps the
+// draw records in LDS, so more paths stay resident per CU; kFwdLean also
+// passes the draw records with ds_bpermute.  Each item leaves one LSE
+// partial; a second kernel merges an experiment's partials in FP64.
 #include "batched.hpp"
 #include "path_kernels.hpp"
 
 namespace twisty_hip {
 
-template<int G, int BLOCK, bool LEAN>
+template<int G, int BLOCK, int MODE>
 __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e, const BatchItem &item,
       float *smem, const TableView &tv, LseAcc &acc, uint32_t &clamped)
 {
@@ -29,7 +31,9 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     const uint32_t lane = threadIdx.x;
     const uint32_t j = lane & (G - 1);
     const uint32_t groupBase = lane - j;
-    uint32_t *drawbuf = LEAN ? nullptr : reinterpret_cast<uint32_t *>(smem + p.rows * SLOT + p.tableStride);
+    constexpr bool LEAN = MODE == kFwdLean;
+    uint32_t *drawbuf = LEAN ? nullptr
+                             : reinterpret_cast<uint32_t *>(smem + p.rows * SLOT + (MODE == kFwdLds ? p.tableStride : 0u));
     const uint32_t scr = e.scratchRow * SLOT + lane;
     smem[scr] = 0.0f;
     smem[scr + BLOCK] = 0.0f;
@@ -83,46 +87,57 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     }
 }
 
-template<int BLOCK, bool LEAN>
-__global__ void __launch_bounds__(BLOCK) batched_kernel(const BatchParams p)
+// Three waves per SIMD (168 VGPRs, no spills) for the LDS-recorded modes: at
+// eight segments per lane three 256-lane blocks fit a CU's LDS.
+template<int BLOCK, int MODE>
+__global__ void __launch_bounds__(BLOCK, MODE == kFwdLean ? 1 : 3) batched_kernel(const BatchParams p)
 {
     // No static LDS: the byte addressing of perturb_one takes the dynamic
     // block to start at groupstaticsize(), so the item slot lives at its end.
     extern __shared__ float smem[];
     const uint32_t tabOff = p.rows * 3u * BLOCK;
-    uint32_t &sItem = *reinterpret_cast<uint32_t *>(smem + tabOff + (LEAN ? 0u : p.tableStride + 4u * BLOCK));
+    constexpr bool TABLE_LDS = MODE == kFwdLds;
+    const uint32_t itemOff = (TABLE_LDS ? p.tableStride : 0u) + (MODE == kFwdLean ? 0u : 4u * BLOCK);
+    uint32_t &sItem = *reinterpret_cast<uint32_t *>(smem + tabOff + itemOff);
     uint32_t curTable = 0xFFFFFFFFu;
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(p.queue, 1u);
         __syncthreads();
         const uint32_t it = sItem;
-        __syncthreads();  // every thread has read sItem before it is rewritten
         if (it >= p.numItems) break;
         const BatchItem item = p.items[it];
         const BatchExp e = p.exps[item.exp];
         const BatchTable bt = p.tables[e.table];
-        if (!LEAN && e.table != curTable) {
+        if (TABLE_LDS && e.table != curTable) {
             // The previous item ended with a block barrier, so no lane still
             // reads the old table.
             for (int i = threadIdx.x; i < bt.size; i += BLOCK) smem[tabOff + i] = p.tableData[bt.offset + i];
             __syncthreads();
             curTable = e.table;
         }
-        const TableView tv = make_table_view(LEAN ? p.tableData + bt.offset : smem + tabOff, bt.size, bt.minK, bt.maxK,
+        const TableView tv = make_table_view(TABLE_LDS ? smem + tabOff : p.tableData + bt.offset, bt.size, bt.minK, bt.maxK,
               bt.step);
         LseAcc acc;
         lse_init(acc);
         uint32_t clamped = 0;
         switch (e.G) {
-        case 2: run_item<2, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
-        case 4: run_item<4, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
-        case 8: run_item<8, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
-        case 16: run_item<16, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
-        case 32: run_item<32, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
-        default: run_item<64, BLOCK, LEAN>(p, e, item, smem, tv, acc, clamped); break;
+        case 2: run_item<2, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
+        case 4: run_item<4, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
+        case 8: run_item<8, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
+        case 16: run_item<16, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
+        case 32: run_item<32, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
+        default: run_item<64, BLOCK, MODE>(p, e, item, smem, tv, acc, clamped); break;
         }
         acc.nClamped = clamped;
         block_reduce_lse(acc, smem, p.itemPartials + it);
+        // Every thread has read sItem before thread 0 rewrites it, and thread
+        // 0's end of the reduction and its fetch of the next item are kept
+        // apart by a barrier: with nothing between them the compiler threads
+        // thread 0 from one to the other, and the structured control flow it
+        // builds from that runs the loop's barriers for the other lanes of
+        // wave 0 without thread 0, so they re-read the same item without end
+        // (seen as a stall of the table-in-global-memory forms).
+        __syncthreads();
     }
 }
 
@@ -136,34 +151,42 @@ __global__ void __launch_bounds__(256) batch_finalize_kernel(const BlockPartial 
     finalize_partials(parts, expItems + b, expItemStart[e + 1] - b, absorbLog2[e], outs + e);
 }
 
-bool batched_block_supported(uint32_t block) { return block == 256 || block == 512; }
-
-static const void *batched_fn(uint32_t block, bool lean)
+bool batched_block_supported(uint32_t block, int mode)
 {
-    if (block == 256) return lean ? (const void *)batched_kernel<256, true> : (const void *)batched_kernel<256, false>;
-    if (block == 512) return lean ? (const void *)batched_kernel<512, true> : (const void *)batched_kernel<512, false>;
+    if (mode == kFwdTableGlobal) return block == 256 || block == 320 || block == 384;
+    return (mode == kFwdLds || mode == kFwdLean) && (block == 256 || block == 512);
+}
+
+#define TWISTY_BATCHED_INSTANCES(X) X(256, 0) X(512, 0) X(256, 1) X(512, 1) X(256, 2) X(320, 2) X(384, 2)
+
+static const void *batched_fn(uint32_t block, int mode)
+{
+#define X(B, M) \
+    if (block == B && mode == M) return (const void *)batched_kernel<B, M>;
+    TWISTY_BATCHED_INSTANCES(X)
+#undef X
     return nullptr;
 }
 
-hipError_t batched_prepare(uint32_t block, bool lean, uint32_t lds)
+hipError_t batched_prepare(uint32_t block, int mode, uint32_t lds)
 {
-    const void *fn = batched_fn(block, lean);
+    const void *fn = batched_fn(block, mode);
     if (!fn) return hipErrorInvalidValue;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, bool lean, uint32_t lds,
+hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, int mode, uint32_t lds,
       const uint32_t *expItemStart, const uint32_t *expItems, const double *absorbLog2, uint32_t numExps,
       FinalOut *outs, hipStream_t s)
 {
+    if (!batched_fn(block, mode)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, p.queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (block == 256 && lean) hipLaunchKernelGGL((batched_kernel<256, true>), dim3(grid), dim3(256), lds, s, p);
-    else if (block == 256) hipLaunchKernelGGL((batched_kernel<256, false>), dim3(grid), dim3(256), lds, s, p);
-    else if (block == 512 && lean) hipLaunchKernelGGL((batched_kernel<512, true>), dim3(grid), dim3(512), lds, s, p);
-    else if (block == 512) hipLaunchKernelGGL((batched_kernel<512, false>), dim3(grid), dim3(512), lds, s, p);
-    else return hipErrorInvalidValue;
+#define X(B, M) \
+    if (block == B && mode == M) hipLaunchKernelGGL((batched_kernel<B, M>), dim3(grid), dim3(B), lds, s, p);
+    TWISTY_BATCHED_INSTANCES(X)
+#undef X
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(numExps), dim3(256), 0, s, p.itemPartials, expItemStart, expItems,
           absorbLog2, outs);

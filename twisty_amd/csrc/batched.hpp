@@ -54,20 +54,22 @@ struct BatchParams {
     uint32_t pad;
 };
 
-// LDS bytes: state rows, one table and the draw records (not lean), the
-// work-item slot.
-inline uint32_t batched_lds_bytes(uint32_t block, uint32_t rows, uint32_t tableStride, bool lean = false)
+// LDS bytes: state rows, one table (kFwdLds), the draw records (not
+// kFwdLean), the work-item slot.
+inline uint32_t batched_lds_bytes(uint32_t block, uint32_t rows, uint32_t tableStride, int mode = kFwdLds)
 {
-    const uint64_t extra = lean ? 0ull : (uint64_t)tableStride + 4ull * block;
+    const uint64_t extra = (mode == kFwdLds ? (uint64_t)tableStride : 0ull) + (mode != kFwdLean ? 4ull * block : 0ull);
     return (uint32_t)(((uint64_t)rows * 3ull * block + extra + 4ull) * 4ull);
 }
 
-bool batched_block_supported(uint32_t block);
+// Block sizes compiled per mode (FwdMode): 256 and 512 for kFwdLds and
+// kFwdLean, 256, 320 and 384 for kFwdTableGlobal.
+bool batched_block_supported(uint32_t block, int mode);
 // Sets the kernel's dynamic-LDS limit; call outside graph capture.
-hipError_t batched_prepare(uint32_t block, bool lean, uint32_t lds);
+hipError_t batched_prepare(uint32_t block, int mode, uint32_t lds);
 // Reset of the queue counter, the persistent kernel and the per-experiment
 // FP64 finalize, all on stream s (capturable).
-hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, bool lean, uint32_t lds,
+hipError_t launch_batched(const BatchParams &p, uint32_t grid, uint32_t block, int mode, uint32_t lds,
       const uint32_t *expItemStart, const uint32_t *expItems, const double *absorbLog2, uint32_t numExps,
       FinalOut *outs, hipStream_t s);
 

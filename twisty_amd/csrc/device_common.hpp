@@ -569,7 +569,12 @@ __device__ __forceinline__ float unit_tangent(float a0, float a1, float a2, floa
 // ib = 1/|b|): e = (t_b - t_a) N/L and the curvature kappa = |e|
 // (CurvePerturbUtils.cpp:219-243).  The difference is formed as
 // fma(b, ib, -ta), one rounding, then scaled: written out so that every kernel
-// forms the same bits whatever the compiler would contract.
+// forms the same bits whatever the compiler would contract.  kappa is the
+// hardware square root (v_sqrt_f32, one instruction): the IEEE-rounded sqrtf
+// costs about 17 more per joint (denormal scaling and two correction steps).
+// Measured on gfx950 over every non-negative float: on normal inputs 15% of the
+// results differ from the IEEE ones, each by one unit in the last place; inputs
+// below 2^-126 (kappa < 1.1e-19, far inside the table's first bin) give 0.
 struct JointDiff {
     float e0, e1, e2, kappa;
 };
@@ -581,7 +586,7 @@ __device__ __forceinline__ JointDiff joint_diff(float ta0, float ta1, float ta2,
     d.e0 = __fmul_rn(fmaf(b0, ib, -ta0), invDs);
     d.e1 = __fmul_rn(fmaf(b1, ib, -ta1), invDs);
     d.e2 = __fmul_rn(fmaf(b2, ib, -ta2), invDs);
-    d.kappa = sqrtf(fmaf(d.e0, d.e0, fmaf(d.e1, d.e1, __fmul_rn(d.e2, d.e2))));
+    d.kappa = __builtin_amdgcn_sqrtf(fmaf(d.e0, d.e0, fmaf(d.e1, d.e1, __fmul_rn(d.e2, d.e2))));
     return d;
 }
 

@@ -335,6 +335,8 @@ def main(argv=None):
     ap.add_argument("--no-roulette", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched leg: run the library's default schedule instead of the autotuner's")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -411,7 +413,7 @@ def main(argv=None):
         progress(f"C4: forward {out['c4']['forward_ms']:.3f} ms, exact full gradient "
                  f"{out['c4']['full']['fwd_adjoint_ms']:.3f} ms")
     if not a.no_batched:
-        out["batched"] = measure_batched(ctx, capi)
+        out["batched"] = measure_batched(ctx, capi, autotune=not a.no_autotune)
         progress(f"batched: {out['batched']['batched_ms']:.3f} ms")
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
     if not a.no_c5:

@@ -26,7 +26,10 @@ def _kernel_rows(db):
     return con.execute("select name, duration from kernels order by start").fetchall()
 
 
-def kernels(db, skip_first=0, match=None):
+def kernels(db, skip_first=0, match=None, first=0):
+    """Per-kernel dispatch statistics, in dispatch order: the first skip_first
+    dispatches of each kernel are left out and, with first > 0, only the next
+    first dispatches are kept."""
     rows = _kernel_rows(db)
     by = {}
     for name, dur in rows:
@@ -37,6 +40,8 @@ def kernels(db, skip_first=0, match=None):
         if match and match not in name:
             continue
         d = durs[skip_first:] if len(durs) > skip_first else durs
+        if first > 0:
+            d = d[:first]
         ms = [x / 1e6 for x in d]
         out[name] = {"dispatches": len(ms), "mean_ms": statistics.fmean(ms), "min_ms": min(ms), "max_ms": max(ms)}
         lines.append(f"| `{name}` | {len(ms)} | {out[name]['mean_ms']:.4f} | {min(ms):.4f} | {max(ms):.4f} |")
@@ -106,6 +111,7 @@ def main(argv=None):
     k.add_argument("db")
     k.add_argument("--skip-first", type=int, default=0)
     k.add_argument("--match")
+    k.add_argument("--first", type=int, default=0)
     t = sub.add_parser("traffic")
     for n in ("fetch", "write", "calib-fetch", "calib-write"):
         t.add_argument(f"--{n}", required=True)
@@ -118,7 +124,7 @@ def main(argv=None):
     c.add_argument("--json")
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
-        text, _ = kernels(a.db, a.skip_first, a.match)
+        text, _ = kernels(a.db, a.skip_first, a.match, a.first)
         print(text)
     elif a.cmd == "counters":
         text, res = counters(a.db, a.match)

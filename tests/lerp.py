@@ -30,3 +30,36 @@ def joint_log10w(curv, table, absorption):
     with np.errstate(divide="ignore", invalid="ignore"):
         lw = np.log10(w * absorb)
     return lw, left
+
+
+def joint_log10w_spread(curv, table, absorption, delta):
+    """For curvatures known only to within +-delta (per joint, absolute):
+    each joint's log10 weight at curv and the spread (max - min) of the log10
+    weight over [curv - delta, curv + delta].  The weight is the lerp of a
+    table, linear inside a bin and jumping at each bin edge (the reference's
+    fraction is in curvature units), so its extremes on the interval lie at
+    the two ends or on either side of an edge inside it: those are the points
+    sampled (each edge with its float neighbours two ulps either way, since
+    the bin index is formed in float arithmetic)."""
+    k = np.asarray(curv, np.float64)
+    d = np.broadcast_to(np.asarray(delta, np.float64), k.shape)
+    minK, step = float(np.float32(table.minK)), float(np.float32(table.step))
+    lw, _ = joint_log10w(k, table, absorption)
+    pts = [k - d, k + d]
+    lo_bin = np.floor((k - d - minK) / step)
+    hi_bin = np.floor((k + d - minK) / step)
+    # an interval of 2 delta spans at most a few edges; sample each of them
+    for m in range(int(np.max(hi_bin - lo_bin, initial=0)) + 1):
+        e = (minK + (lo_bin + 1 + m) * step).astype(np.float32)
+        inside = (e >= k - d) & (e <= k + d)
+        lo, hi = e, e
+        pts.append(np.where(inside, e.astype(np.float64), k))
+        for _ in range(2):
+            lo = np.nextafter(lo, np.float32(-np.inf))
+            hi = np.nextafter(hi, np.float32(np.inf))
+            pts.append(np.where(inside, lo.astype(np.float64), k))
+            pts.append(np.where(inside, hi.astype(np.float64), k))
+    vals = np.stack([joint_log10w(p, table, absorption)[0] for p in pts] + [lw])
+    with np.errstate(invalid="ignore"):
+        spread = np.nanmax(vals, axis=0) - np.nanmin(vals, axis=0)
+    return lw, spread

@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.lerp import joint_log10w
+from tests.lerp import joint_log10w_spread
 from twisty_amd import capi, host
 
 pytestmark = pytest.mark.gpu
@@ -144,12 +144,32 @@ def test_runner_path_batch_export(hip_device, tmp_path):
     table = host.build_rt_table(float(ds), 1.0, float(np.float32(sigma) ** 2), 0.1, 0.004, numStepsInt=20000,
                                 numCurvatureSteps=4096)
     curv = np.linalg.norm(np.diff(tan, axis=1), axis=2) * (N / np.float32(L))
-    lw, _ = joint_log10w(curv, table, 0.004)
+    # The exported points are the kernel's FP32 segments summed in FP64 and
+    # rounded once to float, so each segment taken back from them is within
+    # one ulp of the largest coordinate per component, each unit tangent within
+    # twice that over ds, and the curvature within (N/L) times the two
+    # tangents' error; the kernel's own FP32 curvature adds a few ulps.  delta
+    # is three times that bound.  Every path's exported log10 weight must
+    # then lie within 1e-5 relative of the re-weighted one plus the spread of
+    # its joints' log10 weights over curvature +- delta (which holds the lerp
+    # jump where a bin edge lies that close).
+    ulp = float(np.spacing(np.float32(np.abs(pos).max())))
+    dseg = math.sqrt(3.0) * ulp
+    delta = 3.0 * ((N / L) * 2.0 * (2.0 * dseg / float(ds)) + 1e-6 * (1.0 + curv))
+    lw, spread = joint_log10w_spread(curv, table, 0.004, delta)
     norm = host.norm_log10(N, float(ds), host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L))
     got = np.log10(r["weights"]) - norm
-    err = np.abs(lw.sum(axis=1) - got) / np.abs(got)
-    print(f"[export] re-weighted curves: median rel {np.median(err):.2e}, 99% {np.quantile(err, 0.99):.2e}", flush=True)
-    assert np.median(err) < 1e-5 and np.quantile(err, 0.99) < 1e-3
+    allow = spread.sum(axis=1)
+    excess = np.abs(lw.sum(axis=1) - got) - (1e-5 * np.abs(got) + allow)
+    loose = allow > 1e-5 * np.abs(got)
+    print(f"[export] re-weighted curves: delta {float(np.max(delta)):.2e}, worst excess {excess.max():.2e}, "
+          f"largest allowance {allow.max():.2e}, {int(loose.sum())} of {n} paths above 1e-5 relative", flush=True)
+    assert excess.max() <= 0.0
+    # The allowance comes from the lerp jumps of joints within delta of a bin
+    # edge (about 2 delta / step of them) and stays far below one joint's
+    # log10 weight, so a joint dropped, doubled or weighed with another table
+    # cannot hide in it.
+    assert allow.max() < 1e-2 * np.abs(lw).min()
     total = math.log10(np.sum(r["weights"]))
     assert abs(total - b["log10_experiment_weight"]) < 1e-5
     assert np.abs(r["five_segment"][:, 0] - np.arccos(tan[:, 1, 2])).max() < 1e-3

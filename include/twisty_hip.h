@@ -317,7 +317,7 @@ typedef struct twisty_hip_experiment {
  * per work item; lean places the tables and the draw records: 0 both in LDS
  * (blockSize 256 or 512), 1 tables read from global memory and draw records
  * exchanged without LDS (blockSize 256 or 512), 2 tables read from global
- * memory and draw records in LDS (blockSize 256, 320 or 384).  Every placement
+ * memory and draw records in LDS (blockSize 256).  Every placement
  * gives the same bits per path. */
 typedef struct twisty_hip_batch_schedule {
     uint32_t block_size;

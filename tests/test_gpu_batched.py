@@ -163,8 +163,8 @@ def test_batched_replays_are_deterministic(hip_device):
 
 @pytest.mark.parametrize("sched", [(256, 0, 8, 1024), (512, 1, 16, 333), (256, 2, 16, 4096), (256, 0, 32, 100000),
                                    (256, 0, 16, 4096, 1), (256, 3, 16, 333, 1), (512, 0, 8, 1024, 1),
-                                   (256, 0, 16, 4096, 2), (320, 0, 16, 1024, 2), (384, 2, 8, 333, 2),
-                                   (320, 3, 32, 4096, 2)])
+                                   (256, 0, 16, 4096, 2), (256, 0, 8, 1024, 2), (256, 2, 8, 333, 2),
+                                   (256, 3, 32, 4096, 2)])
 def test_batched_schedules_agree(hip_device, sched):
     w = hetero_batch()
     ctx = capi.Context(0)

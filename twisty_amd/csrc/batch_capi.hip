@@ -78,7 +78,7 @@ int plan(twisty_hip_batch *b)
         return fail(ctx, TWISTY_HIP_ERR_ARG, "batch: segmentsPerLane in [1, 64], tilePaths >= 1, lean 0, 1 or 2");
     if (!batched_block_supported(sc.block_size, (int)sc.lean))
         return fail(ctx, TWISTY_HIP_ERR_UNSUPPORTED,
-              "batch: blockSize must be 256 or 512 (lean 0 or 1), 256, 320 or 384 (lean 2)");
+              "batch: blockSize must be 256 or 512 (lean 0 or 1), 256 (lean 2)");
     const uint32_t nE = (uint32_t)b->exps.size();
     std::vector<BatchExp> de(nE);
     uint32_t rows = 0;

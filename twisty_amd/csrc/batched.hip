@@ -153,11 +153,11 @@ __global__ void __launch_bounds__(256) batch_finalize_kernel(const BlockPartial 
 
 bool batched_block_supported(uint32_t block, int mode)
 {
-    if (mode == kFwdTableGlobal) return block == 256 || block == 320 || block == 384;
+    if (mode == kFwdTableGlobal) return block == 256;
     return (mode == kFwdLds || mode == kFwdLean) && (block == 256 || block == 512);
 }
 
-#define TWISTY_BATCHED_INSTANCES(X) X(256, 0) X(512, 0) X(256, 1) X(512, 1) X(256, 2) X(320, 2) X(384, 2)
+#define TWISTY_BATCHED_INSTANCES(X) X(256, 0) X(512, 0) X(256, 1) X(512, 1) X(256, 2)
 
 static const void *batched_fn(uint32_t block, int mode)
 {

@@ -63,7 +63,9 @@ inline uint32_t batched_lds_bytes(uint32_t block, uint32_t rows, uint32_t tableS
 }
 
 // Block sizes compiled per mode (FwdMode): 256 and 512 for kFwdLds and
-// kFwdLean, 256, 320 and 384 for kFwdTableGlobal.
+// kFwdLean, 256 for kFwdTableGlobal (at 168 VGPRs a SIMD holds three waves,
+// and a second block of 320 or 384 lanes did not become resident beside the
+// first: those sizes ran at one block per CU).
 bool batched_block_supported(uint32_t block, int mode);
 // Sets the kernel's dynamic-LDS limit; call outside graph capture.
 hipError_t batched_prepare(uint32_t block, int mode, uint32_t lds);

@@ -4,11 +4,12 @@
 // with many geometries (BeamSpreadExperiment.cpp:209-284) launches them one
 // after another.  Here one launch covers a whole batch: the experiments' paths
 // are cut into work items (a range of paths of one experiment), the items are
-// ordered longest first, and a grid of one or two blocks per CU pulls them from
-// a global counter until the list is empty, so short and long experiments
-// share the machine without a tail of idle CUs.  Each item runs the same
-// per-path code as the plain kernel (perturb_path / weigh_state) with the
-// lanes-per-path G of its experiment, chosen from its segment count; the
+// ordered longest first, and a grid of a few blocks per CU pulls them from a
+// global counter until the list is empty, so short and long experiments share
+// the machine without a tail of idle CUs.  Each item runs the same per-path
+// code as the plain kernel (perturb_path / weigh_state) with the lanes-per-path
+// G of its experiment, chosen from its segment count.
+//
 // MODE (FwdMode) places the tables and the draw records as in forward_kernel:
 // kFwdLds keeps the experiment's table in LDS and reloads it only when the next
 // item belongs to an experiment with another table; kFwdTableGlobal reads the
@@ -87,8 +88,9 @@ __device__ __forceinline__ void run_item(const BatchParams &p, const BatchExp &e
     }
 }
 
-// Three waves per SIMD (168 VGPRs, no spills) for the LDS-recorded modes: at
-// eight segments per lane three 256-lane blocks fit a CU's LDS.
+// Three waves per SIMD (168 VGPRs, no spills) for the modes that keep the draw
+// records in LDS, so three 256-lane blocks share a CU wherever their LDS fits
+// (short paths, or the tables in global memory).
 template<int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK, MODE == kFwdLean ? 1 : 3) batched_kernel(const BatchParams p)
 {
@@ -115,8 +117,8 @@ __global__ void __launch_bounds__(BLOCK, MODE == kFwdLean ? 1 : 3) batched_kerne
             __syncthreads();
             curTable = e.table;
         }
-        const TableView tv = make_table_view(TABLE_LDS ? smem + tabOff : p.tableData + bt.offset, bt.size, bt.minK, bt.maxK,
-              bt.step);
+        const TableView tv = make_table_view(TABLE_LDS ? smem + tabOff : p.tableData + bt.offset, bt.size, bt.minK,
+              bt.maxK, bt.step);
         LseAcc acc;
         lse_init(acc);
         uint32_t clamped = 0;

@@ -400,6 +400,11 @@ class Batch:
             pass
 
     def set_schedule(self, block_size=0, blocks_per_cu=0, segments_per_lane=0, tile_paths=0, lean=0):
+        """Launch schedule (twisty_hip_batch_schedule): lean 0 keeps the tables
+        and draw records in LDS, 1 reads the tables from global memory and
+        exchanges the records with ds_bpermute, 2 reads the tables from global
+        memory and keeps the records in LDS (block 256).  Raises and keeps the
+        previous schedule when the combination is not compiled."""
         s = BatchSchedule(block_size, blocks_per_cu, segments_per_lane, tile_paths, lean)
         self.ctx._check(lib().twisty_hip_batch_set_schedule(self._h, C.byref(s)))
 

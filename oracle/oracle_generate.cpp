@@ -24,12 +24,18 @@
 // extent) marks the path invalid instead; the result keeps the reference's
 // sum over valid paths and their count, min and max.
 //
-// The multi-arclength overload (:1388-1597) is oracle_generate_multi below.
+// The multi-arclength overload (:1388-1597) is oracle_generate_multi below, the
+// two-table Metropolis-Hastings overload (:1599-1818, the NoisyCircle
+// experiments' estimator) oracle_generate_chains.
 //
 // Pinned by (tests/test_oracle.py): exact segment lengths and hinge points of
 // every generated path, the closed forms of the straight-line case, the
 // exhaustive segment-count rule, the drawn theta and phi recovered from the
-// geometry, and (multi-ds) the table choice read back from flat tables.
+// geometry, (multi-ds) the table choice read back from flat tables, and
+// (chains) the log against the natural log, the ds draw against the truncated
+// exponential's distribution function, the sampling weight against its closed
+// form, each proposal's table, weight and u read back from flat tables, and
+// the chains against a replay of the acceptance rule.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -252,6 +258,80 @@ bool GeneratePath(const double bc12[12], double ds, uint32_t N, int levels, uint
     return true;
 }
 
+namespace {
+
+// Build-defined natural logarithm of y in (0, 1] in FP64 from basic operations
+// only, so that any IEEE double implementation forms the same bits (the
+// chains' ds draw uses it): y = m 2^e with m in [sqrt(1/2), sqrt(2)), and
+// ln m = 2 atanh(s) = 2 s (1 + z/3 + z^2/5 + ... + z^11/23), s = (m-1)/(m+1),
+// z = s^2 <= 0.0295 (the next term is below 1e-18 of the sum).
+double bd_log(double y)
+{
+    int e = 0;
+    double m = std::frexp(y, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+    for (int k = 10; k >= 0; --k) p = std::fma(p, z, 1.0 / (double)(2 * k + 1));
+    return std::fma((double)e, 0.69314718055994530942, (2.0 * s) * p);
+}
+
+// The chains' ds: minDs + x with x exponential of rate t truncated to
+// [0, maxDs], by the inverse of its distribution function,
+// x = -ln(1 - u (1 - exp(-t maxDs))) / t, u from node 0's first word; oneMinusQ
+// = 1 - exp(-t maxDs) is formed once by the caller.  The reference draws
+// exponentials and rejects those past maxDs + minDs (ExperimentBase.cpp:
+// 1662-1667): the same distribution.
+float chain_ds(double u, double oneMinusQ, float t, float minDs)
+{
+    const double y = std::fma(-u, oneMinusQ, 1.0);
+    const double x = -bd_log(y) / (double)t;
+    return (float)((double)minDs + x);
+}
+
+// log10 of the reference's sampling weight 1 / (t exp(-t (ds - minDs) N)),
+// formed in float as ExperimentBase.cpp:1669-1670 forms it.
+double chain_log10_sampling_weight(float t, float ds, float minDs, uint32_t N)
+{
+    const float sw = 1.0f / (t * std::exp(-t * (ds - minDs) * (float)N));
+    return std::log10((double)sw);
+}
+
+// The table of a multi-ds set whose ds is nearest (first on ties,
+// PathWeightUtils.cpp:223-236).
+uint32_t nearest_table(const std::vector<WeightTable> &wts, float ds)
+{
+    uint32_t best = 0;
+    float bestDist = std::numeric_limits<float>::max();
+    for (uint32_t t = 0; t < (uint32_t)wts.size(); ++t) {
+        const float dist = std::fabs(wts[t].ds - ds);
+        if (dist < bestDist) {
+            bestDist = dist;
+            best = t;
+        }
+    }
+    return best;
+}
+
+std::vector<WeightTable> multi_tables(const float *tables, int32_t bins, const float *meta, uint32_t numTables)
+{
+    std::vector<WeightTable> wts(numTables);
+    for (uint32_t t = 0; t < numTables; ++t) {
+        wts[t].table.assign(tables + (uint64_t)t * bins, tables + (uint64_t)(t + 1) * bins);
+        wts[t].ds = meta[4 * t];
+        wts[t].minCurvature = meta[4 * t + 1];
+        wts[t].maxCurvature = meta[4 * t + 2];
+        wts[t].curvatureStepSize = meta[4 * t + 3];
+    }
+    return wts;
+}
+
+}  // namespace
+
 extern "C" {
 
 int oracle_generate_levels_c(uint32_t N, uint32_t *leafSpan) { return oracle_generate_levels(N, leafSpan); }
@@ -361,14 +441,7 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
     uint32_t leafSpan = 0;
     const int levels = oracle_generate_levels(N, &leafSpan);
     if (levels < 0 || bins < 2 || numTables < 1 || !(minDs > 0.0f) || !(maxDs >= minDs)) return 1;
-    std::vector<WeightTable> wts(numTables);
-    for (uint32_t t = 0; t < numTables; ++t) {
-        wts[t].table.assign(tables + (uint64_t)t * bins, tables + (uint64_t)(t + 1) * bins);
-        wts[t].ds = meta[4 * t];
-        wts[t].minCurvature = meta[4 * t + 1];
-        wts[t].maxCurvature = meta[4 * t + 2];
-        wts[t].curvatureStepSize = meta[4 * t + 3];
-    }
+    const std::vector<WeightTable> wts = multi_tables(tables, bins, meta, numTables);
     if (threads < 1) threads = 1;
     if ((uint64_t)threads > nPaths) threads = (int)std::max<uint64_t>(1, nPaths);
     struct Acc {
@@ -393,15 +466,7 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
                 const uint64_t path = pathOffset + li;
                 const float u = (float)gen_words(seed, path, 0, 0).u0;
                 const float ds = std::fma(u, maxDs - minDs, minDs);
-                uint32_t best = 0;
-                float bestDist = std::numeric_limits<float>::max();
-                for (uint32_t t = 0; t < numTables; ++t) {
-                    const float dist = std::fabs(wts[t].ds - ds);
-                    if (dist < bestDist) {
-                        bestDist = dist;
-                        best = t;
-                    }
-                }
+                const uint32_t best = nearest_table(wts, ds);
                 PathSpec ps;
                 ps.N = N;
                 ps.arclength = ds * (float)N;
@@ -441,6 +506,141 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
     result[4] = tot.mx;
     result[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
+}
+
+// The two-table Metropolis-Hastings overload (ExperimentBase.cpp:1599-1818),
+// the estimator of the NoisyCircle experiments, with chains in place of the
+// reference's OpenMP threads: chains [chainOffset, chainOffset + nChains), each
+// of chainLength proposals; proposal s of chain c draws as path
+// c * chainLength + s.  Per proposal (:1658-1720): ds = chain_ds (node 0's first
+// word, minDs and maxDs as the reference's minDs and actualMaxDs), the table
+// nearest to it, the path built with that ds and weighted as the perturbed
+// paths are, with absorption + scatter as the absorption coefficient
+// (WeightCurveViaPositionLog10_PositionDependent, PathWeighters.cpp:105-177: its
+// object-table branch is commented out, so every joint takes the environment
+// table and the loss exp(-(absorption + scatter) ds) of the table's ds), plus
+// log10 of the sampling weight; extinction = absorption + scatter in float, the
+// rate of the ds draw too.  An invalid proposal is skipped (:1720-1723); a
+// chain's first valid proposal is accepted, a later one when u <= 10^w / 10^w'
+// with the normaliser in both and the ratio in FP64 as pow(10, w)
+// (:1730-1755; u from node 0's second word), and every valid proposal adds the
+// chain's current weight to the sum (:1757-1784).
+// Outputs per proposal (nullable): log10wOut (log10 weight with the sampling
+// weight, without the normaliser; NaN when invalid), validOut, chainOut (the
+// chain's current log10 weight after the proposal, without the normaliser; NaN
+// before its first valid one), curvOut, marginOut (branch margin of the path),
+// acceptMarginOut (|u - ratio| / ratio of the acceptance test; +inf where there
+// was none), uOut (u), tableOut.  result: [log10 of the sum of 10^(chain weight +
+// normaliser) over valid proposals, nValid, nInvalid, min and max of those
+// chain weights with the normaliser, seconds].
+int oracle_generate_chains(const double bc12[12], uint32_t N, uint64_t seed, uint64_t chainOffset, uint64_t nChains,
+      uint64_t chainLength, const float *tables, int32_t bins, const float *meta, uint32_t numTables,
+      float extinction, float minDs, float maxDs, double normLog10, int threads, double *log10wOut,
+      uint8_t *validOut, double *chainOut, float *curvOut, double *marginOut, double *acceptMarginOut,
+      double *uOut, uint32_t *tableOut, double *result)
+{
+    uint32_t leafSpan = 0;
+    const int levels = oracle_generate_levels(N, &leafSpan);
+    if (levels < 0 || bins < 2 || numTables < 1 || !(minDs > 0.0f) || !(maxDs > 0.0f) || !(extinction > 0.0f) ||
+          chainLength < 1 || !((double)extinction * (double)maxDs < 700.0))
+        return 1;
+    const std::vector<WeightTable> wts = multi_tables(tables, bins, meta, numTables);
+    const double oneMinusQ = 1.0 - std::exp(-(double)extinction * (double)maxDs);
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > nChains) threads = (int)std::max<uint64_t>(1, nChains);
+    struct Acc {
+        long double sum = -std::numeric_limits<long double>::infinity();
+        uint64_t nv = 0, ni = 0;
+        double mn = INFINITY, mx = -INFINITY;
+    };
+    std::vector<Acc> accs(threads);
+    std::vector<std::thread> pool;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int th = 0; th < threads; ++th) {
+        const uint64_t b = nChains * (uint64_t)th / (uint64_t)threads;
+        const uint64_t e = nChains * (uint64_t)(th + 1) / (uint64_t)threads;
+        pool.emplace_back([&, th, b, e] {
+            PinToAllowedCpu(th);
+            std::vector<V> p;
+            std::vector<V> tan;
+            std::vector<double> curvT;
+            std::vector<float> curvF;
+            Acc &a = accs[th];
+            for (uint64_t lc = b; lc < e; ++lc) {
+                bool first = true;
+                double prev = NAN;  // the chain's current weight with the normaliser
+                for (uint64_t st = 0; st < chainLength; ++st) {
+                    const uint64_t li = lc * chainLength + st;
+                    const uint64_t path = (chainOffset + lc) * chainLength + st;
+                    const GenWords w0 = gen_words(seed, path, 0, 0);
+                    const float ds = chain_ds(w0.u0, oneMinusQ, extinction, minDs);
+                    const uint32_t best = nearest_table(wts, ds);
+                    PathSpec ps;
+                    ps.N = N;
+                    ps.arclength = ds * (float)N;
+                    if (tableOut) tableOut[li] = best;
+                    if (uOut) uOut[li] = w0.u1;
+                    if (acceptMarginOut) acceptMarginOut[li] = INFINITY;
+                    const bool ok = GeneratePath(bc12, (double)ds, N, levels, leafSpan, seed, path, p,
+                          marginOut ? marginOut + li : nullptr);
+                    PathWeightValue pw { false, NAN };
+                    if (ok)
+                        pw = WeightState(p, ps, wts[best], extinction, tan, curvT, curvF,
+                              curvOut ? curvOut + li * (uint64_t)(N - 1) : nullptr);
+                    if (pw.isValid) pw.weight += chain_log10_sampling_weight(extinction, ds, minDs, N);
+                    if (log10wOut) log10wOut[li] = pw.isValid ? pw.weight : NAN;
+                    if (validOut) validOut[li] = pw.isValid ? 1 : 0;
+                    if (!pw.isValid) {
+                        ++a.ni;
+                        if (chainOut) chainOut[li] = first ? NAN : prev - normLog10;
+                        continue;
+                    }
+                    const double cur = pw.weight + normLog10;
+                    if (first) {
+                        first = false;
+                        prev = cur;
+                    } else {
+                        const double ratio = std::pow(10.0, cur) / std::pow(10.0, prev);
+                        const float u = (float)w0.u1;
+                        if (acceptMarginOut) acceptMarginOut[li] = std::fabs((double)u - ratio) / ratio;
+                        if ((double)u <= ratio) prev = cur;
+                    }
+                    if (chainOut) chainOut[li] = prev - normLog10;
+                    ++a.nv;
+                    a.sum = LogSumExp10(a.sum, (long double)prev);
+                    a.mn = std::min(a.mn, prev);
+                    a.mx = std::max(a.mx, prev);
+                }
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+    Acc tot;
+    for (const Acc &a : accs) {
+        tot.sum = LogSumExp10(tot.sum, a.sum);
+        tot.nv += a.nv;
+        tot.ni += a.ni;
+        tot.mn = std::min(tot.mn, a.mn);
+        tot.mx = std::max(tot.mx, a.mx);
+    }
+    result[0] = (double)tot.sum;
+    result[1] = (double)tot.nv;
+    result[2] = (double)tot.ni;
+    result[3] = tot.mn;
+    result[4] = tot.mx;
+    result[5] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// The build-defined pieces of the chains' draws, for the pins.
+double oracle_bd_log(double y) { return bd_log(y); }
+float oracle_chain_ds(double u, float extinction, float minDs, float maxDs)
+{
+    return chain_ds(u, 1.0 - std::exp(-(double)extinction * (double)maxDs), extinction, minDs);
+}
+double oracle_chain_log10_sampling_weight(float extinction, float ds, float minDs, uint32_t N)
+{
+    return chain_log10_sampling_weight(extinction, ds, minDs, N);
 }
 
 }  // extern "C"

@@ -87,6 +87,16 @@ def _declare(L):
                                         f32, f32, f32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, _p(np.float64)]
     L.oracle_generate_multi.restype = C.c_int
+    L.oracle_generate_chains.argtypes = [_p(np.float64), u32, u64, u64, u64, u64, _p(np.float32), i32,
+                                         _p(np.float32), u32, f32, f32, f32, C.c_double, C.c_int] + \
+        [C.c_void_p] * 8 + [_p(np.float64)]
+    L.oracle_generate_chains.restype = C.c_int
+    L.oracle_bd_log.argtypes = [C.c_double]
+    L.oracle_bd_log.restype = C.c_double
+    L.oracle_chain_ds.argtypes = [C.c_double, f32, f32, f32]
+    L.oracle_chain_ds.restype = C.c_float
+    L.oracle_chain_log10_sampling_weight.argtypes = [f32, f32, f32, u32]
+    L.oracle_chain_log10_sampling_weight.restype = C.c_double
 
 
 # ---------------------------------------------------------------------------
@@ -277,6 +287,49 @@ def generate_multi(bc, N, n_paths, tables, meta, absorption, min_ds, max_ds, see
     return {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
             "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool),
             "curv": curv.reshape(n_paths, N - 1), "margin": margin, "table": tab}
+
+
+def generate_chains(bc, N, n_chains, chain_length, tables, meta, extinction, min_ds, max_ds, norm_log10=0.0, seed=1,
+                    chain_offset=0, threads=None):
+    """Metropolis-Hastings chains of the two-table independent-path estimator
+    (oracle_generate_chains): tables (T, bins), meta (T, 4).  Per proposal
+    (chain-major): log10 weight, validity, the chain's weight after it, joint
+    curvatures, branch margin, acceptance margin, u and table."""
+    threads = threads or os.cpu_count() or 1
+    t = np.ascontiguousarray(tables, np.float32)
+    m = np.ascontiguousarray(meta, np.float32)
+    n = n_chains * chain_length
+    res = np.zeros(6, np.float64)
+    w = np.zeros(n, np.float64)
+    v = np.zeros(n, np.uint8)
+    ch = np.zeros(n, np.float64)
+    curv = np.full(n * (N - 1), np.nan, np.float32)
+    margin = np.zeros(n, np.float64)
+    am = np.zeros(n, np.float64)
+    u = np.zeros(n, np.float64)
+    tab = np.zeros(n, np.uint32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = lib().oracle_generate_chains(np.ascontiguousarray(bc, np.float64), N, seed, chain_offset, n_chains,
+                                      chain_length, t, t.shape[1], m, t.shape[0], extinction, min_ds, max_ds,
+                                      norm_log10, threads, p(w), p(v), p(ch), p(curv), p(margin), p(am), p(u), p(tab),
+                                      res)
+    if rc != 0:
+        raise ValueError("oracle_generate_chains rejected its arguments")
+    return {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
+            "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool), "chain": ch,
+            "curv": curv.reshape(n, N - 1), "margin": margin, "accept_margin": am, "u": u, "table": tab}
+
+
+def bd_log(y):
+    return lib().oracle_bd_log(float(y))
+
+
+def chain_ds(u, extinction, min_ds, max_ds):
+    return lib().oracle_chain_ds(float(u), extinction, min_ds, max_ds)
+
+
+def chain_log10_sampling_weight(extinction, ds, min_ds, N):
+    return lib().oracle_chain_log10_sampling_weight(extinction, ds, min_ds, N)
 
 
 def exit_bins(tangents, e, n_theta, n_phi):

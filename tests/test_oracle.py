@@ -668,3 +668,128 @@ def test_generated_multi_ds_table_choice(oracle):
         absorb = float(np.float32(math.exp(float(np.float32(np.float32(-0.004) * dss[t])))))
         want = (N - 1) * math.log10(float(consts[t]) * absorb)
         assert r["log10w"][p] == pytest.approx(want, rel=1e-6), p  # float lerp and product per joint
+
+
+def test_chain_log_is_the_natural_log(oracle):
+    """The build-defined log of the chains' ds draw (frexp, atanh series) is
+    the natural log to a few ulps over (0, 1]."""
+    ys = np.concatenate([np.geomspace(1e-300, 1.0, 2000), 1.0 - np.geomspace(1e-16, 0.5, 500), [1.0, 0.5, 0.75]])
+    for y in ys:
+        want = math.log(y)
+        got = oracle.bd_log(float(y))
+        assert abs(got - want) <= 4e-16 * abs(want) + 1e-300, (y, got, want)
+
+
+@pytest.mark.parametrize("t,max_ds", [(0.104, 0.03), (5.0, 1.0), (40.0, 0.2)])
+def test_chain_ds_is_the_truncated_exponential(oracle, t, max_ds):
+    """ds = minDs + x with x exponential of rate t truncated to [0, maxDs]
+    (ExperimentBase.cpp:1662-1667 draws the same distribution by rejection):
+    the draw inverts the truncated distribution function
+    F(x) = (1 - exp(-t x)) / (1 - exp(-t maxDs)), so F(ds(u) - minDs) = u up to
+    the float rounding of ds, ds(0) = minDs and ds rises with u."""
+    t32, lo, hi = np.float32(t), np.float32(0.02), np.float32(max_ds)
+    us = np.concatenate([np.arange(0, 1 << 24, 4099) / float(1 << 24), [1.0 - 2.0 ** -24]])
+    ds = np.array([oracle.chain_ds(u, float(t32), float(lo), float(hi)) for u in us], np.float64)
+    assert ds[0] == float(lo) and np.all(np.diff(ds) >= 0.0) and ds[-1] <= float(lo) + float(hi) * (1 + 1e-6)
+    x = ds - float(lo)
+    F = -np.expm1(-float(t32) * x) / -math.expm1(-float(t32) * float(hi))
+    # one float ulp of ds moves F by at most the density times that ulp
+    dens = float(t32) * np.exp(-float(t32) * x) / -math.expm1(-float(t32) * float(hi))
+    slack = dens * np.spacing(ds.astype(np.float32)).astype(np.float64) + 1e-12
+    assert np.all(np.abs(F - us) <= slack), np.max(np.abs(F - us) - slack)
+
+
+def test_chain_sampling_weight_closed_form(oracle):
+    """log10 of 1 / (t exp(-t (ds - minDs) N)), formed in float
+    (ExperimentBase.cpp:1669-1670), against -log10 t + t (ds - minDs) N log10 e."""
+    for t, ds, lo, N in [(0.104, 0.05, 0.03, 34), (1.5, 0.2, 0.1, 66), (0.01, 0.031, 0.03, 10)]:
+        got = oracle.chain_log10_sampling_weight(t, ds, lo, N)
+        t32, d32 = float(np.float32(t)), float(np.float32(np.float32(ds) - np.float32(lo)))
+        want = -math.log10(t32) + t32 * d32 * N / math.log(10.0)
+        assert got == pytest.approx(want, rel=1e-6, abs=1e-6)
+
+
+def _chain_case(oracle, n_chains=48, length=40, norm=2.0, seed=3):
+    from twisty_amd import host
+    N = 34
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
+    bc13[9:12] /= np.linalg.norm(bc13[9:12])
+    lo = np.float32(host.minimum_arclength(bc13, N) / N)
+    hi = np.float32(0.03)
+    tabs, _, _ = host.cached_rt_tables(float(lo) - 0.001, float(lo + hi) + 0.001, 8, 1.0,
+                                       float(np.float32(0.08) ** 2), 0.1, 0.004, numStepsInt=2000,
+                                       numCurvatureSteps=512)
+    tables = np.stack([t.table for t in tabs])
+    meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    r = oracle.generate_chains(bc13[:12].astype(np.float64), N, n_chains, length, tables, meta, ext, float(lo),
+                               float(hi), norm_log10=norm, seed=seed)
+    return r, n_chains, length, norm
+
+
+def test_chain_acceptance_rule(oracle):
+    """The chains against a replay of ExperimentBase.cpp:1730-1784 from the
+    oracle's own proposal weights and draws: an invalid proposal is skipped, a
+    chain's first valid proposal is taken, a later one when
+    u <= 10^(w + norm) / 10^(w' + norm), and every valid proposal adds the
+    chain's current weight; the result is the sum over those.  Also: a
+    proposal at least as heavy as the current state is always taken, and the
+    current state is always one of the chain's own proposals."""
+    r, C, L, norm = _chain_case(oracle)
+    w, v, u, ch = (r[k].reshape(C, L) for k in ("log10w", "valid", "u", "chain"))
+    total, nv = -math.inf, 0
+    tests = taken = 0
+    for c in range(C):
+        cur = None
+        for s in range(L):
+            if v[c, s]:
+                if cur is None:
+                    cur = w[c, s] + norm
+                else:
+                    tests += 1
+                    ratio = 10.0 ** (w[c, s] + norm) / 10.0 ** cur
+                    if float(np.float32(u[c, s])) <= ratio:
+                        cur = w[c, s] + norm
+                        taken += 1
+                    if w[c, s] + norm >= cur:
+                        assert ch[c, s] + norm == w[c, s] + norm
+                nv += 1
+                m = max(total, cur)
+                total = m + math.log10(10.0 ** (total - m) + 10.0 ** (cur - m))
+            if cur is None:
+                assert np.isnan(ch[c, s])
+            else:
+                assert ch[c, s] == pytest.approx(cur - norm, abs=1e-12), (c, s)
+                assert np.any(np.abs(w[c, :s + 1] - ch[c, s]) <= 1e-12)
+    assert nv == r["n_valid"] and 0 < taken < tests
+    assert r["log10_sum_w"] == pytest.approx(total, rel=1e-12)
+
+
+def test_chain_proposals_with_flat_tables(oracle):
+    """Each proposal of the chains (ExperimentBase.cpp:1658-1718): with flat
+    tables of distinct constants its weight names its table, which must be the
+    one nearest the proposal's ds drawn from node 0's first word, weighted with
+    absorption + scatter on every joint (PathWeighters.cpp:105-177) plus log10
+    of the sampling weight; u is node 0's second word."""
+    N, T, bins, C, L = 10, 7, 64, 20, 10
+    dss = np.linspace(0.09, 0.16, T).astype(np.float32)
+    consts = np.float32(1.0) + np.arange(T, dtype=np.float32)
+    tables = np.repeat(consts[:, None], bins, axis=1)
+    meta = np.stack([dss, np.zeros(T, np.float32), np.full(T, 100.0, np.float32), np.full(T, 100.0 / (bins - 1),
+                                                                                        np.float32)], axis=1)
+    lo, hi = np.float32(0.1), np.float32(0.05)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    r = oracle.generate_chains(oracle.bc12(), N, C, L, tables, meta, ext, float(lo), float(hi), seed=6, chain_offset=3)
+    assert r["valid"].all()
+    for c in range(C):
+        for s in range(L):
+            p, path = c * L + s, (3 + c) * L + s
+            wd = _words(oracle, 6, path, 0)
+            assert r["u"][p] == wd[1]
+            ds = np.float32(oracle.chain_ds(wd[0], ext, float(lo), float(hi)))
+            t = int(np.argmin(np.abs(dss - ds)))
+            assert r["table"][p] == t
+            absorb = float(np.float32(math.exp(float(np.float32(np.float32(-ext) * dss[t])))))
+            want = (N - 1) * math.log10(float(consts[t]) * absorb) + \
+                oracle.chain_log10_sampling_weight(ext, float(ds), float(lo), N)
+            assert r["log10w"][p] == pytest.approx(want, rel=1e-6), p

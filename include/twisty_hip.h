@@ -198,6 +198,34 @@ int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint6
       float minDs, float maxDs, twisty_hip_partials *out, double *minLog10W, float *perPathLog10,
       float *perJointCurvature);
 
+/* The two-table Metropolis-Hastings form of the independent-path estimator
+ * (Experiments/ExperimentBase.cpp:1599-1818, the NoisyCircle experiments'
+ * MSegmentPathGenerationMC with environment and object tables), with chains in
+ * place of the reference's OpenMP threads: chains [chainOffset, chainOffset +
+ * numChains), each of chainLength proposals; proposal s of chain c draws as path
+ * c * chainLength + s.  A proposal draws ds = minDs + x, x exponential of rate
+ * extinction truncated to [0, maxDs] (minDs = the boundary's minimum arclength
+ * / N and maxDs the reference's actualMaxDs; the reference rejects draws past
+ * maxDs + minDs), is built with that ds and weighted with the cached table
+ * nearest to it and the loss exp(-extinction * table ds) on every joint
+ * (WeightCurveViaPositionLog10_PositionDependent, PathWeighters.cpp:105-177,
+ * whose object-table branch is commented out), plus log10 of the sampling
+ * weight 1 / (t exp(-t (ds - minDs) N)).  extinction = absorption + scatter
+ * (float), and set_multi_tables must have been given it as its absorption.  An
+ * invalid proposal is skipped; a chain takes its first valid proposal and a
+ * later one when u <= 10^(w + norm) / 10^(w' + norm), and every valid proposal
+ * adds the chain's current weight to the partials (so the estimate is their
+ * sum over out->num_valid, with pathNormalizerLog10 added by the caller).
+ * perPathLog10 (nullable): each proposal's log10 weight, sampling weight
+ * included, normaliser not (NaN when invalid); chainLog10 (nullable): the
+ * chain's log10 weight after each proposal (NaN before its first valid one);
+ * both numChains * chainLength floats, chain-major; minLog10W: the smallest
+ * chain weight summed.  Draws are build-defined (Philox; the ds draw through
+ * a log formed from basic FP64 operations), not the reference's mt19937_64. */
+int twisty_hip_run_generated_chains(twisty_hip_ctx *ctx, uint64_t numChains, uint64_t chainLength,
+      uint64_t chainOffset, uint64_t seed, float minDs, float maxDs, float extinction, double pathNormalizerLog10,
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10, float *chainLog10, float *perJointCurvature);
+
 /* Burn-in of the following twisty_hip_run calls: every chain first applies
  * pathsPerChain unweighted paths of K perturbations (draws keyed by path id
  * 2^63 | (chain * pathsPerChain + b), chain = absolute path id / chainLength),

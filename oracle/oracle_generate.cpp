@@ -530,14 +530,14 @@ int oracle_generate_multi(const double bc12[12], uint32_t N, uint64_t seed, uint
 // chain's current log10 weight after the proposal, without the normaliser; NaN
 // before its first valid one), curvOut, marginOut (branch margin of the path),
 // acceptMarginOut (|u - ratio| / ratio of the acceptance test; +inf where there
-// was none), uOut (u), tableOut.  result: [log10 of the sum of 10^(chain weight +
+// was none), uOut (u), dsOut (ds), tableOut.  result: [log10 of the sum of 10^(chain weight +
 // normaliser) over valid proposals, nValid, nInvalid, min and max of those
 // chain weights with the normaliser, seconds].
 int oracle_generate_chains(const double bc12[12], uint32_t N, uint64_t seed, uint64_t chainOffset, uint64_t nChains,
       uint64_t chainLength, const float *tables, int32_t bins, const float *meta, uint32_t numTables,
       float extinction, float minDs, float maxDs, double normLog10, int threads, double *log10wOut,
       uint8_t *validOut, double *chainOut, float *curvOut, double *marginOut, double *acceptMarginOut,
-      double *uOut, uint32_t *tableOut, double *result)
+      double *uOut, float *dsOut, uint32_t *tableOut, double *result)
 {
     uint32_t leafSpan = 0;
     const int levels = oracle_generate_levels(N, &leafSpan);
@@ -580,6 +580,7 @@ int oracle_generate_chains(const double bc12[12], uint32_t N, uint64_t seed, uin
                     ps.arclength = ds * (float)N;
                     if (tableOut) tableOut[li] = best;
                     if (uOut) uOut[li] = w0.u1;
+                    if (dsOut) dsOut[li] = ds;
                     if (acceptMarginOut) acceptMarginOut[li] = INFINITY;
                     const bool ok = GeneratePath(bc12, (double)ds, N, levels, leafSpan, seed, path, p,
                           marginOut ? marginOut + li : nullptr);

@@ -89,7 +89,7 @@ def _declare(L):
     L.oracle_generate_multi.restype = C.c_int
     L.oracle_generate_chains.argtypes = [_p(np.float64), u32, u64, u64, u64, u64, _p(np.float32), i32,
                                          _p(np.float32), u32, f32, f32, f32, C.c_double, C.c_int] + \
-        [C.c_void_p] * 8 + [_p(np.float64)]
+        [C.c_void_p] * 9 + [_p(np.float64)]
     L.oracle_generate_chains.restype = C.c_int
     L.oracle_bd_log.argtypes = [C.c_double]
     L.oracle_bd_log.restype = C.c_double
@@ -294,7 +294,7 @@ def generate_chains(bc, N, n_chains, chain_length, tables, meta, extinction, min
     """Metropolis-Hastings chains of the two-table independent-path estimator
     (oracle_generate_chains): tables (T, bins), meta (T, 4).  Per proposal
     (chain-major): log10 weight, validity, the chain's weight after it, joint
-    curvatures, branch margin, acceptance margin, u and table."""
+    curvatures, branch margin, acceptance margin, u, ds and table."""
     threads = threads or os.cpu_count() or 1
     t = np.ascontiguousarray(tables, np.float32)
     m = np.ascontiguousarray(meta, np.float32)
@@ -307,17 +307,19 @@ def generate_chains(bc, N, n_chains, chain_length, tables, meta, extinction, min
     margin = np.zeros(n, np.float64)
     am = np.zeros(n, np.float64)
     u = np.zeros(n, np.float64)
+    ds = np.zeros(n, np.float32)
     tab = np.zeros(n, np.uint32)
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     rc = lib().oracle_generate_chains(np.ascontiguousarray(bc, np.float64), N, seed, chain_offset, n_chains,
                                       chain_length, t, t.shape[1], m, t.shape[0], extinction, min_ds, max_ds,
-                                      norm_log10, threads, p(w), p(v), p(ch), p(curv), p(margin), p(am), p(u), p(tab),
-                                      res)
+                                      norm_log10, threads, p(w), p(v), p(ch), p(curv), p(margin), p(am), p(u), p(ds),
+                                      p(tab), res)
     if rc != 0:
         raise ValueError("oracle_generate_chains rejected its arguments")
     return {"log10_sum_w": res[0], "n_valid": int(res[1]), "n_invalid": int(res[2]), "min_log10_w": res[3],
             "max_log10_w": res[4], "seconds": res[5], "log10w": w, "valid": v.astype(bool), "chain": ch,
-            "curv": curv.reshape(n, N - 1), "margin": margin, "accept_margin": am, "u": u, "table": tab}
+            "curv": curv.reshape(n, N - 1), "margin": margin, "accept_margin": am, "u": u, "ds": ds,
+            "table": tab}
 
 
 def bd_log(y):

@@ -245,3 +245,91 @@ def test_host_path_generation_mc_multi(hip_device, tmp_path):
     print(f"[path_generation_mc_multi] {r}", flush=True)
     assert r["num_total"] == 20000 and r["num_valid"] == g["num_valid"]
     assert abs(r["log10_total_weight"] - want) < 1e-5 * abs(want)
+
+
+def test_generated_chains_match_oracle(hip_device, oracle):
+    """The two-table Metropolis-Hastings form (ExperimentBase.cpp:1599-1818)
+    against the oracle on the same draws: every proposal by the per-path rule
+    (truncated-exponential ds, nearest cached table, loss absorption + scatter
+    on every joint; the sampling weight, a function of ds alone, taken off
+    both sides first); every chain step against a replay of the acceptance rule
+    on the kernel's own proposal weights and the draws u; the partials against
+    the chain weights they sum; and the chains against the oracle's."""
+    N = 34
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
+    bc13[9:12] /= np.linalg.norm(bc13[9:12])
+    lo = np.float32(host.minimum_arclength(bc13, N) / N)
+    hi = np.float32(0.03)
+    tabs, _, _ = host.cached_rt_tables(float(lo) - 0.001, float(lo + hi) + 0.001, 12, 1.0,
+                                       float(np.float32(0.08) ** 2), 0.1, 0.004, numStepsInt=2000,
+                                       numCurvatureSteps=1024)
+    tables = np.stack([t.table for t in tabs])
+    meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    C, L, norm = 256, 16, 2.0
+    ctx = capi.Context(0)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    ctx.set_multi_tables(tables, meta, ext, N)
+    g = ctx.run_generated_chains(C, L, float(lo), float(hi), ext, norm_log10=norm, seed=9, chain_offset=5,
+                                 per_path=True, per_joint=True)
+    ref = oracle.generate_chains(bc13[:12].astype(np.float64), N, C, L, tables, meta, ext, float(lo), float(hi),
+                                 norm_log10=norm, seed=9, chain_offset=5)
+    n = C * L
+    assert g["num_paths"] == n and g["num_valid"] + g["num_invalid"] == n
+    # proposals: the per-path rule, the sampling weight taken off both sides
+    sw = np.array([oracle.chain_log10_sampling_weight(ext, float(d), float(lo), N) for d in ref["ds"]])
+    views = [types.SimpleNamespace(table=tables[i], ds=float(meta[i, 0]), minK=float(meta[i, 1]),
+                                   maxK=float(meta[i, 2]), step=float(meta[i, 3])) for i in range(len(tables))]
+    gp = {"log10w": (g["log10w"].astype(np.float64) - sw), "curv": g["curv"]}
+    rp = dict(ref, log10w=ref["log10w"] - sw)
+    check_generated(f"generated chains N={N}", gp, rp, views, ref["table"].astype(np.int64), ext)
+    # the chains: the acceptance rule replayed on the kernel's proposals and draws
+    w = g["log10w"].astype(np.float64).reshape(C, L) + norm
+    u = ref["u"].reshape(C, L)
+    ch = g["chain"].astype(np.float64).reshape(C, L) + norm
+    close, resync, total, nv = 0, 0, -math.inf, 0
+    for c in range(C):
+        cur = None
+        for s in range(L):
+            if not np.isnan(w[c, s]):
+                if cur is None:
+                    cur = w[c, s]
+                else:
+                    ratio = 10.0 ** w[c, s] / 10.0 ** cur
+                    take = float(np.float32(u[c, s])) <= ratio
+                    if abs(u[c, s] - ratio) <= 1e-5 * ratio:  # the kernel tests its FP64 weight, not the float
+                        close += 1
+                        take = ch[c, s] == w[c, s]
+                    if take:
+                        cur = w[c, s]
+                nv += 1
+                m = max(total, cur)
+                total = m + math.log10(10.0 ** (total - m) + 10.0 ** (cur - m))
+            if cur is None:
+                assert np.isnan(ch[c, s]), (c, s)
+            else:
+                assert ch[c, s] == cur, (c, s, ch[c, s], cur)
+    assert nv == g["num_valid"] and close <= 2
+    assert abs(g["log10_sum_w"] + norm - total) < 1e-6 * abs(total)
+    # against the oracle's chains: the same decision (valid or not, proposal
+    # taken or not) at every step until one falls within the two sides' weight
+    # difference (the oracle's acceptance margin) or a marginal geometry
+    gv, rv = ~np.isnan(w), ref["valid"].reshape(C, L)
+    gtake = gv & (ch == w)
+    rtake = rv & np.isclose(ref["chain"].reshape(C, L), ref["log10w"].reshape(C, L), rtol=1e-13, atol=0)
+    am, gm = ref["accept_margin"].reshape(C, L), ref["margin"].reshape(C, L)
+    diverged = 0
+    for c in range(C):
+        for s in range(L):
+            if gv[c, s] != rv[c, s]:
+                assert gm[c, s] < MARGIN, (c, s)
+            elif gtake[c, s] != rtake[c, s]:
+                assert am[c, s] < 2e-2, (c, s, am[c, s])
+            else:
+                continue
+            diverged += 1
+            break
+    print(f"[generated chains] {n} proposals, valid {g['num_valid']}, close replay decisions {close}, chains "
+          f"leaving the oracle's {diverged} of {C}", flush=True)
+    assert diverged <= C // 10
+    ctx.close()

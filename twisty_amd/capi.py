@@ -23,7 +23,7 @@ EXPORTED = [
     "twisty_hip_set_weight_histogram", "twisty_hip_get_weight_histogram",
     "twisty_hip_set_keep_final_state", "twisty_hip_get_final_positions", "twisty_hip_set_burn_in",
     "twisty_hip_run_generated", "twisty_hip_set_adjoint_roulette", "twisty_hip_last_adjoint_paths",
-    "twisty_hip_set_multi_tables", "twisty_hip_run_generated_multi",
+    "twisty_hip_set_multi_tables", "twisty_hip_run_generated_multi", "twisty_hip_run_generated_chains",
     "twisty_hip_set_exit_histogram", "twisty_hip_get_exit_histogram", "twisty_hip_run_adjoint",
     "twisty_hip_set_adjoint_schedule", "twisty_hip_last_adjoint_timing",
     "twisty_hip_batch_create", "twisty_hip_batch_destroy", "twisty_hip_batch_set_schedule",
@@ -116,6 +116,8 @@ def _declare(L):
     L.twisty_hip_last_adjoint_paths.argtypes = [vp, C.POINTER(u64)]
     L.twisty_hip_set_multi_tables.argtypes = [vp, vp, u32, u32, vp, f32, u32]
     L.twisty_hip_run_generated_multi.argtypes = [vp, u64, u64, u64, f32, f32, vp, C.POINTER(C.c_double), vp, vp]
+    L.twisty_hip_run_generated_chains.argtypes = [vp, u64, u64, u64, u64, f32, f32, f32, C.c_double, vp,
+                                                  C.POINTER(C.c_double), vp, vp, vp]
     L.twisty_hip_run_generated.argtypes = [vp, u64, u64, u64, vp, C.POINTER(C.c_double), vp, vp]
     L.twisty_hip_get_final_positions.argtypes = [vp, vp, u64]
     L.twisty_hip_get_weight_histogram.argtypes = [vp, vp]
@@ -267,6 +269,29 @@ class Context:
             res["log10w"] = pp
         if per_joint:
             res["curv"] = pj.reshape(num_paths, self.N - 1)
+        return res
+
+    def run_generated_chains(self, num_chains, chain_length, min_ds, max_ds, extinction, norm_log10=0.0, seed=1,
+                             chain_offset=0, per_path=False, per_joint=False):
+        """Metropolis-Hastings chains of the two-table independent-path estimator
+        (twisty_hip_run_generated_chains): per proposal (chain-major) its log10
+        weight ("log10w") and the chain's weight after it ("chain")."""
+        n = num_chains * chain_length
+        out = Partials()
+        mn = C.c_double()
+        pp = np.empty(n, np.float32) if per_path else None
+        ch = np.empty(n, np.float32) if per_path else None
+        pj = np.empty(n * (self.N - 1), np.float32) if per_joint else None
+        self._check(lib().twisty_hip_run_generated_chains(self._h, num_chains, chain_length, chain_offset, seed, min_ds,
+                                                          max_ds, extinction, norm_log10, C.byref(out), C.byref(mn),
+                                                          _ptr(pp), _ptr(ch), _ptr(pj)))
+        res = out.as_dict()
+        res["min_log10_w"] = mn.value
+        if per_path:
+            res["log10w"] = pp
+            res["chain"] = ch
+        if per_joint:
+            res["curv"] = pj.reshape(n, self.N - 1)
         return res
 
     def last_timing(self):

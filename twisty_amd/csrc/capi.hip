@@ -407,6 +407,7 @@ int twisty_hip_destroy(twisty_hip_ctx *ctx)
     (void)hipFree(ctx->dPick);
     (void)hipFree(ctx->dMultiTables);
     (void)hipFree(ctx->dMultiMeta);
+    (void)hipFree(ctx->dChainLog);
     (void)hipFree(ctx->dPickCount);
     (void)hipFree(ctx->dGradTPartial);
     (void)hipFree(ctx->dGradSPartial);
@@ -647,17 +648,42 @@ int twisty_hip_last_adjoint_paths(twisty_hip_ctx *ctx, uint64_t *pathsVisited)
     return TWISTY_HIP_OK;
 }
 
+// The chains form's extra arguments (kGenChains).
+struct ChainArgs {
+    uint64_t numChains = 0, chainLength = 0;
+    float extinction = 0.0f;
+    double normLog10 = 0.0;
+    float *chainLog10 = nullptr;
+};
+
 static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pathOffset, uint64_t seed,
-      bool multi, float dsLo, float dsHi, twisty_hip_partials *out, double *minLog10W, float *perPathLog10,
-      float *perJointCurvature)
+      int mode, float dsLo, float dsHi, const ChainArgs &ch, twisty_hip_partials *out, double *minLog10W,
+      float *perPathLog10, float *perJointCurvature)
 {
+    const bool multi = mode != kGenSingle;
     if (!out) return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: out is NULL");
     if (!ctx->haveCurve) return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_curve first");
     if (!multi && !ctx->haveTables) return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated: set_tables first");
     if (multi && ctx->multiCount == 0)
         return fail(ctx, TWISTY_HIP_ERR_STATE, "run_generated_multi: set_multi_tables first");
-    if (multi && !(dsLo > 0.0f && dsHi >= dsLo && std::isfinite(dsHi)))
+    if (mode == kGenMulti && !(dsLo > 0.0f && dsHi >= dsLo && std::isfinite(dsHi)))
         return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_multi: need 0 < minDs <= maxDs");
+    if (mode == kGenChains) {
+        if (!(dsLo > 0.0f && dsHi > 0.0f && std::isfinite(dsHi) && ch.extinction > 0.0f &&
+                  (double)ch.extinction * (double)dsHi < 700.0 && std::isfinite(ch.normLog10)))
+            return fail(ctx, TWISTY_HIP_ERR_ARG,
+                  "run_generated_chains: need minDs > 0, maxDs > 0, extinction > 0 with extinction * maxDs < 700 "
+                  "and a finite normaliser");
+        if (ch.extinction != ctx->multiAbsorption)
+            return fail(ctx, TWISTY_HIP_ERR_ARG,
+                  "run_generated_chains: set_multi_tables must be given absorption = extinction (absorption + "
+                  "scatter, the loss of every joint)");
+        if (ch.numChains == 0 || ch.chainLength == 0 || ch.chainLength > (1ull << 32) ||
+              ch.numChains > (1ull << 40) / ch.chainLength || pathOffset > (1ull << 62) / ch.chainLength - ch.numChains)
+            return fail(ctx, TWISTY_HIP_ERR_ARG,
+                  "run_generated_chains: need 1 <= numChains * chainLength <= 2^40 and chain ids below 2^62 / "
+                  "chainLength");
+    }
     if (numPaths == 0 || numPaths > (1ull << 40))
         return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated: need 1 <= numPaths <= 2^40");
     const uint32_t N = ctx->N;
@@ -689,6 +715,8 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
           (rc = ensure_dev(ctx, ctx->dPerJoint, ctx->perJointCap, numPaths * (uint64_t)(N - 1), "per-joint buffer")))
         return rc;
     if ((rc = ensure_dev(ctx, ctx->dGenMin, ctx->genMinCap, 1, "min buffer"))) return rc;
+    if (ch.chainLog10 && (rc = ensure_dev(ctx, ctx->dChainLog, ctx->chainLogCap, numPaths, "chain buffer")))
+        return rc;
     // the multi-ds form adds each path's own table absorption in the kernel
     const double absorbLog2PerPath =
           multi ? 0.0 : (double)(N - 1) * std::log2((double)std::exp((float)(-ctx->absorption * ctx->ds)));
@@ -722,6 +750,14 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
         p.dsLo = dsLo;
         p.dsHi = dsHi;
     }
+    if (mode == kGenChains) {
+        p.numChains = ch.numChains;
+        p.chainLength = ch.chainLength;
+        p.extinction = ch.extinction;
+        p.oneMinusQ = 1.0 - std::exp(-(double)ch.extinction * (double)dsHi);
+        p.normLog10 = ch.normLog10;
+        p.chainLog10 = ch.chainLog10 ? ctx->dChainLog : nullptr;
+    }
     p.minK = ctx->minK;
     p.maxK = ctx->maxK;
     p.step = ctx->step;
@@ -730,7 +766,7 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
     const float inf = INFINITY;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->dGenMin, &inf, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipError_t e = launch_generate(p, G, multi, grid, B, lds, ctx->stream);
+    hipError_t e = launch_generate(p, G, mode, grid, B, lds, ctx->stream);
     if (e != hipSuccess) return hipfail(ctx, e, "generate kernel launch");
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     e = launch_finalize(ctx->dPartials, grid, absorbLog2PerPath, ctx->dFinal, ctx->stream);
@@ -746,6 +782,9 @@ static int run_generated_impl(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t p
     if (perJointCurvature)
         HIP_TRY(ctx, hipMemcpyAsync(perJointCurvature, ctx->dPerJoint, numPaths * (uint64_t)(N - 1) * sizeof(float),
                            hipMemcpyDeviceToHost, ctx->stream));
+    if (ch.chainLog10)
+        HIP_TRY(ctx, hipMemcpyAsync(ch.chainLog10, ctx->dChainLog, numPaths * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastFwdMs, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&ctx->lastTotalMs, ctx->ev0, ctx->ev2));
@@ -764,8 +803,8 @@ int twisty_hip_run_generated(twisty_hip_ctx *ctx, uint64_t numPaths, uint64_t pa
       twisty_hip_partials *out, double *minLog10W, float *perPathLog10, float *perJointCurvature)
 {
     if (!ctx) return TWISTY_HIP_ERR_ARG;
-    return run_generated_impl(ctx, numPaths, pathOffset, seed, false, 0.0f, 0.0f, out, minLog10W, perPathLog10,
-          perJointCurvature);
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, kGenSingle, 0.0f, 0.0f, ChainArgs {}, out, minLog10W,
+          perPathLog10, perJointCurvature);
 }
 
 int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32_t numBins, uint32_t numTables,
@@ -802,6 +841,7 @@ int twisty_hip_set_multi_tables(twisty_hip_ctx *ctx, const float *tables, uint32
     ctx->multiBins = numBins;
     ctx->multiCount = numTables;
     ctx->multiSegments = numSegments;
+    ctx->multiAbsorption = absorption;
     return TWISTY_HIP_OK;
 }
 
@@ -812,8 +852,27 @@ int twisty_hip_run_generated_multi(twisty_hip_ctx *ctx, uint64_t numPaths, uint6
     if (!ctx) return TWISTY_HIP_ERR_ARG;
     if (ctx->multiCount && ctx->haveCurve && ctx->multiSegments != ctx->N)
         return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_multi: set_multi_tables was given another numSegments");
-    return run_generated_impl(ctx, numPaths, pathOffset, seed, true, minDs, maxDs, out, minLog10W, perPathLog10,
-          perJointCurvature);
+    return run_generated_impl(ctx, numPaths, pathOffset, seed, kGenMulti, minDs, maxDs, ChainArgs {}, out, minLog10W,
+          perPathLog10, perJointCurvature);
+}
+
+int twisty_hip_run_generated_chains(twisty_hip_ctx *ctx, uint64_t numChains, uint64_t chainLength,
+      uint64_t chainOffset, uint64_t seed, float minDs, float maxDs, float extinction, double pathNormalizerLog10,
+      twisty_hip_partials *out, double *minLog10W, float *perPathLog10, float *chainLog10, float *perJointCurvature)
+{
+    if (!ctx) return TWISTY_HIP_ERR_ARG;
+    if (ctx->multiCount && ctx->haveCurve && ctx->multiSegments != ctx->N)
+        return fail(ctx, TWISTY_HIP_ERR_ARG, "run_generated_chains: set_multi_tables was given another numSegments");
+    ChainArgs ch;
+    ch.numChains = numChains;
+    ch.chainLength = chainLength;
+    ch.extinction = extinction;
+    ch.normLog10 = pathNormalizerLog10;
+    ch.chainLog10 = chainLog10;
+    const uint64_t n = (numChains && chainLength && numChains <= (1ull << 40) / chainLength) ? numChains * chainLength
+                                                                                               : 0;
+    return run_generated_impl(ctx, n ? n : 1, chainOffset, seed, kGenChains, minDs, maxDs, ch, out, minLog10W,
+          perPathLog10, perJointCurvature);
 }
 
 int twisty_hip_set_burn_in(twisty_hip_ctx *ctx, uint32_t pathsPerChain)

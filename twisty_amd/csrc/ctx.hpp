@@ -86,6 +86,9 @@ struct twisty_hip_ctx {
     float *dMultiMeta = nullptr;
     uint64_t multiTablesCap = 0, multiMetaCap = 0;
     uint32_t multiBins = 0, multiCount = 0, multiSegments = 0;
+    float multiAbsorption = 0.0f;   // the absorption coefficient the multi-ds meta was formed with
+    float *dChainLog = nullptr;     // chains form: the chain's log10 weight after each proposal
+    uint64_t chainLogCap = 0;
     // Adjoint roulette of the exit-direction sweep: 0 = every path; the
     // default 1 keeps about a tenth of the paths (the table gradient always
     // takes every path).

@@ -205,9 +205,35 @@ __device__ __forceinline__ uint32_t nearest_table(const GenParams &p, float ds)
     return best;
 }
 
-template<int G, int BLOCK, bool MULTI>
+// Build-defined natural logarithm of y in (0, 1] in FP64 from basic operations,
+// the same steps as the oracle's, so both form the same bits: y = m 2^e with m
+// in [sqrt(1/2), sqrt(2)), ln m = 2 s (1 + z/3 + ... + z^11/23), s = (m-1)/(m+1),
+// z = s^2.
+__device__ __forceinline__ double bd_log(double y)
+{
+    int e = 0;
+    double m = frexp(y, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = __ddiv_rn(m - 1.0, m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+#pragma unroll
+    for (int k = 10; k >= 0; --k) p = fma(p, z, 1.0 / (double)(2 * k + 1));
+    return fma((double)e, 0.69314718055994530942, (2.0 * s) * p);
+}
+
+// MODE: kGenSingle one table and ds; kGenMulti a ds per path, drawn uniform,
+// and the nearest cached table; kGenChains the two-table overload's
+// Metropolis-Hastings chains (ExperimentBase.cpp:1599-1818): per proposal a
+// truncated-exponential ds and the nearest cached table, the sampling weight,
+// and the acceptance test against the chain's current weight.
+template<int G, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
 {
+    constexpr bool MULTI = MODE != kGenSingle;
     constexpr uint32_t SLOT = 3u * BLOCK;
     constexpr uint32_t groupsPerBlock = BLOCK / G;
     constexpr int SMAX = 16;  // segment rows per lane: ceil(N / G) <= ceil((N + 1) / G) <= 16
@@ -245,13 +271,22 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
     uint32_t clamped = 0;
     float minL = INFINITY;
     const uint32_t n0 = N - 2;
-    for (uint64_t local = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; local < p.numPaths;
-          local += (uint64_t)gridDim.x * groupsPerBlock) {
-        const uint64_t path = p.pathOffset + local;
+    // Builds and weighs path `path` (output slot `local`); every lane of the
+    // group returns the group's totals.
+    auto one_path = [&](uint64_t local, uint64_t path) -> PathLog {
         if constexpr (MULTI) {
-            // ds = U(0,1) (maxDs - minDs) + minDs (ExperimentBase.cpp:1486-1487) from node 0's
-            // words, the nearest cached table, curvature with arclength ds N (:1512-1518)
-            ds = fmaf(gen_words(p.seed, path, 0, 0).u0, p.dsHi - p.dsLo, p.dsLo);
+            const Words w0 = gen_words(p.seed, path, 0, 0);
+            if constexpr (MODE == kGenChains) {
+                // ds = minDs + x, x exponential of rate t truncated to [0, maxDs], by the
+                // inverse distribution function (the reference rejects draws past maxDs +
+                // minDs, ExperimentBase.cpp:1662-1667: the same distribution)
+                const double y = fma(-(double)w0.u0, p.oneMinusQ, 1.0);
+                ds = (float)((double)p.dsLo + __ddiv_rn(-bd_log(y), (double)p.extinction));
+            } else {
+                // ds = U(0,1) (maxDs - minDs) + minDs (ExperimentBase.cpp:1486-1487)
+                ds = fmaf(w0.u0, p.dsHi - p.dsLo, p.dsLo);
+            }
+            // the nearest cached table, curvature with arclength ds N (:1512-1518)
             const uint32_t t = nearest_table(p, ds);
             const float *m = p.multiMeta + 5 * t;
             tv = make_table_view(p.multiTables + (uint64_t)t * p.tableStride, p.tableSize, m[1], m[2], m[3]);
@@ -319,27 +354,88 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
                   [&](uint32_t i, const JointEval &je) { p.perJointCurv[local * (uint64_t)(N - 1) + i] = je.kappa; });
         }
         if constexpr (MULTI) pl.lsum = ff_add_f(pl.lsum, absorbLog2);  // the table's own absorption
+        if constexpr (MODE == kGenChains) {
+            // log2 of the sampling weight 1 / (t exp(-t (ds - minDs) N)), formed in
+            // float as ExperimentBase.cpp:1669-1670 forms it
+            const float t = p.extinction;
+            const float sw = 1.0f / (t * expf(-t * (ds - p.dsLo) * (float)N));
+            const double l2 = log2((double)sw);
+            const float hi = (float)l2;
+            pl.lsum = ff_add_f(ff_add_f(pl.lsum, hi), (float)(l2 - (double)hi));
+        }
         wave_sync_lds();  // joints read other lanes' columns before the next path's points
-        if (j == 0) {
-            const bool invalid = (pl.flags & 1u) || ((pl.flags & 6u) == 6u);
-            float outv;
-            if (invalid) {
-                ++acc.nInvalid;
-                outv = __int_as_float(0x7fc00000);
-            } else {
-                ++acc.nValid;
-                if (pl.flags & 2u) {
-                    outv = -INFINITY;
-                } else if (pl.flags & 4u) {
-                    outv = INFINITY;
-                    lse_add(acc, INFINITY, 0.0f);
-                } else {
-                    lse_add(acc, pl.lsum.hi, pl.lsum.lo);
-                    outv = fmaf(pl.lsum.hi, p.log10of2, fmaf(pl.lsum.lo, p.log10of2, p.absorbLog10PerPath));
-                }
-                minL = fminf(minL, outv);
+        return pl;
+    };
+    // A path's log10 weight as it is reported (NaN when invalid).
+    auto log10_of = [&](const PathLog &pl, bool &invalid) -> float {
+        invalid = (pl.flags & 1u) || ((pl.flags & 6u) == 6u);
+        if (invalid) return __int_as_float(0x7fc00000);
+        if (pl.flags & 2u) return -INFINITY;
+        if (pl.flags & 4u) return INFINITY;
+        return fmaf(pl.lsum.hi, p.log10of2, fmaf(pl.lsum.lo, p.log10of2, p.absorbLog10PerPath));
+    };
+    // Adds a valid path (or the chain state) of log10 weight v and log2 sum ls.
+    auto add_valid = [&](float v, FF ls) {
+        ++acc.nValid;
+        if (v == INFINITY) lse_add(acc, INFINITY, 0.0f);
+        else if (v != -INFINITY) lse_add(acc, ls.hi, ls.lo);
+        minL = fminf(minL, v);
+    };
+
+    if constexpr (MODE != kGenChains) {
+        for (uint64_t local = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; local < p.numPaths;
+              local += (uint64_t)gridDim.x * groupsPerBlock) {
+            const PathLog pl = one_path(local, p.pathOffset + local);
+            if (j == 0) {
+                bool invalid;
+                const float outv = log10_of(pl, invalid);
+                if (invalid) ++acc.nInvalid;
+                else add_valid(outv, pl.lsum);
+                if (p.perPathLog10) p.perPathLog10[local] = outv;
             }
-            if (p.perPathLog10) p.perPathLog10[local] = outv;
+        }
+    } else {
+        // One chain per group at a time, its proposals in order.  Every lane of
+        // the group holds the same totals, so the acceptance test is the same in
+        // all of them.
+        for (uint64_t chain = (uint64_t)blockIdx.x * groupsPerBlock + lane / G; chain < p.numChains;
+              chain += (uint64_t)gridDim.x * groupsPerBlock) {
+            bool first = true;
+            float curV = __int_as_float(0x7fc00000);  // the chain's current log10 weight
+            FF curS { 0.0f, 0.0f };
+            double curN = 0.0;                        // and with the normaliser, in FP64
+            for (uint64_t st = 0; st < p.chainLength; ++st) {
+                const uint64_t local = chain * p.chainLength + st;
+                const uint64_t path = (p.pathOffset + chain) * p.chainLength + st;
+                const PathLog pl = one_path(local, path);
+                bool invalid;
+                const float outv = log10_of(pl, invalid);
+                if (invalid) {
+                    // skipped: the chain keeps its state (ExperimentBase.cpp:1720-1723)
+                    if (j == 0) ++acc.nInvalid;
+                } else {
+                    // the FP64 weight with the normaliser; pow(10, w) as the reference forms it
+                    const double w = (outv == INFINITY || outv == -INFINITY)
+                          ? (double)outv
+                          : ((double)pl.lsum.hi + (double)pl.lsum.lo) * 0.30102999566398119521 + p.normLog10;
+                    bool take = first;
+                    if (!first) {
+                        const double ratio = exp10(w) / exp10(curN);
+                        take = (double)gen_words(p.seed, path, 0, 0).u1 <= ratio;  // :1738-1755
+                    }
+                    if (take) {
+                        curV = outv;
+                        curS = pl.lsum;
+                        curN = w;
+                    }
+                    first = false;
+                    if (j == 0) add_valid(curV, curS);  // :1757-1784
+                }
+                if (j == 0) {
+                    if (p.perPathLog10) p.perPathLog10[local] = outv;
+                    if (p.chainLog10) p.chainLog10[local] = curV;
+                }
+            }
         }
     }
     acc.nClamped = clamped;
@@ -351,7 +447,7 @@ __global__ void __launch_bounds__(BLOCK) generate_kernel(const GenParams p)
 
 #define TWISTY_GEN_INSTANCES(X) X(2, 256) X(4, 256) X(8, 256) X(16, 256) X(32, 256) X(64, 256)
 
-template<int G, int B, bool M>
+template<int G, int B, int M>
 static hipError_t launch_gen_gb(const GenParams &p, uint32_t grid, uint32_t lds, hipStream_t s)
 {
     auto *fn = generate_kernel<G, B, M>;
@@ -361,12 +457,15 @@ static hipError_t launch_gen_gb(const GenParams &p, uint32_t grid, uint32_t lds,
     return hipGetLastError();
 }
 
-hipError_t launch_generate(const GenParams &p, uint32_t G, bool multi, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_generate(const GenParams &p, uint32_t G, int mode, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s)
 {
 #define X(g, b)                                                                                                   \
-    if (G == g && block == b) return multi ? launch_gen_gb<g, b, true>(p, grid, lds, s)                          \
-                                           : launch_gen_gb<g, b, false>(p, grid, lds, s);
+    if (G == g && block == b) {                                                                                   \
+        if (mode == kGenChains) return launch_gen_gb<g, b, kGenChains>(p, grid, lds, s);                          \
+        return mode == kGenMulti ? launch_gen_gb<g, b, kGenMulti>(p, grid, lds, s)                                \
+                                 : launch_gen_gb<g, b, kGenSingle>(p, grid, lds, s);                              \
+    }
     TWISTY_GEN_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;

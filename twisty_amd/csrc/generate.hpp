@@ -37,7 +37,20 @@ struct GenParams {
     const float *multiMeta;    // per table: ds, minK, maxK, step, absorption log2 per path
     uint32_t numTables;
     float dsLo, dsHi;
+    // Chains form (ExperimentBase.cpp:1599-1818): numChains chains of
+    // chainLength proposals, the first chain pathOffset; ds = dsLo + x with x
+    // exponential of rate extinction truncated to [0, maxDs] (oneMinusQ = 1 -
+    // exp(-extinction maxDs)); normLog10 enters the acceptance test.
+    uint64_t numChains;
+    uint64_t chainLength;
+    double oneMinusQ;
+    double normLog10;
+    float extinction;
+    float *chainLog10;         // nullable: the chain's log10 weight after each proposal
 };
+
+// Forms of the independent-path estimator.
+enum GenMode : int { kGenSingle = 0, kGenMulti = 1, kGenChains = 2 };
 
 // Levels of the recursion (N - 2 halved until 2 or 3), -1 when N is not
 // 2 + 2^k * {2, 3} with k >= 1.
@@ -73,7 +86,7 @@ inline uint32_t generate_lds_bytes(uint32_t G, uint32_t block, uint32_t N, uint3
     return (uint32_t)(bytes > reduceBytes ? bytes : reduceBytes);
 }
 
-hipError_t launch_generate(const GenParams &p, uint32_t G, bool multi, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launch_generate(const GenParams &p, uint32_t G, int mode, uint32_t grid, uint32_t block, uint32_t lds,
       hipStream_t s);
 
 }  // namespace twisty_hip

@@ -205,11 +205,34 @@ def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22,
     for _ in range(reps):
         r = ctx.run_generated(paths, seed=1)
         ms.append(ctx.last_timing()[0])
+    # the two-table Metropolis-Hastings form (ExperimentBase.cpp:1599-1818) on
+    # the same boundary: 2^16 chains x 64 proposals, ds from the minimum
+    # arclength up by half the nominal ds, 8 cached tables over that range
+    lo = np.float32(host.minimum_arclength(bc, N) / N)
+    hi = np.float32(0.5 * ds)
+    tabs, _, _ = host.cached_rt_tables(float(lo), float(lo + hi), 8, 1.0, float(np.float32(0.05) ** 2), 0.1, 0.004,
+                                       numStepsInt=20000, numCurvatureSteps=4096)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    ctx.set_multi_tables(np.stack([t.table for t in tabs]),
+                         np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32), ext, N)
+    chains, length = 1 << 16, 64
+    ctx.run_generated_chains(chains, length, float(lo), float(hi), ext, seed=1)
+    cms = []
+    for _ in range(reps):
+        rc = ctx.run_generated_chains(chains, length, float(lo), float(hi), ext, seed=1)
+        cms.append(ctx.last_timing()[0])
     ctx.close()
+    ccms = statistics.fmean(cms)
     kms = statistics.fmean(ms)
     out = {"workload": f"independent paths: N={N}, arclength {L}, 2^22 paths, 4096-bin RT table sigma=0.05",
            "kernel": "generate_kernel", "kernel_ms": kms, "joint_weights_per_s": paths * (N - 1) / (kms / 1e3),
-           "num_valid": r["num_valid"], "log10_sum_w": r["log10_sum_w"], "cpu_baseline": None}
+           "num_valid": r["num_valid"], "log10_sum_w": r["log10_sum_w"], "cpu_baseline": None,
+           "chains": {"workload": f"two-table Metropolis-Hastings form: {chains} chains x {length} proposals, ds in "
+                                  f"[{float(lo):.5f}, {float(lo + hi):.5f}], 8 cached tables, absorption + scatter "
+                                  f"{ext:.4f}", "kernel_ms": ccms,
+                      "proposals_per_s": chains * length / (ccms / 1e3),
+                      "joint_weights_per_s": chains * length * (N - 1) / (ccms / 1e3),
+                      "num_valid": rc["num_valid"], "log10_sum_w": rc["log10_sum_w"]}}
     if cpu_seconds:
         from oracle import pyoracle
 
@@ -421,7 +444,8 @@ def main(argv=None):
         progress(f"C5: {out['c5']['forward_plus_histogram_ms']:.2f} ms")
     if not a.no_generated:
         out["generated"] = measure_generated(capi, host, None if a.no_cpu_baseline else a.cpu_seconds / 4)
-        progress(f"independent paths: {out['generated']['kernel_ms']:.3f} ms")
+        progress(f"independent paths: {out['generated']['kernel_ms']:.3f} ms; chains "
+                 f"{out['generated']['chains']['kernel_ms']:.3f} ms")
     if not a.no_cpu_baseline:
         cb, refpart = cpu_baseline(table, pos, w, a.cpu_seconds)
         progress(f"cpu baseline: {cb['value']:.3e} joint-weights/s on {cb['cores']} threads")

@@ -168,6 +168,18 @@ int twisty_host_path_generation_mc_multi(const char *jsonText, uint64_t seed, do
       float minDsTables, float maxDsTables, uint32_t numDsSteps, float maxDs, const char *cacheRoot,
       twisty_host_generation_result *out, char *errorOut, uint32_t errorLen);
 
+/* The two-table overload (Experiments/ExperimentBase.cpp:1599-1818), the
+ * NoisyCircle experiments' estimator: Metropolis-Hastings chains of
+ * chainLength proposals (ceil(paths / chainLength) chains; num_total = chains
+ * x chainLength), ds on [minDs, minDs + maxDs] with minDs the boundary's
+ * CalculateMinimumArclength over N, the cached environment tables as in
+ * twisty_host_path_generation_mc_multi with absorption + scatter on every joint
+ * (the reference's object tables are not read by its weighter); num_total = 0
+ * when maxDs < minDs. */
+int twisty_host_path_generation_mc_chains(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
+      float minDsTables, float maxDsTables, uint32_t numDsSteps, float maxDs, uint64_t chainLength,
+      const char *cacheRoot, twisty_host_generation_result *out, char *errorOut, uint32_t errorLen);
+
 /* The FullExperiment flow (Experiments/FullExperiment.cpp:130-266) on a JSON
  * config given as text: parse experiment.experimentParams and
  * experiment.basicExperiment, run FullExperimentRunnerOptimalPerturb_HIP.

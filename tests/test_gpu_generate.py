@@ -247,6 +247,45 @@ def test_host_path_generation_mc_multi(hip_device, tmp_path):
     assert abs(r["log10_total_weight"] - want) < 1e-5 * abs(want)
 
 
+def test_host_path_generation_mc_chains(hip_device, tmp_path):
+    """The C++ two-table overload on a config equals the device call it makes:
+    ceil(paths / chainLength) chains, minDs the boundary's minimum arclength
+    over N, the cached tables with absorption + scatter."""
+    N, L, paths, length = 34, 2.0, 20000, 64
+    cfg = {"experiment": {
+        "experimentParams": {
+            "pathsToGenerate": paths, "pathsToSkip": 0, "numSegments": N, "name": "gen", "experimentDir": "/tmp",
+            "maxPerturbThreads": 0, "maxWeightThreads": 0, "outputBigFloatWeights": False,
+            "outputPathBatches": False, "perturbMethod": 0, "useGpu": True,
+            "random": {"bootstrapSeed": 1, "perturbSeed": 1},
+            "weighting": {"weightFunction": 0, "mu": 1.0, "eps": float(np.float32(0.08) ** 2), "numStepsInt": 2000,
+                          "numCurvatureSteps": 1024, "absorption": 0.004, "scatter": 0.1}},
+        "basicExperiment": {"startPos": [0, 0, 0], "startDir": [0, 0, 1], "endPos": [0, 0, 1], "endDir": [0, 0, 1],
+                            "arclength": L}}}
+    # the reference returns nothing unless maxDs >= minDs (ExperimentBase.cpp:1639-1642), although its
+    # draws then span [minDs, minDs + maxDs]
+    max_ds = 0.03
+    r = host.path_generation_mc_chains(cfg, 0.025, 0.06, 11, max_ds, chain_length=length, seed=3, norm_log10=1.5,
+                                       cache_root=tmp_path)
+    tabs, loaded, _ = host.cached_rt_tables(0.025, 0.06, 11, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
+                                            numStepsInt=2000, numCurvatureSteps=1024, cache_root=tmp_path)
+    assert loaded
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), L)
+    min_ds = float(np.float32(host.minimum_arclength(bc13, N)) / np.float32(N))
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    chains = (paths + length - 1) // length
+    ctx = capi.Context(0)
+    ctx.set_curve(np.stack([np.zeros(N + 1), np.zeros(N + 1), np.linspace(0, 1, N + 1)], 1).astype(np.float32), bc13)
+    ctx.set_multi_tables(np.stack([t.table for t in tabs]), np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs],
+                                                                     np.float32), ext, N)
+    g = ctx.run_generated_chains(chains, length, min_ds, max_ds, ext, norm_log10=1.5, seed=3)
+    ctx.close()
+    want = g["log10_sum_w"] + 1.5 - math.log10(g["num_valid"])
+    print(f"[path_generation_mc_chains] {r}", flush=True)
+    assert r["num_total"] == chains * length and r["num_valid"] == g["num_valid"]
+    assert abs(r["log10_total_weight"] - want) < 1e-9 * abs(want)
+
+
 def test_generated_chains_match_oracle(hip_device, oracle):
     """The two-table Metropolis-Hastings form (ExperimentBase.cpp:1599-1818)
     against the oracle on the same draws: every proposal by the per-path rule

@@ -410,6 +410,21 @@ def test_path_generation_mc_multi_minimum_ds(tmp_path):
             host.path_generation_mc_multi(cfg, 0.03, 0.07, 5, D / N * 1.001, cache_root=tmp_path)
 
 
+def test_path_generation_mc_chains_minimum_ds(tmp_path):
+    """Two-table overload: its minimum ds is CalculateMinimumArclength / N (the
+    end distance for a straight geometry), so maxDs below it gives the
+    reference's empty result (ExperimentBase.cpp:1639-1642) and above it
+    reaches the device."""
+    N, D = 26, 1.3
+    cfg = _config(numSegments=N, pathsToGenerate=1000)
+    cfg["experiment"]["basicExperiment"]["endPos"] = [0, 0, D]
+    empty = host.path_generation_mc_chains(cfg, 0.03, 0.12, 5, D / N * 0.999, cache_root=tmp_path)
+    assert empty["num_total"] == 0 and empty["num_valid"] == 0
+    if capi.device_count() == 0:
+        with pytest.raises(RuntimeError, match="twisty_hip_create"):
+            host.path_generation_mc_chains(cfg, 0.03, 0.12, 5, D / N * 1.001, cache_root=tmp_path)
+
+
 def test_experiment_gradients_errors():
     with pytest.raises(RuntimeError, match="dTableLen"):
         host.experiment_gradients(_config(), table_bins=7)

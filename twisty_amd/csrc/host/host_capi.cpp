@@ -325,6 +325,38 @@ int twisty_host_path_generation_mc_multi(const char *jsonText, uint64_t seed, do
     return 0;
 }
 
+int twisty_host_path_generation_mc_chains(const char *jsonText, uint64_t seed, double pathNormalizerLog10,
+      float minDsTables, float maxDsTables, uint32_t numDsSteps, float maxDs, uint64_t chainLength,
+      const char *cacheRoot, twisty_host_generation_result *out, char *errorOut, uint32_t errorLen)
+{
+    if (!jsonText || !out) return 1;
+    try {
+        const Json cfg = Json::Parse(jsonText);
+        const ExperimentRunner::ExperimentParameters ep = ExperimentRunner::ParseExperimentParamsFromConfig(cfg);
+        const PerturbUtils::BoundaryConditions bc = ParseBasicExperiment(cfg);
+        const PathWeighting::CachedMultiArclengthWeightLookupTable tables(ep.weightingParameters, minDsTables,
+              maxDsTables, numDsSteps, cacheRoot ? cacheRoot : "", ep.maxWeightThreads > 0 ? ep.maxWeightThreads : 0);
+        std::string err;
+        // the object set is not read (the reference's weighter has its object branch commented out)
+        const auto r = ExperimentBase::MSegmentPathGenerationMC(seed, ep.numPathsInExperiment, bc, ep,
+              pathNormalizerLog10, tables, tables, maxDs, err, chainLength);
+        if (!r) {
+            put_error(errorOut, errorLen, err);
+            return 1;
+        }
+        out->log10_total_weight = r->totalWeight.log10;
+        out->min_log10_w = r->minPathWeightLog10;
+        out->max_log10_w = r->maxPathWeightLog10;
+        out->num_valid = r->numValidPaths;
+        out->num_total = r->numPathsTotal;
+        out->kernel_ms = r->kernelMs;
+    } catch (const std::exception &e) {
+        put_error(errorOut, errorLen, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int twisty_host_experiment_gradients(const char *jsonText, twisty_host_gradients *out, double *dTableOut,
       uint32_t dTableLen, char *errorOut, uint32_t errorLen)
 {

@@ -156,5 +156,72 @@ std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperi
     return r;
 }
 
+std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperimentPaths,
+      const PerturbUtils::BoundaryConditions &bc, const ExperimentRunner::ExperimentParameters &ep,
+      double pathNormalizerLog10, const PathWeighting::CachedMultiArclengthWeightLookupTable &envTables,
+      const PathWeighting::CachedMultiArclengthWeightLookupTable & /*objectTables*/, float maxDs, std::string &error,
+      uint64_t chainLength)
+{
+    error.clear();
+    const uint32_t N = ep.numSegmentsPerCurve;
+    if (N < 4 || numExperimentPaths == 0 || envTables.Count() == 0 || chainLength == 0) {
+        error = "need numSegments >= 4, at least one path, one cached table and chainLength >= 1";
+        return std::nullopt;
+    }
+    const float minDs = PathGeneration::CalculateMinimumArclength(bc, N) / (float)N;
+    if (maxDs < minDs) return Result {};  // "No solution for current environment"
+    const size_t T = envTables.Count();
+    const uint32_t bins = (uint32_t)envTables.Table(0).AccessLookupTable().size();
+    std::vector<float> data, meta;
+    data.reserve(T * bins);
+    for (size_t t = 0; t < T; ++t) {
+        const auto &tb = envTables.Table(t);
+        const auto &v = tb.AccessLookupTable();
+        data.insert(data.end(), v.begin(), v.end());
+        meta.insert(meta.end(), { tb.GetDs(), tb.GetMinCurvature(), tb.GetMaxCurvature(), tb.GetCurvatureStepSize() });
+    }
+    std::vector<Vec3> line(N + 1);
+    for (uint32_t i = 0; i <= N; ++i) line[i] = bc.m_startPos + (bc.m_endPos - bc.m_startPos) * ((float)i / (float)N);
+    PerturbUtils::BoundaryConditions b = bc;
+    b.arclength = (minDs + maxDs) * (float)N;
+    twisty_hip_ctx *ctx = nullptr;
+    if (twisty_hip_create(ep.device, &ctx) != TWISTY_HIP_OK) {
+        error = "twisty_hip_create failed (device " + std::to_string(ep.device) + ")";
+        return std::nullopt;
+    }
+    struct Guard {
+        twisty_hip_ctx *c;
+        ~Guard() { twisty_hip_destroy(c); }
+    } guard { ctx };
+    // the rate of the ds draw and the loss of every joint (:1663-1664, PathWeighters.cpp:145)
+    const float extinction = ep.weightingParameters.absorption + ep.weightingParameters.scatter;
+    const uint64_t numChains = (numExperimentPaths + chainLength - 1) / chainLength;
+    const auto bc13 = PerturbUtils::ToCudaSafe(b);
+    int rc = twisty_hip_set_curve(ctx, reinterpret_cast<const float *>(line.data()), N, bc13.data());
+    if (rc == TWISTY_HIP_OK)
+        rc = twisty_hip_set_multi_tables(ctx, data.data(), bins, (uint32_t)T, meta.data(), extinction, N);
+    twisty_hip_partials part {};
+    double mn = INFINITY;
+    if (rc == TWISTY_HIP_OK)
+        rc = twisty_hip_run_generated_chains(ctx, numChains, chainLength, 0, seed, minDs, maxDs, extinction,
+              pathNormalizerLog10, &part, &mn, nullptr, nullptr, nullptr);
+    if (rc != TWISTY_HIP_OK) {
+        error = twisty_hip_last_error(ctx);
+        return std::nullopt;
+    }
+    float fwdMs = 0.0f, totMs = 0.0f;
+    twisty_hip_last_timing(ctx, &fwdMs, &totMs);
+    Result r;
+    r.numValidPaths = part.num_valid;
+    r.numPathsTotal = numChains * chainLength;
+    r.totalWeight.log10 = part.num_valid
+          ? part.log10_sum_w + pathNormalizerLog10 - std::log10((double)part.num_valid)
+          : -INFINITY;
+    r.minPathWeightLog10 = mn + pathNormalizerLog10;
+    r.maxPathWeightLog10 = part.max_log10_w + pathNormalizerLog10;
+    r.kernelMs = totMs;
+    return r;
+}
+
 }  // namespace ExperimentBase
 }  // namespace twisty

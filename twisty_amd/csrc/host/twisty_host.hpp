@@ -413,6 +413,23 @@ std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperi
       const ExperimentRunner::ExperimentParameters &experimentParams, double pathNormalizerLog10,
       const PathWeighting::CachedMultiArclengthWeightLookupTable &cachedWeightLookupTable, float maxDs,
       std::string &error);
+// The two-table overload (ExperimentBase.cpp:1599-1818), the estimator of the
+// NoisyCircle experiments, on twisty_hip_run_generated_chains: Metropolis-
+// Hastings chains of chainLength proposals (the reference runs one per OpenMP
+// thread; here ceil(numExperimentPaths / chainLength) of them, so numPathsTotal
+// is that many chains times chainLength), minDs = CalculateMinimumArclength / N
+// (an empty Result when maxDs < minDs, :1639-1642), ds on [minDs, minDs +
+// maxDs], every joint weighted with the environment table nearest its ds and
+// absorption + scatter.  The object tables are taken and not read: the
+// reference's weighter has its object branch commented out
+// (PathWeighters.cpp:137-141).  totalWeight is the sum over valid proposals of
+// 10^(chain weight + normaliser) / numValidPaths.
+std::optional<Result> MSegmentPathGenerationMC(uint64_t seed, uint64_t numExperimentPaths,
+      const PerturbUtils::BoundaryConditions &experimentGeometry,
+      const ExperimentRunner::ExperimentParameters &experimentParams, double pathNormalizerLog10,
+      const PathWeighting::CachedMultiArclengthWeightLookupTable &environmentCachedWeightLookupTable,
+      const PathWeighting::CachedMultiArclengthWeightLookupTable &objectCachedWeightLookupTable, float maxDs,
+      std::string &error, uint64_t chainLength = 64);
 // ExperimentBase.cpp:1437-1462: the smaller positive root of the quadratic in
 // the arclength for M segments (1e6 when neither root is positive).
 float QuadraticMinimumArclength(const PerturbUtils::BoundaryConditions &experimentGeometry, uint32_t M);

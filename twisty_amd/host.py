@@ -13,7 +13,8 @@ EXPORTED = ["twisty_host_build_rt_table", "twisty_host_build_rt_tables", "twisty
             "twisty_host_norm_log10", "twisty_host_norm_log10_steps", "twisty_host_run_config", "twisty_host_rt_table_eps_derivative",
             "twisty_host_initial_arc_exit_jacobian", "twisty_host_cached_rt_tables",
             "twisty_host_write_path_batches", "twisty_host_run_experiments", "twisty_host_path_generation_mc",
-            "twisty_host_path_generation_mc_multi", "twisty_host_experiment_gradients"]
+            "twisty_host_path_generation_mc_multi", "twisty_host_path_generation_mc_chains",
+            "twisty_host_experiment_gradients"]
 
 _lib = None
 
@@ -89,6 +90,9 @@ def lib():
                                                      C.c_char_p, u32]
         L.twisty_host_path_generation_mc_multi.argtypes = [C.c_char_p, C.c_uint64, f64, f32, f32, u32, f32,
                                                            C.c_char_p, C.POINTER(GenerationResult), C.c_char_p, u32]
+        L.twisty_host_path_generation_mc_chains.argtypes = [C.c_char_p, C.c_uint64, f64, f32, f32, u32, f32,
+                                                            C.c_uint64, C.c_char_p, C.POINTER(GenerationResult),
+                                                            C.c_char_p, u32]
         L.twisty_host_experiment_gradients.argtypes = [C.c_char_p, C.POINTER(Gradients), vp, u32, C.c_char_p, u32]
         L.twisty_host_write_path_batches.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, vp, u32, vp, vp,
                                                      C.c_uint64, f64, C.c_char_p, u32]
@@ -290,6 +294,21 @@ def path_generation_mc_multi(config, min_ds_tables, max_ds_tables, num_ds_steps,
     if lib().twisty_host_path_generation_mc_multi(text.encode(), seed, norm_log10, min_ds_tables, max_ds_tables,
                                                   num_ds_steps, max_ds, root, C.byref(res), err, 1024) != 0:
         raise RuntimeError(f"path_generation_mc_multi failed: {err.value.decode()}")
+    return {k: getattr(res, k) for k, _ in res._fields_}
+
+
+def path_generation_mc_chains(config, min_ds_tables, max_ds_tables, num_ds_steps, max_ds, chain_length=64, seed=1,
+                              norm_log10=0.0, cache_root=None):
+    """The two-table overload (the NoisyCircle experiments' estimator):
+    Metropolis-Hastings chains over the cached tables with absorption + scatter."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    res = GenerationResult()
+    err = C.create_string_buffer(1024)
+    root = None if cache_root is None else str(cache_root).encode()
+    if lib().twisty_host_path_generation_mc_chains(text.encode(), seed, norm_log10, min_ds_tables, max_ds_tables,
+                                                   num_ds_steps, max_ds, chain_length, root, C.byref(res), err,
+                                                   1024) != 0:
+        raise RuntimeError(f"path_generation_mc_chains failed: {err.value.decode()}")
     return {k: getattr(res, k) for k, _ in res._fields_}
 
 

@@ -287,7 +287,7 @@ def measure_c5(capi, host, reps=2):
             "histogram_sum": float(h.sum()), "schedule": sched}
 
 
-def measure_batched(ctx, capi, n_exp=4096, autotune=True):
+def measure_batched(ctx, capi, n_exp=4096, autotune=True, schedule=None):
     """The batched workload in one graph-replayed launch vs one plain run per experiment."""
     import numpy as np
 
@@ -300,7 +300,9 @@ def measure_batched(ctx, capi, n_exp=4096, autotune=True):
     jw = sum(x["paths"] * (x["N"] - 1) for x in exps)
     b = capi.Batch(ctx, exps, wl["tables"], wl["meta"])
     tuned, tried = None, []
-    if autotune:
+    if schedule:
+        b.set_schedule(*schedule)
+    elif autotune:
         tuned, tried = b.autotune(reps=1)
     b.run()
     ms = min(b.run()[2] for _ in range(3))
@@ -360,6 +362,9 @@ def main(argv=None):
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched leg: run the library's default schedule instead of the autotuner's")
+    ap.add_argument("--batched-schedule", default=None,
+                    help="batched leg: run this schedule (block,blocks_per_cu,segments_per_lane,tile,lean) "
+                         "instead of the autotuner's")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -436,7 +441,8 @@ def main(argv=None):
         progress(f"C4: forward {out['c4']['forward_ms']:.3f} ms, exact full gradient "
                  f"{out['c4']['full']['fwd_adjoint_ms']:.3f} ms")
     if not a.no_batched:
-        out["batched"] = measure_batched(ctx, capi, autotune=not a.no_autotune)
+        sched = tuple(int(v) for v in a.batched_schedule.split(",")) if a.batched_schedule else None
+        out["batched"] = measure_batched(ctx, capi, autotune=not a.no_autotune, schedule=sched)
         progress(f"batched: {out['batched']['batched_ms']:.3f} ms")
         out["batched"]["c2_one_experiment"] = measure_batched_c2(ctx, capi, w, table, pos, bc)
     if not a.no_c5:

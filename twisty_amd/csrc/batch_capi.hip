@@ -436,17 +436,22 @@ int twisty_hip_batch_autotune(twisty_hip_batch *b, uint32_t reps, twisty_hip_bat
     twisty_hip_ctx *ctx = b->ctx;
     if (reps == 0) reps = 1;
     // Candidates: block size x lanes policy x blocks per CU x tile size with
-    // the tables in LDS, and the two other placements at block 256 with as
-    // many blocks per CU as fit.  A candidate whose LDS does not fit is skipped.
+    // the tables in LDS; the tables in global memory at block 256 with as many
+    // blocks per CU as fit, where 15 segments per lane keep every state within
+    // 16 rows, so three blocks share a CU (the C3 batch: 127 ms against 149 ms
+    // for the best table-in-LDS schedule), and the lean form.  A candidate
+    // whose LDS does not fit is skipped.
     std::vector<twisty_hip_batch_schedule> cand;
     for (uint32_t block : { 256u, 512u })
         for (uint32_t spl : { 8u, 16u })
             for (uint32_t bpc : { 1u, 2u })
                 for (uint32_t tile : { 1024u, 4096u, 16384u })
                     cand.push_back({ block, bpc, spl, tile, 0u });
-    for (uint32_t lean : { 1u, 2u })
-        for (uint32_t spl : { 8u, 16u })
-            cand.push_back({ 256u, 0u, spl, 4096u, lean });
+    for (uint32_t spl : { 8u, 12u, 15u, 16u })
+        for (uint32_t tile : { 1024u, 4096u, 16384u })
+            cand.push_back({ 256u, 0u, spl, tile, 2u });
+    for (uint32_t spl : { 8u, 16u })
+        cand.push_back({ 256u, 0u, spl, 4096u, 1u });
     twisty_hip_batch_schedule bestS = b->sched;
     float bestMs = INFINITY;
     uint32_t n = 0;

@@ -372,3 +372,38 @@ def test_generated_chains_match_oracle(hip_device, oracle):
           f"leaving the oracle's {diverged} of {C}", flush=True)
     assert diverged <= C // 10
     ctx.close()
+
+
+def test_generated_chains_argument_errors(hip_device):
+    """The chains entry point refuses, before any launch: an extinction other
+    than the absorption coefficient the cached tables were given, bad ds bounds,
+    an empty or oversized chain set, and a run before the tables."""
+    N = 34
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 2.0)
+    tabs, _, _ = host.cached_rt_tables(0.025, 0.06, 4, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
+                                       numStepsInt=500, numCurvatureSteps=256)
+    tables = np.stack([t.table for t in tabs])
+    meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    ctx = capi.Context(0)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    with pytest.raises(capi.TwistyHipError) as e:
+        ctx.run_generated_chains(8, 8, 0.03, 0.02, ext)
+    assert e.value.code == capi.ERR_STATE  # no cached tables yet
+    ctx.set_multi_tables(tables, meta, 0.004, N)
+    for args, code in [((8, 8, 0.03, 0.02, ext), capi.ERR_ARG),        # tables given another coefficient
+                       ((8, 8, 0.03, 0.02, 0.004), None),             # matching: runs
+                       ((8, 8, 0.0, 0.02, 0.004), capi.ERR_ARG),      # minDs
+                       ((8, 8, 0.03, -1.0, 0.004), capi.ERR_ARG),     # maxDs
+                       ((8, 8, 0.03, 0.02, 0.0), capi.ERR_ARG),       # extinction
+                       ((0, 8, 0.03, 0.02, 0.004), capi.ERR_ARG),     # no chains
+                       ((8, 0, 0.03, 0.02, 0.004), capi.ERR_ARG),     # empty chains
+                       ((1 << 40, 2, 0.03, 0.02, 0.004), capi.ERR_ARG)]:  # more than 2^40 proposals
+        if code is None:
+            r = ctx.run_generated_chains(*args, seed=2)
+            assert r["num_paths"] == 64 and r["num_valid"] + r["num_invalid"] == 64
+        else:
+            with pytest.raises(capi.TwistyHipError) as e:
+                ctx.run_generated_chains(*args)
+            assert e.value.code == code, args
+    ctx.close()

@@ -249,6 +249,19 @@ def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22,
         out["cpu_baseline"] = {"value": n * (N - 1) / secs, "unit": "joint-weights/s", "cores": threads,
                                "kind": "port", "sample": f"{n} paths of the same workload, oracle/ FP64 restatement "
                                                          f"on {threads} std::threads, {secs:.1f} s"}
+        tabs_np = np.stack([t.table for t in tabs])
+        meta_np = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+        nc = 1 << 10
+        run_c = lambda k: pyoracle.generate_chains(bc[:12].astype(np.float64), N, k, length, tabs_np, meta_np, ext,
+                                                   float(lo), float(hi), seed=1, threads=threads)["seconds"]
+        secs = run_c(nc)
+        while secs * 2 <= cpu_seconds and nc < chains:
+            nc *= 2
+            secs = run_c(nc)
+        out["chains"]["cpu_baseline"] = {"value": nc * length / secs, "unit": "proposals/s", "cores": threads,
+                                         "kind": "port", "sample": f"{nc} chains x {length} proposals of the same "
+                                                                   f"workload, oracle/ FP64 restatement on {threads} "
+                                                                   f"std::threads, {secs:.1f} s"}
     return out
 
 

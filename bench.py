@@ -21,6 +21,7 @@ import math
 import os
 import statistics
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -210,8 +211,9 @@ def measure_generated(capi, host, cpu_seconds=None, N=66, L=1.65, paths=1 << 22,
     # arclength up by half the nominal ds, 8 cached tables over that range
     lo = np.float32(host.minimum_arclength(bc, N) / N)
     hi = np.float32(0.5 * ds)
-    tabs, _, _ = host.cached_rt_tables(float(lo), float(lo + hi), 8, 1.0, float(np.float32(0.05) ** 2), 0.1, 0.004,
-                                       numStepsInt=20000, numCurvatureSteps=4096)
+    with tempfile.TemporaryDirectory() as cache_root:  # the tables' .cwt cache, not kept
+        tabs, _, _ = host.cached_rt_tables(float(lo), float(lo + hi), 8, 1.0, float(np.float32(0.05) ** 2), 0.1,
+                                           0.004, numStepsInt=20000, numCurvatureSteps=4096, cache_root=cache_root)
     ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
     ctx.set_multi_tables(np.stack([t.table for t in tabs]),
                          np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32), ext, N)

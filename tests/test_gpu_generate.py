@@ -191,7 +191,7 @@ def test_host_path_generation_mc(hip_device, oracle):
     ctx.close()
 
 
-def test_generated_multi_ds_matches_oracle(hip_device, oracle):
+def test_generated_multi_ds_matches_oracle(hip_device, oracle, tmp_path):
     """Multi-ds form against the oracle: per-path ds, nearest cached table
     (CachedMultiArclengthWeightLookupTable from the host), curvature with
     arclength ds N, the table's own absorption."""
@@ -199,7 +199,7 @@ def test_generated_multi_ds_matches_oracle(hip_device, oracle):
     bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
     bc13[9:12] /= np.linalg.norm(bc13[9:12])
     tabs, _, _ = host.cached_rt_tables(0.035, 0.08, 12, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
-                                       numStepsInt=2000, numCurvatureSteps=1024)
+                                       numStepsInt=2000, numCurvatureSteps=1024, cache_root=tmp_path)
     tables = np.stack([t.table for t in tabs])
     meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
     ctx = capi.Context(0)
@@ -286,7 +286,7 @@ def test_host_path_generation_mc_chains(hip_device, tmp_path):
     assert abs(r["log10_total_weight"] - want) < 1e-9 * abs(want)
 
 
-def test_generated_chains_match_oracle(hip_device, oracle):
+def test_generated_chains_match_oracle(hip_device, oracle, tmp_path):
     """The two-table Metropolis-Hastings form (ExperimentBase.cpp:1599-1818)
     against the oracle on the same draws: every proposal by the per-path rule
     (truncated-exponential ds, nearest cached table, loss absorption + scatter
@@ -301,7 +301,7 @@ def test_generated_chains_match_oracle(hip_device, oracle):
     hi = np.float32(0.03)
     tabs, _, _ = host.cached_rt_tables(float(lo) - 0.001, float(lo + hi) + 0.001, 12, 1.0,
                                        float(np.float32(0.08) ** 2), 0.1, 0.004, numStepsInt=2000,
-                                       numCurvatureSteps=1024)
+                                       numCurvatureSteps=1024, cache_root=tmp_path)
     tables = np.stack([t.table for t in tabs])
     meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
     ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
@@ -374,14 +374,14 @@ def test_generated_chains_match_oracle(hip_device, oracle):
     ctx.close()
 
 
-def test_generated_chains_argument_errors(hip_device):
+def test_generated_chains_argument_errors(hip_device, tmp_path):
     """The chains entry point refuses, before any launch: an extinction other
     than the absorption coefficient the cached tables were given, bad ds bounds,
     an empty or oversized chain set, and a run before the tables."""
     N = 34
     bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0, 0, 1), (0, 0, 1), 2.0)
     tabs, _, _ = host.cached_rt_tables(0.025, 0.06, 4, 1.0, float(np.float32(0.08) ** 2), 0.1, 0.004,
-                                       numStepsInt=500, numCurvatureSteps=256)
+                                       numStepsInt=500, numCurvatureSteps=256, cache_root=tmp_path)
     tables = np.stack([t.table for t in tabs])
     meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
     ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))

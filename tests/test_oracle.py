@@ -709,7 +709,7 @@ def test_chain_sampling_weight_closed_form(oracle):
         assert got == pytest.approx(want, rel=1e-6, abs=1e-6)
 
 
-def _chain_case(oracle, n_chains=48, length=40, norm=2.0, seed=3):
+def _chain_case(oracle, cache_root, n_chains=48, length=40, norm=2.0, seed=3):
     from twisty_amd import host
     N = 34
     bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
@@ -718,7 +718,7 @@ def _chain_case(oracle, n_chains=48, length=40, norm=2.0, seed=3):
     hi = np.float32(0.03)
     tabs, _, _ = host.cached_rt_tables(float(lo) - 0.001, float(lo + hi) + 0.001, 8, 1.0,
                                        float(np.float32(0.08) ** 2), 0.1, 0.004, numStepsInt=2000,
-                                       numCurvatureSteps=512)
+                                       numCurvatureSteps=512, cache_root=cache_root)
     tables = np.stack([t.table for t in tabs])
     meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
     ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
@@ -727,7 +727,7 @@ def _chain_case(oracle, n_chains=48, length=40, norm=2.0, seed=3):
     return r, n_chains, length, norm
 
 
-def test_chain_acceptance_rule(oracle):
+def test_chain_acceptance_rule(oracle, tmp_path):
     """The chains against a replay of ExperimentBase.cpp:1730-1784 from the
     oracle's own proposal weights and draws: an invalid proposal is skipped, a
     chain's first valid proposal is taken, a later one when
@@ -735,7 +735,7 @@ def test_chain_acceptance_rule(oracle):
     chain's current weight; the result is the sum over those.  Also: a
     proposal at least as heavy as the current state is always taken, and the
     current state is always one of the chain's own proposals."""
-    r, C, L, norm = _chain_case(oracle)
+    r, C, L, norm = _chain_case(oracle, tmp_path)
     w, v, u, ch = (r[k].reshape(C, L) for k in ("log10w", "valid", "u", "chain"))
     total, nv = -math.inf, 0
     tests = taken = 0

@@ -239,7 +239,9 @@ __device__ __forceinline__ float clamp_curvature(const TableView &t, float k, bo
 {
     const bool lo = k < t.minK;
     const bool hi = k > t.maxK;
-    counted = (lo & !(fabsf(k - t.minK) < 1e-3f)) | (hi & !(fabsf(t.maxK - k) < 1e-3f));
+    const bool farLo = lo & !(fabsf(k - t.minK) < 1e-3f);
+    const bool farHi = hi & !(fabsf(t.maxK - k) < 1e-3f);
+    counted = farLo | farHi;
     moved = lo | hi;
     return lo ? t.minK : (hi ? t.maxK : k);
 }

@@ -407,3 +407,39 @@ def test_generated_chains_argument_errors(hip_device, tmp_path):
                 ctx.run_generated_chains(*args)
             assert e.value.code == code, args
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_generated_chains_split_equals_whole(hip_device, tmp_path):
+    """A chain set run in two parts by chain_offset is the run of the whole
+    set: every proposal weight, chain weight and joint curvature bit for bit
+    (each chain's draws are keyed by its global index), the path counts add
+    up, and the two partial sums combine (log-sum-exp) to the whole run's."""
+    N = 34
+    bc13 = host.boundary13((0, 0, 0), (0, 0, 1), (0.4, 0, 1), (0.6, 0, 0.8), 2.0)
+    bc13[9:12] /= np.linalg.norm(bc13[9:12])
+    lo = np.float32(host.minimum_arclength(bc13, N) / N)
+    hi = np.float32(0.03)
+    tabs, _, _ = host.cached_rt_tables(float(lo) - 0.001, float(lo + hi) + 0.001, 6, 1.0,
+                                       float(np.float32(0.08) ** 2), 0.1, 0.004, numStepsInt=1000,
+                                       numCurvatureSteps=512, cache_root=tmp_path)
+    tables = np.stack([t.table for t in tabs])
+    meta = np.array([[t.ds, t.minK, t.maxK, t.step] for t in tabs], np.float32)
+    ext = float(np.float32(np.float32(0.004) + np.float32(0.1)))
+    C, L, cut = 3000, 24, 1234  # the cut is not a multiple of any block's chain count
+    ctx = capi.Context(0)
+    ctx.set_curve(host.initial_arc(bc13, N), bc13)
+    ctx.set_multi_tables(tables, meta, ext, N)
+    run = lambda c, off: ctx.run_generated_chains(c, L, float(lo), float(hi), ext, norm_log10=1.5, seed=4,
+                                                  chain_offset=off, per_path=True, per_joint=True)
+    whole, a, b = run(C, 0), run(cut, 0), run(C - cut, cut)
+    for key in ("log10w", "chain", "curv"):
+        got = np.concatenate([a[key], b[key]])
+        assert np.array_equal(got, whole[key], equal_nan=True), key
+    assert a["num_paths"] + b["num_paths"] == whole["num_paths"] == C * L
+    assert a["num_valid"] + b["num_valid"] == whole["num_valid"]
+    assert a["num_invalid"] + b["num_invalid"] == whole["num_invalid"]
+    combined = np.logaddexp(a["log10_sum_w"] * math.log(10), b["log10_sum_w"] * math.log(10)) / math.log(10)
+    print(f"[chains split] whole {whole['log10_sum_w']:.12f} parts {combined:.12f} valid {whole['num_valid']}",
+          flush=True)
+    assert abs(combined - whole["log10_sum_w"]) <= 1e-9 * abs(whole["log10_sum_w"]) + 1e-9

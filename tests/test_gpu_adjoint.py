@@ -280,7 +280,7 @@ def test_host_experiment_gradients_match_autograd(hip_device, oracle):
                                 "endDir": list(edir), "arclength": L}}}
         g = host.experiment_gradients(cfg, table_bins=case.bins)
         d_sigma = g["d_eps"] * 2.0 * float(np.float32(case.sigma))
-        print(f"[host gradients e={edir}] E {g['log10_estimate']:.9f} vs {float(E):.9f}; dsigma {d_sigma:.6e} vs "
+        print(f"[host gradients e={edir}] E {g['log10_estimate']:.9f} vs {float(E.detach()):.9f}; dsigma {d_sigma:.6e} vs "
               f"{float(sigma.grad):.6e}; dtheta {g['d_theta']:.6e} vs {float(ang.grad[0]):.6e}; dphi {g['d_phi']:.6e} "
               f"vs {float(ang.grad[1]):.6e}", flush=True)
         print(f"  exit-angle errors: roulette std ({g['d_theta_roulette_std']:.3e}, {g['d_phi_roulette_std']:.3e}), "
@@ -288,7 +288,7 @@ def test_host_experiment_gradients_match_autograd(hip_device, oracle):
               flush=True)
         assert 0 < g["paths_swept"] < paths
         assert g["d_theta_mc_std"] > 0.0 and g["d_theta_roulette_std"] <= g["d_theta_mc_std"]
-        assert abs(g["log10_estimate"] - float(E)) < 1e-9 * abs(float(E))
+        assert abs(g["log10_estimate"] - float(E.detach())) < 1e-9 * abs(float(E.detach()))
         assert d_sigma == pytest.approx(float(sigma.grad), rel=1e-4)
         scale = max(abs(float(ang.grad[0])), abs(float(ang.grad[1])))
         assert abs(g["d_theta"] - float(ang.grad[0])) < 1e-4 * scale

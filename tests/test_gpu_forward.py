@@ -58,9 +58,10 @@ def check_against_oracle(case, oracle, gpu, ref, ref32, table, tag):
     lg, bg = joint_log10w(gpu["curv"], table, case.absorption)
     lr, br = joint_log10w(ref["curv"], table, case.absorption)
     straddle = bg != br
-    jump = np.where(straddle, np.abs(lg - lr), 0.0).sum(axis=1)
+    with np.errstate(invalid="ignore"):  # invalid (NaN) and zero-weight (-inf) paths are selected out below
+        jump = np.where(straddle, np.abs(lg - lr), 0.0).sum(axis=1)
+        err = np.abs(g - r)
     clean = valid & ~straddle.any(axis=1)
-    err = np.abs(g - r)
     scale = np.maximum(np.abs(r), 1e-30)
     rel_clean = (err[clean] / scale[clean]).max() if clean.any() else 0.0
     excess = (err[valid] - REL_PATH * scale[valid] - jump[valid]).max()

@@ -75,6 +75,9 @@ def build(asm=False, verbose=True):
     calib = os.path.join(LIBDIR, "twisty_copy_calib")
     run([cc, *HIPCC_FLAGS, "-o", calib + ".tmp", os.path.join(CSRC, "copy_calib.hip")])
     os.replace(calib + ".tmp", calib)
+    rate = os.path.join(LIBDIR, "twisty_valu_rate")
+    run([cc, *HIPCC_FLAGS, "-o", rate + ".tmp", os.path.join(CSRC, "valu_rate.hip")])
+    os.replace(rate + ".tmp", rate)
     return out
 
 

@@ -289,6 +289,27 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
       const uint32_t (&scrB)[(25 + G - 1) / G], uint32_t *drawbuf, uint64_t seed, uint32_t stream, uint64_t path,
       uint32_t K, uint32_t N, uint32_t maxDiff)
 {
+    if constexpr (LEAN && G > 1 && G <= 8) {
+        // Batch kb + G is drawn while batch kb is applied: the draws do not
+        // depend on the state, so their Philox rounds can fill the waits of
+        // the batch's first perturbation.  The records are applied in the
+        // same order; draws past the last perturbation are never read.
+        const int firstB = (int)(((lane & 63u) - j) * 4u);  // the group's first lane, bytes
+        DrawRec mine = pack_draw(make_draw(seed, stream, path, j, N, maxDiff));
+        for (uint32_t kb = 0; kb < K; kb += G) {
+            const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
+            DrawRec recs[G];
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) recs[kk] = bperm_rec(firstB + 4 * kk, mine);
+            mine = pack_draw(make_draw(seed, stream, path, kb + G + j, N, maxDiff));
+#pragma unroll
+            for (int kk = 0; kk < G; ++kk) {
+                if ((uint32_t)kk >= kcount) break;
+                perturb_one<G, BLOCK, true>(laneB, j, scrB, recs[kk]);
+            }
+        }
+        return;
+    }
     for (uint32_t kb = 0; kb < K; kb += G) {
         const uint32_t kcount = (K - kb < (uint32_t)G) ? (K - kb) : (uint32_t)G;
         if constexpr (G == 1) {
@@ -298,16 +319,7 @@ __device__ __forceinline__ void perturb_path(uint32_t laneB, uint32_t lane, uint
             if (kb + j < K) mine = pack_draw(make_draw(seed, stream, path, kb + j, N, maxDiff));
             // byte index of the group's first lane within the wave
             const int firstB = (int)(((lane & 63u) - j) * 4u);
-            if constexpr (G <= 8) {
-                DrawRec recs[G];
-#pragma unroll
-                for (int kk = 0; kk < G; ++kk) recs[kk] = bperm_rec(firstB + 4 * kk, mine);
-#pragma unroll
-                for (int kk = 0; kk < G; ++kk) {
-                    if ((uint32_t)kk >= kcount) break;
-                    perturb_one<G, BLOCK, true>(laneB, j, scrB, recs[kk]);
-                }
-            } else if constexpr (G >= 25) {
+            if constexpr (G >= 25) {  // G <= 8 takes the loop above
                 perturb_batch_multi<G, BLOCK, true, kMultiPerturb, false>(laneB, j, scrB, kcount,
                       [&](uint32_t t) { return bperm_rec(firstB + 4 * (int)t, mine); });
             } else {

@@ -453,8 +453,9 @@ def main(argv=None):
                                        "sweep over every path"}
     if not a.no_c4:
         out["c4"] = measure_c4(capi, host)
-        progress(f"C4: forward {out['c4']['forward_ms']:.3f} ms, exact full gradient "
-                 f"{out['c4']['full']['fwd_adjoint_ms']:.3f} ms")
+        progress(f"C4: forward {out['c4']['forward_ms']:.3f} ms, full gradient "
+                 f"{out['c4']['full']['fwd_adjoint_ms']:.3f} ms (roulette sweep), "
+                 f"{out['c4']['full_exact']['fwd_adjoint_ms']:.3f} ms (exact sweep)")
     if not a.no_batched:
         sched = tuple(int(v) for v in a.batched_schedule.split(",")) if a.batched_schedule else None
         out["batched"] = measure_batched(ctx, capi, autotune=not a.no_autotune, schedule=sched)
